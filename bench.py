@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched integration hot path on one MI355X.
+
+Workload eiger9m_batch1d: 256 distinct uint32 frames of 3269 x 3110 pixels
+(75 um, d = 0.25 m, lambda = 0.4 A, beam at the detector centre), 2000 bins of
+pixel splitting, one device-built table shared by every frame. A step is one
+batch1d pass over the 256 device-resident frames: the fused CSR x frame-tile
+gather, tile picked by the autotuner, replayed from a hipGraph.
+
+Prints one JSON line (value = frames/s of the whole job). Extra keys report
+the plain per-frame GPU path, the table build, the roofline of the dominant
+kernel and the CPU baseline (the restated reference CPU batch1d path timed on
+a bounded sample on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "eiger9m_batch1d"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_frame(npix, nnz, bins, tile, px_bytes=4, outputs=1):
+    # frame once + table (u32 col, f32 weight, u32 row_ptr, f64 den) per tile
+    # + f64 outputs.
+    return px_bytes * npix + (8 * nnz + 4 * (bins + 1) + 8 * bins) / tile + 8 * bins * outputs
+
+
+def cpu_baseline(frames_dev, poni, w, seconds=12.0):
+    """Restated reference CPU batch1d (CSC, 4-frame tiles, all host cores)."""
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    lut = O.build_lut1d(O.Poni(**poni), *w["shape"], w["n_radial"])
+    build_s = time.time() - t0
+    norm = O.normalization(O.Poni(**poni), *w["shape"], w["polarization"])
+    csc = O.CscBaseline(lut, w["shape"][0] * w["shape"][1], None, norm)
+    nf = 4 * threads
+    host = frames_dev[:nf].cpu().numpy().view(np.uint32).reshape(nf, -1)
+    done, t0 = 0, time.time()
+    while True:
+        csc.batch_u32(host, threads)
+        done += nf
+        el = time.time() - t0
+        if el >= seconds:
+            break
+    return {"value": done / el, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{done} frames ({nf} distinct frames x {done // nf} passes) of the "
+                      f"same workload, {el:.1f} s wall",
+            "lut_build_ms": build_s * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="f32_compensated")
+    ap.add_argument("--tile", type=int, default=0, help="frames per tile (0 = autotune)")
+    ap.add_argument("--frames", type=int, default=256)
+    ap.add_argument("--errors", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from dioptrin_amd import GpuPlan, device_info, synth
+
+    w = synth.WORKLOADS[WORKLOAD]
+    poni = synth.workload_poni(WORKLOAD)
+    npix = w["shape"][0] * w["shape"][1]
+    nfr = args.frames
+    log(f"[bench] device {device_info(0)['name']} {device_info(0)['arch']}")
+    t0 = time.time()
+    frames = synth.frames_device(WORKLOAD, nfr, seed=2026)
+    torch.cuda.synchronize()
+    log(f"[bench] generated {nfr} frames in {time.time() - t0:.1f} s")
+
+    t0 = time.time()
+    plan = GpuPlan(poni, w["shape"], n_radial=w["n_radial"], precision=args.precision)
+    torch.cuda.synchronize()
+    build_wall = (time.time() - t0) * 1e3
+    info = plan.info
+    log(f"[bench] table: nnz {info['nnz']} segments {info['n_segments']} build "
+        f"{info['build_ms']:.1f} ms uncertain {info['uncertain_angles']} singular "
+        f"{info['singular_pixels']}")
+    tile = args.tile or plan.autotune("uint32", 64, errors=args.errors)
+    plan.set_tile(tile)
+    log(f"[bench] frames per tile {tile}")
+
+    bins = plan.nbins
+    out_i = torch.empty((nfr, bins), dtype=torch.float64, device="cuda")
+    out_s = torch.empty((nfr, bins), dtype=torch.float64, device="cuda") if args.errors else None
+
+    if args.no_graph:
+        def step():
+            plan.integrate_device(frames, out_i, out_s, errors=args.errors)
+    else:
+        graph = plan.capture_graph(frames, out_i, out_s, errors=args.errors)
+
+        def step():
+            graph.launch()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_per_step = el * 1e3 / args.steps
+    fps = nfr * args.steps / el
+    log(f"[bench] batched: {ms_per_step:.3f} ms/step, {fps:.0f} frames/s")
+
+    # Plain per-frame path: one call and one synchronisation per frame.
+    plan1 = plan
+    plan1.set_tile(1)
+    n1 = min(nfr, 64)
+    for k in range(4):
+        plan1.integrate_device(frames[k:k + 1], out_i[k:k + 1], None if out_s is None else
+                               out_s[k:k + 1], errors=args.errors)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n1):
+        plan1.integrate_device(frames[k:k + 1], out_i[k:k + 1], None if out_s is None else
+                               out_s[k:k + 1], errors=args.errors)
+        torch.cuda.synchronize()
+    per_frame_fps = n1 / (time.perf_counter() - t0)
+    plan.set_tile(tile)
+    log(f"[bench] per-frame: {per_frame_fps:.0f} frames/s")
+
+    # Dominant kernel: the segment gather of one tile, timed with HIP events
+    # on the plan's stream.
+    seg_ms = plan.time_gather(frames[:tile], errors=args.errors, reps=20)
+    per_launch = algorithmic_bytes_per_frame(npix, info["nnz_masked"], bins, tile,
+                                             outputs=1 + int(args.errors)) * tile
+    achieved = per_launch / (seg_ms * 1e-3) / 1e9
+    log(f"[bench] gather kernel {seg_ms:.4f} ms per {tile}-frame launch, {achieved:.0f} GB/s")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_eiger9m_batch1d.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("tile") == tile and tr.get("precision") == args.precision:
+            traffic = tr.get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        cpu = cpu_baseline(frames, poni, w)
+        log(f"[bench] CPU baseline {cpu['value']:.1f} frames/s on {cpu['cores']} threads")
+
+    bytes_frame = algorithmic_bytes_per_frame(npix, info["nnz_masked"], bins, tile,
+                                              outputs=1 + int(args.errors))
+    line = {
+        "metric": "frames/s + HBM GB/s (% peak), batch1d 9M-px frames; p50/p99 serve latency",
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32_compensated": "f32x2 (u32 frames, f32 weights)",
+                  "f32_wide": "f64 (u32 frames, f32 weights)", "f32": "f32",
+                  "f64": "f64"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"{WORKLOAD}: {nfr} x {w['shape'][0]}x{w['shape'][1]} uint32, "
+                               f"{bins} bins pixel_split, device-resident frames",
+                   "frames_per_step": nfr, "frames_per_tile": tile,
+                   "precision": args.precision, "errors": bool(args.errors),
+                   "graph": not args.no_graph, "table_entries": info["nnz_masked"]},
+        "hbm_gbs_algorithmic": round(bytes_frame * fps / 1e9, 1),
+        "hbm_pct_peak": round(100 * bytes_frame * fps / 1e9 / HBM_PEAK_GBS, 1),
+        "per_frame_fps": round(per_frame_fps, 1),
+        "batched_vs_per_frame": round(fps / per_frame_fps, 2),
+        "lut_build_ms_gpu": round(info["build_ms"], 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "k_seg (segment gather, one tile)",
+                     "kernel_ms": round(seg_ms, 5), "bytes_per_launch": int(per_launch)},
+        "cpu_baseline": None if cpu is None else {k: (round(v, 2) if isinstance(v, float) else v)
+                                                  for k, v in cpu.items()},
+        "serve": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,495 @@
+// C ABI of the MI355X integration path: contexts, plans (device-built
+// tables), integration entry points, autotuner, graph replay.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "builder.hpp"
+#include "gather.hpp"
+
+struct dio_ctx {
+  int device = 0;
+  hipDeviceProp_t prop{};
+  int refs = 0;
+};
+
+struct dio_plan {
+  dio_ctx* ctx = nullptr;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  dio_plan_desc desc{};
+  dio::Geo geo{};
+  dio::Built b;
+  dio::GatherTable gt;
+  dio::StdTable st;
+  dio::GatherScratch sc;
+  uint32_t tile = 8;
+  double build_ms = 0;
+  std::mutex mu;
+  dio::DevBuf<uint8_t> stage;
+  dio::DevBuf<double> out_i, out_s, out_d;
+  ~dio_plan() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct dio_graph {
+  dio_plan* plan = nullptr;
+  dio::GatherScratch sc;  // owned: the plan's scratch may be reallocated later
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~dio_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+std::mutex g_ctx_mu;
+std::map<int, std::unique_ptr<dio_ctx>> g_ctx;
+
+template <class Fn>
+int guarded(Fn&& fn) {
+  try {
+    fn();
+    g_last_error.clear();
+    return DIO_OK;
+  } catch (const dio::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host out of memory";
+    return DIO_ERR_DEVICE;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return DIO_ERR_DEVICE;
+  }
+}
+
+void require(bool ok, int code, const char* msg) {
+  if (!ok) dio::fail(code, msg);
+}
+
+size_t dtype_bytes(int dtype) {
+  switch (dtype) {
+    case DIO_U16: return 2;
+    case DIO_U32: case DIO_I32: case DIO_F32T: return 4;
+    case DIO_F64T: return 8;
+    default: dio::fail(DIO_ERR_INVALID, "unknown frame dtype");
+  }
+}
+
+// Configuration checks before any device work (cuda.rs:1001-1057).
+void check_desc(const dio_plan_desc& d) {
+  require(d.dim1 > 0 && d.dim2 > 0, DIO_ERR_INVALID, "image shape has a zero side");
+  require((uint64_t)d.dim1 * d.dim2 <= 0xffffffffull, DIO_ERR_INVALID,
+          "image exceeds 2^32 pixels");
+  require(d.ndim == 1 || d.ndim == 2, DIO_ERR_INVALID, "ndim must be 1 or 2");
+  require(d.n_radial > 0 && d.n_radial <= 0x7fffffffu, DIO_ERR_INVALID,
+          "n_points must be between 1 and 2^31-1");
+  if (d.ndim == 2) {
+    require(d.n_azimuthal > 0 && (uint64_t)d.n_radial * d.n_azimuthal <= 0x7fffffffull,
+            DIO_ERR_INVALID, "2D bin count must be in 1..2^31-1");
+  }
+  require(d.method >= DIO_COUNTING && d.method <= DIO_SUPERSAMPLED, DIO_ERR_INVALID,
+          "unknown method");
+  require(d.method != DIO_SUPERSAMPLED || d.n_ss > 0, DIO_ERR_INVALID, "n_ss must be positive");
+  require(d.unit >= DIO_UNIT_TTH_RAD && d.unit <= DIO_UNIT_Q_NM, DIO_ERR_INVALID, "unknown unit");
+  require(d.precision >= DIO_F32 && d.precision <= DIO_F32_WIDE, DIO_ERR_INVALID,
+          "unknown precision");
+  if (d.has_radial_range)
+    require(std::isfinite(d.radial_min) && std::isfinite(d.radial_max) &&
+                d.radial_min < d.radial_max,
+            DIO_ERR_INVALID, "radial_range must be finite with min < max");
+  if (d.has_polarization)
+    require(d.polarization_factor >= -1.0 && d.polarization_factor <= 1.0, DIO_ERR_INVALID,
+            "polarization factor must lie in [-1, 1]");
+  require(d.unit < DIO_UNIT_Q_A, DIO_ERR_UNSUPPORTED,
+          "q units are not available for device-built tables");
+}
+
+void ensure_out(dio::DevBuf<double>& b, size_t n) {
+  if (b.n < n) b.alloc(n);
+}
+
+// Enqueue the whole integration of device frames on `s`.
+void run_device(dio_plan* p, dio::GatherScratch& sc, const void* frames, int dtype,
+                uint64_t n_frames, uint32_t flags, double* I, double* S, double* SD,
+                hipStream_t s) {
+  bool var = flags & DIO_WITH_ERRORS;
+  if (flags & DIO_WITH_AZIMUTHAL_STD) {
+    require(p->desc.ndim == 1, DIO_ERR_UNSUPPORTED, "azimuthal std is defined for 1D plans");
+    require(SD != nullptr, DIO_ERR_INVALID, "azimuthal std output missing");
+    dio::azimuthal_std_frames(p->st, frames, dtype, n_frames, var, I, S, SD, s);
+    return;
+  }
+  require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
+  dio::gather_frames(p->gt, sc, frames, dtype, n_frames, var, p->desc.precision, p->tile, I, S,
+                     s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dio_last_error(void) { return g_last_error.c_str(); }
+const char* dio_version(void) { return "dioptrin_hip 0.1.0 (gfx950)"; }
+
+int dio_open(int device, dio_ctx** out) {
+  return guarded([&] {
+    require(out != nullptr, DIO_ERR_INVALID, "null output handle");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+      dio::fail(DIO_ERR_UNAVAILABLE, "no HIP device available");
+    require(device >= 0 && device < count, DIO_ERR_UNAVAILABLE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    auto& slot = g_ctx[device];
+    if (!slot) {
+      auto c = std::make_unique<dio_ctx>();
+      c->device = device;
+      DIO_HIP(hipSetDevice(device));
+      DIO_HIP(hipGetDeviceProperties(&c->prop, device));
+      slot = std::move(c);
+    }
+    slot->refs++;
+    *out = slot.get();
+  });
+}
+
+int dio_close(dio_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    ctx->refs = std::max(0, ctx->refs - 1);
+  });
+}
+
+int dio_device_info_get(dio_ctx* ctx, dio_device_info* out) {
+  return guarded([&] {
+    require(ctx && out, DIO_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    std::strncpy(out->name, ctx->prop.name, sizeof(out->name) - 1);
+    std::strncpy(out->arch, ctx->prop.gcnArchName, sizeof(out->arch) - 1);
+    out->compute_units = ctx->prop.multiProcessorCount;
+    out->xcds = 8;
+    out->total_memory = ctx->prop.totalGlobalMem;
+    int v = 0;
+    (void)hipRuntimeGetVersion(&v);
+    out->runtime_version = v;
+  });
+}
+
+int dio_plan_create(dio_ctx* ctx, const dio_poni* poni, const dio_plan_desc* desc,
+                    dio_plan** out) {
+  return guarded([&] {
+    require(poni && desc && out, DIO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    check_desc(*desc);
+    require(ctx != nullptr, DIO_ERR_INVALID, "null context");
+    auto t0 = std::chrono::steady_clock::now();
+    DIO_HIP(hipSetDevice(ctx->device));
+    auto p = std::make_unique<dio_plan>();
+    p->ctx = ctx;
+    p->device = ctx->device;
+    p->desc = *desc;
+    p->desc.mask = nullptr;
+    DIO_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->geo = dio::make_geo(*poni, *desc);
+    dio::build_tables(p->geo, *desc, p->stream, p->b);
+    const dio::Table& t = p->b.exec_table();
+    p->gt.col = t.col.p;
+    p->gt.w32 = t.w32.p;
+    p->gt.w64 = t.w64.p;
+    p->gt.den = p->b.den.p;
+    p->gt.nbins = t.nbins;
+    p->gt.npix = (uint64_t)desc->dim1 * desc->dim2;
+    p->gt.nnz = t.nnz;
+    dio::make_segments(p->gt, t.row_ptr.p, p->stream);
+    p->st.row_ptr = p->b.full.row_ptr.p;
+    p->st.col = p->b.full.col.p;
+    p->st.w32 = p->b.full.w32.p;
+    p->st.mask = p->b.mask.p;
+    p->st.norm = p->b.norm.p;
+    p->st.nbins = p->b.full.nbins;
+    p->st.npix = p->gt.npix;
+    p->build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = p.release();
+  });
+}
+
+int dio_plan_destroy(dio_plan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    (void)hipSetDevice(plan->device);
+    (void)hipStreamSynchronize(plan->stream);
+    delete plan;
+  });
+}
+
+int dio_plan_info_get(dio_plan* p, dio_plan_info* out) {
+  return guarded([&] {
+    require(p && out, DIO_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->nbins = p->b.full.nbins;
+    out->n_radial = p->b.n_radial;
+    out->n_azimuthal = p->b.n_azimuthal;
+    out->nnz = p->b.full.nnz;
+    out->nnz_masked = p->b.exec_table().nnz;
+    out->weighted = p->b.weighted ? 1 : 0;
+    out->n_segments = p->gt.nseg;
+    out->uncertain_angles = p->b.uncertain;
+    out->singular_pixels = p->b.singular;
+    out->build_ms = p->build_ms;
+    out->hist_min = p->b.hist_min;
+    out->hist_step = p->b.hist_step;
+    auto tb = [](const dio::Table& t) {
+      return t.row_ptr.bytes() + t.col.bytes() + t.w64.bytes() + t.w32.bytes();
+    };
+    out->device_bytes = tb(p->b.full) + (p->b.has_exec ? tb(p->b.exec) : 0) +
+                        p->b.norm.bytes() + p->b.den.bytes() + p->b.mask.bytes() +
+                        p->gt.seg_begin.bytes() * 2 + p->gt.row_seg.bytes() + p->sc.part.bytes();
+  });
+}
+
+int dio_plan_radial(dio_plan* p, double* out) {
+  return guarded([&] {
+    require(p && out, DIO_ERR_INVALID, "null argument");
+    // histogram.rs:199-203: min + (i + 0.5) * step.
+    for (uint32_t i = 0; i < p->b.n_radial; i++)
+      out[i] = p->b.hist_min + ((double)i + 0.5) * p->b.hist_step;
+  });
+}
+
+int dio_plan_azimuthal(dio_plan* p, double* out) {
+  return guarded([&] {
+    require(p && out, DIO_ERR_INVALID, "null argument");
+    require(p->desc.ndim == 2, DIO_ERR_INVALID, "not a 2D plan");
+    for (uint32_t i = 0; i < p->b.n_azimuthal; i++)
+      out[i] = p->b.az_min + ((double)i + 0.5) * p->b.az_step;
+  });
+}
+
+int dio_plan_export_csr(dio_plan* p, int masked, uint32_t* row_ptr, uint32_t* col_idx,
+                        double* w64, float* w32, uint64_t capacity) {
+  return guarded([&] {
+    require(p != nullptr, DIO_ERR_INVALID, "null plan");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    const dio::Table& t = masked ? p->b.exec_table() : p->b.full;
+    if (row_ptr)
+      DIO_HIP(hipMemcpy(row_ptr, t.row_ptr.p, (t.nbins + 1) * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost));
+    if (col_idx || w64 || w32) require(capacity >= t.nnz, DIO_ERR_INVALID, "capacity too small");
+    if (t.nnz) {
+      if (col_idx)
+        DIO_HIP(hipMemcpy(col_idx, t.col.p, t.nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      if (w64) DIO_HIP(hipMemcpy(w64, t.w64.p, t.nnz * sizeof(double), hipMemcpyDeviceToHost));
+      if (w32) DIO_HIP(hipMemcpy(w32, t.w32.p, t.nnz * sizeof(float), hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int dio_plan_denominators(dio_plan* p, double* out) {
+  return guarded([&] {
+    require(p && out, DIO_ERR_INVALID, "null argument");
+    DIO_HIP(hipSetDevice(p->device));
+    DIO_HIP(hipMemcpy(out, p->b.den.p, p->b.den.bytes(), hipMemcpyDeviceToHost));
+  });
+}
+
+int dio_plan_normalization(dio_plan* p, double* out) {
+  return guarded([&] {
+    require(p && out, DIO_ERR_INVALID, "null argument");
+    DIO_HIP(hipSetDevice(p->device));
+    DIO_HIP(hipMemcpy(out, p->b.norm.p, p->b.norm.bytes(), hipMemcpyDeviceToHost));
+  });
+}
+
+int dio_integrate(dio_plan* p, const void* frames, int dtype, uint64_t n_frames, uint32_t flags,
+                  double* intensity, double* sigma, double* std_out) {
+  return guarded([&] {
+    require(p != nullptr, DIO_ERR_INVALID, "null plan");
+    if (n_frames == 0) return;
+    require(frames && intensity, DIO_ERR_INVALID, "null frames or output");
+    require(!(flags & DIO_WITH_ERRORS) || sigma, DIO_ERR_INVALID, "sigma output missing");
+    require(!(flags & DIO_WITH_AZIMUTHAL_STD) || std_out, DIO_ERR_INVALID,
+            "azimuthal std output missing");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    size_t esz = dtype_bytes(dtype);
+    uint64_t fbytes = p->gt.npix * esz;
+    uint32_t nb = p->gt.nbins;
+    // Chunks of at most 1 GiB of frames (and at least one frame).
+    uint64_t chunk = std::max<uint64_t>(1, (1ull << 30) / fbytes);
+    chunk = std::min<uint64_t>(chunk, n_frames);
+    if (p->stage.n < chunk * fbytes) p->stage.alloc(chunk * fbytes);
+    ensure_out(p->out_i, chunk * nb);
+    ensure_out(p->out_s, chunk * nb);
+    ensure_out(p->out_d, chunk * nb);
+    const char* src = static_cast<const char*>(frames);
+    for (uint64_t f0 = 0; f0 < n_frames; f0 += chunk) {
+      uint64_t nf = std::min<uint64_t>(chunk, n_frames - f0);
+      DIO_HIP(hipMemcpyAsync(p->stage.p, src + f0 * fbytes, nf * fbytes, hipMemcpyHostToDevice,
+                             p->stream));
+      run_device(p, p->sc, p->stage.p, dtype, nf, flags, p->out_i.p, p->out_s.p, p->out_d.p, p->stream);
+      DIO_HIP(hipMemcpyAsync(intensity + f0 * nb, p->out_i.p, nf * nb * 8, hipMemcpyDeviceToHost,
+                             p->stream));
+      if (flags & DIO_WITH_ERRORS)
+        DIO_HIP(hipMemcpyAsync(sigma + f0 * nb, p->out_s.p, nf * nb * 8, hipMemcpyDeviceToHost,
+                               p->stream));
+      if (flags & DIO_WITH_AZIMUTHAL_STD)
+        DIO_HIP(hipMemcpyAsync(std_out + f0 * nb, p->out_d.p, nf * nb * 8,
+                               hipMemcpyDeviceToHost, p->stream));
+      DIO_HIP(hipStreamSynchronize(p->stream));
+    }
+  });
+}
+
+int dio_integrate_device(dio_plan* p, const void* d_frames, int dtype, uint64_t n_frames,
+                         uint32_t flags, double* d_intensity, double* d_sigma, double* d_std,
+                         void* stream) {
+  return guarded([&] {
+    require(p != nullptr, DIO_ERR_INVALID, "null plan");
+    if (n_frames == 0) return;
+    require(d_frames && d_intensity, DIO_ERR_INVALID, "null frames or output");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    run_device(p, p->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std, s);
+  });
+}
+
+int dio_plan_set_tile(dio_plan* p, uint32_t tile) {
+  return guarded([&] {
+    require(p != nullptr, DIO_ERR_INVALID, "null plan");
+    require(tile >= 1 && tile <= dio::kMaxTile && (tile & (tile - 1)) == 0, DIO_ERR_INVALID,
+            "frames per tile must be a power of two in 1..32");
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->tile = tile;
+  });
+}
+
+int dio_plan_autotune(dio_plan* p, int dtype, uint64_t n_frames_hint, uint32_t flags,
+                      uint32_t* frames_per_tile) {
+  return guarded([&] {
+    require(p != nullptr, DIO_ERR_INVALID, "null plan");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    size_t esz = dtype_bytes(dtype);
+    bool var = flags & DIO_WITH_ERRORS;
+    // Sweep on 64 zero frames (or fewer when the hint is smaller): timing of
+    // a gather does not depend on pixel values.
+    uint64_t nf = std::max<uint64_t>(1, std::min<uint64_t>(64, n_frames_hint ? n_frames_hint : 64));
+    dio::DevBuf<uint8_t> fr(nf * p->gt.npix * esz);
+    DIO_HIP(hipMemsetAsync(fr.p, 0, fr.bytes(), p->stream));
+    dio::DevBuf<double> oi(nf * p->gt.nbins), os(var ? nf * p->gt.nbins : 1);
+    hipEvent_t e0, e1;
+    DIO_HIP(hipEventCreate(&e0));
+    DIO_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    uint32_t best_tile = p->tile;
+    for (uint32_t tile : {1u, 2u, 4u, 8u, 16u, 32u}) {
+      if (tile > 1 && tile > nf * 2) break;
+      dio::gather_frames(p->gt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, oi.p, os.p,
+                         p->stream);  // warm-up
+      DIO_HIP(hipEventRecord(e0, p->stream));
+      for (int rep = 0; rep < 3; rep++)
+        dio::gather_frames(p->gt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, oi.p,
+                           os.p, p->stream);
+      DIO_HIP(hipEventRecord(e1, p->stream));
+      DIO_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      DIO_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_tile = tile;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->tile = best_tile;
+    if (frames_per_tile) *frames_per_tile = best_tile;
+  });
+}
+
+int dio_graph_create(dio_plan* p, const void* d_frames, int dtype, uint64_t n_frames,
+                     uint32_t flags, double* d_intensity, double* d_sigma, double* d_std,
+                     dio_graph** out) {
+  return guarded([&] {
+    require(p && out && d_frames && d_intensity, DIO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    // Scratch must exist before capture: allocation is not capturable.
+    auto g = std::make_unique<dio_graph>();
+    g->plan = p;
+    g->sc.ensure(p->gt, p->tile, flags & DIO_WITH_ERRORS);
+    DIO_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      run_device(p, g->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std,
+                 p->stream);
+    } catch (...) {
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(p->stream, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    DIO_HIP(hipStreamEndCapture(p->stream, &g->graph));
+    DIO_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+  });
+}
+
+int dio_graph_launch(dio_graph* g, void* stream) {
+  return guarded([&] {
+    require(g != nullptr, DIO_ERR_INVALID, "null graph");
+    DIO_HIP(hipSetDevice(g->plan->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->plan->stream;
+    DIO_HIP(hipGraphLaunch(g->exec, s));
+  });
+}
+
+int dio_graph_destroy(dio_graph* g) {
+  return guarded([&] { delete g; });
+}
+
+int dio_plan_time_gather(dio_plan* p, const void* d_frames, int dtype, uint64_t n_frames,
+                         uint32_t flags, double* d_intensity, double* d_sigma, int reps,
+                         double* avg_ms) {
+  return guarded([&] {
+    require(p && d_frames && avg_ms && reps > 0, DIO_ERR_INVALID, "null argument");
+    (void)d_intensity;
+    (void)d_sigma;
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    bool var = flags & DIO_WITH_ERRORS;
+    uint32_t nf = (uint32_t)std::min<uint64_t>(n_frames, p->tile);
+    hipEvent_t e0, e1;
+    DIO_HIP(hipEventCreate(&e0));
+    DIO_HIP(hipEventCreate(&e1));
+    dio::gather_segments_only(p->gt, p->sc, d_frames, dtype, nf, var, p->desc.precision,
+                              p->stream);
+    DIO_HIP(hipEventRecord(e0, p->stream));
+    for (int r = 0; r < reps; r++)
+      dio::gather_segments_only(p->gt, p->sc, d_frames, dtype, nf, var, p->desc.precision,
+                                p->stream);
+    DIO_HIP(hipEventRecord(e1, p->stream));
+    DIO_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    DIO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+}  // extern "C"

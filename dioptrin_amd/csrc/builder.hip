@@ -1,0 +1,868 @@
+// Device lookup-table builder (gfx950). Compiled with -ffp-contract=off.
+//
+// Pipeline for a 1D table (reference: src/lut/pixel_split.rs:190-455,
+// src/lut/counting_csr.rs:54-134, src/lut/pyfai_1d.rs:188-252,
+// src/lut/prepared.rs:163-262):
+//   1. corner-grid and pixel-centre angles with the certified atan2; the few
+//      uncertain angles are recomputed with the host libm and patched in;
+//   2. data-derived radial range by device min/max reduction;
+//   3. per-pixel entry counts (pixel splitting: Sutherland-Hodgman clip of
+//      the pixel quadrilateral against each touched bin); beam pixels are
+//      set aside and integrated by 128x128x4 detector-plane sampling;
+//   4. exclusive scan of the counts and a pixel-major fill of
+//      (bin, pixel, FP64 weight);
+//   5. stable radix sort by bin: rows list pixels in ascending order, the
+//      order of the reference's builders; exact row pointers from a bin
+//      histogram and a scan;
+//   6. mask compaction (order kept), per-pixel normalization and strict
+//      sequential FP64 denominators per row.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "builder.hpp"
+
+namespace dio {
+
+namespace {
+
+constexpr unsigned kThreads = 256;
+constexpr int kClipCap = 20;  // 4 stages on a quadrilateral: <= 19 vertices
+constexpr unsigned kSamplesPerPixel = 128 * 128 * 4;
+constexpr unsigned kMaxSingular = 4096;
+
+// ------------------------------------------------------------ angle grids
+
+__global__ void k_grid_angles(Geo g, uint32_t rows, uint32_t cols, double off, double* tth,
+                              double* chi, uint32_t* flist, uint32_t* fcount) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t n = (uint64_t)rows * cols;
+  if (i >= n) return;
+  uint32_t r = (uint32_t)(i / cols), c = (uint32_t)(i % cols);
+  Angles a = angles_cert(g, (double)r + off, (double)c + off);
+  tth[i] = a.tth;
+  chi[i] = a.chi;
+  if (a.uncertain) flist[atomicAdd(fcount, 1u)] = (uint32_t)i;
+}
+
+__global__ void k_patch(const uint32_t* idx, const double* tv, const double* cv, uint32_t n,
+                        double* tth, double* chi) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  tth[idx[i]] = tv[i];
+  chi[idx[i]] = cv[i];
+}
+
+__global__ void k_to_unit(Geo g, const double* tth, double* out, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = tth_to_unit(g, tth[i]);
+}
+
+// Block-level min/max partials (src/math/mod.rs:8-32 semantics for finite
+// data: the order of a min/max reduction does not change its value).
+__global__ void k_minmax(const double* v, uint64_t n, double* pmin, double* pmax) {
+  __shared__ double smin[kThreads], smax[kThreads];
+  double lo = INFINITY, hi = -INFINITY;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    double x = v[i];
+    lo = x < lo ? x : lo;
+    hi = x > hi ? x : hi;
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (unsigned s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double a = smin[threadIdx.x + s], b = smax[threadIdx.x + s];
+      smin[threadIdx.x] = a < smin[threadIdx.x] ? a : smin[threadIdx.x];
+      smax[threadIdx.x] = b > smax[threadIdx.x] ? b : smax[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pmin[blockIdx.x] = smin[0];
+    pmax[blockIdx.x] = smax[0];
+  }
+}
+
+// ------------------------------------------------------- pixel splitting
+
+struct PsParams {
+  Geo g;
+  const double* rc;     // corner radial (unit)
+  const double* ac;     // corner chi
+  const double* rad;    // centre radial (unit)
+  const double* chi;    // centre chi
+  uint32_t n;           // bins
+  double min, step;
+  int grid_ok;          // 1D sampling grid representable
+  double glo, ghi;      // sampling grid radial range (min, min + n*step)
+  int clamp_low;
+};
+
+// pixel_split.rs:14-25.
+__device__ inline double polygon_area(const double (*poly)[2], int n) {
+  if (n < 3) return 0.0;
+  double area = 0.0;
+  for (int i = 0; i < n; i++) {
+    int j = (i + 1 == n) ? 0 : i + 1;
+    area += poly[i][0] * poly[j][1] - poly[j][0] * poly[i][1];
+  }
+  return 0.5 * area;
+}
+
+// pixel_split.rs:28-45.
+__device__ inline void intersect(const double* p0, const double* p1, int axis, double e,
+                                 double* out) {
+  int o = 1 - axis;
+  double d = p1[axis] - p0[axis];
+  if (fabs(d) < kEps) {
+    out[axis] = e;
+    out[o] = p0[o];
+    return;
+  }
+  double t = (e - p0[axis]) / d;
+  out[axis] = e;
+  out[o] = p0[o] + t * (p1[o] - p0[o]);
+}
+
+// One Sutherland-Hodgman stage (pixel_split.rs:48-137).
+__device__ inline int clip_stage(const double (*in)[2], int n, int axis, bool keep_ge, double e,
+                                 double (*out)[2]) {
+  if (n == 0) return 0;
+  int m = 0;
+  const double* prev = in[n - 1];
+  bool prev_in = keep_ge ? (prev[axis] >= e) : (prev[axis] <= e);
+  for (int i = 0; i < n; i++) {
+    const double* cur = in[i];
+    bool cur_in = keep_ge ? (cur[axis] >= e) : (cur[axis] <= e);
+    if (cur_in) {
+      if (!prev_in && m < kClipCap) intersect(prev, cur, axis, e, out[m++]);
+      if (m < kClipCap) {
+        out[m][0] = cur[0];
+        out[m][1] = cur[1];
+        m++;
+      }
+    } else if (prev_in && m < kClipCap) {
+      intersect(prev, cur, axis, e, out[m++]);
+    }
+    prev = cur;
+    prev_in = cur_in;
+  }
+  return m;
+}
+
+// pixel_split.rs:140-163: left, right, bottom, top.
+__device__ inline double clipped_area(const double (*poly)[2], double xmin, double xmax,
+                                      double ymin, double ymax) {
+  double a[kClipCap][2], b[kClipCap][2];
+  int m = clip_stage(poly, 4, 0, true, xmin, a);
+  if (m == 0) return 0.0;
+  m = clip_stage(a, m, 0, false, xmax, b);
+  if (m == 0) return 0.0;
+  m = clip_stage(b, m, 1, true, ymin, a);
+  if (m == 0) return 0.0;
+  m = clip_stage(a, m, 1, false, ymax, b);
+  return fabs(polygon_area(b, m));
+}
+
+// pixel_split.rs:166-175.
+__device__ inline double unwrap_angle(double angle, double reference) {
+  double a = angle;
+  double delta = a - reference;
+  if (delta > kPiD) a -= 2.0 * kPiD;
+  else if (delta < -kPiD) a += 2.0 * kPiD;
+  return a;
+}
+
+// pyfai_1d.rs:188-201.
+__device__ inline bool singular4(const double (*poly)[2]) {
+  double winding = 0.0;
+  for (int i = 0; i < 4; i++) {
+    if (poly[i][0] == 0.0) return true;
+    double delta = wrap_tau(poly[(i + 1) & 3][1] - poly[i][1] + kPiD) - kPiD;
+    if (fabs(delta) >= kPiD - 64.0 * kEps) return true;
+    winding += delta;
+  }
+  return fabs(winding) > kPiD;
+}
+
+// The entries of one pixel, in the reference's order (pixel_split.rs:208-326).
+// Returns false for a beam pixel (handled by sampling).
+template <class Emit>
+__device__ bool ps_pixel(const PsParams& P, uint32_t idx, Emit& emit) {
+  uint32_t dim2 = P.g.dim2;
+  uint64_t w = (uint64_t)dim2 + 1;
+  uint64_t b = (uint64_t)(idx / dim2) * w + idx % dim2;
+  uint64_t i00 = b, i01 = b + 1, i11 = b + w + 1, i10 = b + w;
+  double r00 = P.rc[i00], r01 = P.rc[i01], r10 = P.rc[i10], r11 = P.rc[i11];
+  if (P.grid_ok) {
+    double raw[4][2] = {{r00, P.ac[i00]}, {r01, P.ac[i01]}, {r11, P.ac[i11]}, {r10, P.ac[i10]}};
+    if (singular4(raw)) return false;
+  }
+  double ca = P.chi[idx];
+  double a00 = unwrap_angle(P.ac[i00], ca);
+  double a01 = unwrap_angle(P.ac[i01], ca);
+  double a10 = unwrap_angle(P.ac[i10], ca);
+  double a11 = unwrap_angle(P.ac[i11], ca);
+  double poly[4][2] = {{r00, a00}, {r01, a01}, {r11, a11}, {r10, a10}};
+  double area_total = fabs(polygon_area(poly, 4));
+  if (area_total <= 0.0) {
+    double pos = (P.rad[idx] - P.min) / P.step;
+    if (pos < 0.0) return true;
+    long long bin = sat_isize(floor(pos));
+    if (bin >= 0 && (unsigned long long)bin < P.n) emit((uint32_t)bin, 1.0);
+    return true;
+  }
+  double rmin = r00, rmax = r00;
+  double rv[3] = {r01, r10, r11};
+  for (int k = 0; k < 3; k++) {
+    if (rv[k] < rmin) rmin = rv[k];
+    if (rv[k] > rmax) rmax = rv[k];
+  }
+  long long start = sat_isize(floor((rmin - P.min) / P.step));
+  long long end = sat_isize(floor((rmax - P.min) / P.step));
+  if (end < 0 || start >= (long long)P.n) return true;
+  if (start < 0) start = 0;
+  if (end >= (long long)P.n) end = (long long)P.n - 1;
+  double amin = a00, amax = a00;
+  double av[3] = {a01, a10, a11};
+  for (int k = 0; k < 3; k++) {
+    if (av[k] < amin) amin = av[k];
+    if (av[k] > amax) amax = av[k];
+  }
+  for (long long bb = start; bb <= end; bb++) {
+    double bin_min = P.min + (double)bb * P.step;
+    double bin_max = bin_min + P.step;
+    double area = clipped_area(poly, bin_min, bin_max, amin, amax);
+    if (area > 0.0) emit((uint32_t)bb, area / area_total);
+  }
+  return true;
+}
+
+struct CountEmit {
+  uint32_t n = 0;
+  __device__ void operator()(uint32_t, double) { n++; }
+};
+struct FillEmit {
+  uint32_t* bin;
+  uint32_t* pix;
+  double* w;
+  uint64_t at;
+  uint32_t p;
+  __device__ void operator()(uint32_t b, double v) {
+    bin[at] = b;
+    pix[at] = p;
+    w[at] = v;
+    at++;
+  }
+};
+
+__global__ void k_ps_count(PsParams P, uint32_t npix, uint32_t* cnt, uint32_t* sing,
+                           uint32_t* nsing) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npix) return;
+  CountEmit e;
+  if (!ps_pixel(P, idx, e)) {
+    uint32_t s = atomicAdd(nsing, 1u);
+    if (s < kMaxSingular) sing[s] = idx;
+    cnt[idx] = 0;
+    return;
+  }
+  cnt[idx] = e.n;
+}
+
+// --------------------------------------------------------- beam sampling
+
+struct SampleGrid {
+  uint32_t n;
+  double r0, r1;
+  int clamp_low;
+};
+
+__device__ inline void sample_point(const Geo& g, uint32_t pixel, uint32_t s, double& d1,
+                                    double& d2) {
+  const double dy[4] = {0.25, 0.5, 0.75, 0.5};
+  const double dx[4] = {0.5, 0.25, 0.5, 0.75};
+  uint32_t row = s >> 9, col = (s >> 2) & 127u, o = s & 3u;
+  d1 = (double)(pixel / g.dim2) + ((double)row + dy[o]) / 128.0;
+  d2 = (double)(pixel % g.dim2) + ((double)col + dx[o]) / 128.0;
+}
+
+__global__ void k_sing_samples(Geo g, const uint32_t* sing, uint32_t nsing, double* stth,
+                               double* schi, uint32_t* flist, uint32_t* fcount) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)nsing * kSamplesPerPixel) return;
+  uint32_t slot = (uint32_t)(i / kSamplesPerPixel), s = (uint32_t)(i % kSamplesPerPixel);
+  double d1, d2;
+  sample_point(g, sing[slot], s, d1, d2);
+  Angles a = angles_cert(g, d1, d2);
+  stth[i] = a.tth;
+  schi[i] = a.chi;
+  if (a.uncertain) flist[atomicAdd(fcount, 1u)] = (uint32_t)i;
+}
+
+// pyfai_1d.rs:228-251 binning of the samples on the 1D grid (n radial bins,
+// one azimuthal bin over (-pi, pi)).
+__global__ void k_sing_bin(Geo g, SampleGrid G, uint32_t nsing, const double* stth,
+                           const double* schi, uint32_t* hits) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)nsing * kSamplesPerPixel) return;
+  uint32_t slot = (uint32_t)(i / kSamplesPerPixel);
+  double r = tth_to_unit(g, stth[i]);
+  double a0 = -kPiD, a1 = kPiD;
+  double a = a0 + rem_euclid(schi[i] - a0, kTauD);
+  if ((r < G.r0 && !G.clamp_low) || r >= G.r1 || a >= a1) return;
+  unsigned long long ir = r < G.r0 ? 0ULL : sat_usize((r - G.r0) / (G.r1 - G.r0) * (double)G.n);
+  unsigned long long ia = sat_usize((a - a0) / (a1 - a0) * 1.0);
+  if (ir < G.n && ia < 1) atomicAdd(&hits[(uint64_t)slot * G.n + ir], 1u);
+}
+
+__global__ void k_sing_count(const uint32_t* hits, uint32_t n, uint32_t nsing,
+                             const uint32_t* sing, uint32_t* cnt) {
+  uint32_t slot = blockIdx.x;
+  if (slot >= nsing) return;
+  __shared__ uint32_t part[kThreads];
+  uint32_t c = 0;
+  for (uint32_t b = threadIdx.x; b < n; b += blockDim.x) c += hits[(uint64_t)slot * n + b] != 0;
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (unsigned s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cnt[sing[slot]] = part[0];
+}
+
+// Entries of beam pixels: bins ascending, weight hits / 65536.
+__global__ void k_sing_fill(const uint32_t* hits, uint32_t n, uint32_t nsing,
+                            const uint32_t* sing, const uint32_t* off, uint32_t* bin,
+                            uint32_t* pix, double* w) {
+  uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nsing) return;
+  uint32_t p = sing[slot];
+  uint64_t at = off[p];
+  for (uint32_t b = 0; b < n; b++) {
+    uint32_t h = hits[(uint64_t)slot * n + b];
+    if (h) {
+      bin[at] = b;
+      pix[at] = p;
+      w[at] = (double)h / (double)kSamplesPerPixel;
+      at++;
+    }
+  }
+}
+
+__global__ void k_ps_fill(PsParams P, uint32_t npix, const uint32_t* off, uint32_t* bin,
+                          uint32_t* pix, double* w) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npix) return;
+  FillEmit e{bin, pix, w, off[idx], idx};
+  ps_pixel(P, idx, e);
+}
+
+// ------------------------------------------------------------- counting
+
+// counting_csr.rs:54-134: one unweighted entry per in-range pixel centre.
+__global__ void k_count_bins(const double* rad, uint32_t npix, uint32_t n, double min,
+                             double max, double step, int has_range, uint32_t* cnt,
+                             uint32_t* pbin) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  double r = rad[i];
+  if (has_range && !(r >= min && r <= max)) {
+    cnt[i] = 0;
+    return;
+  }
+  unsigned long long b = sat_usize((r - min) / step);
+  if (b > n - 1) b = n - 1;
+  cnt[i] = 1;
+  pbin[i] = (uint32_t)b;
+}
+
+__global__ void k_count_fill(const uint32_t* cnt, const uint32_t* pbin, const uint32_t* off,
+                             uint32_t npix, uint32_t* bin, uint32_t* pix, double* w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix || !cnt[i]) return;
+  uint32_t at = off[i];
+  bin[at] = pbin[i];
+  pix[at] = i;
+  w[at] = 1.0;
+}
+
+// --------------------------------------------------------- CSR assembly
+
+__global__ void k_iota(uint32_t* v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (uint32_t)i;
+}
+
+__global__ void k_bin_hist(const uint32_t* bin, uint64_t n, uint32_t* hist) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&hist[bin[i]], 1u);
+}
+
+__global__ void k_gather_sorted(const uint32_t* order, const uint32_t* pix, const double* w,
+                                uint64_t n, uint32_t* col, double* w64, float* w32) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = order[i];
+  col[i] = pix[k];
+  double v = w[k];
+  w64[i] = v;
+  w32[i] = (float)v;
+}
+
+__global__ void k_keep(const uint32_t* col, const uint8_t* mask, uint64_t n, uint32_t* keep) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keep[i] = mask[col[i]] == 0;
+}
+
+__global__ void k_compact(const uint32_t* keep, const uint32_t* pos, uint64_t n,
+                          const uint32_t* col, const double* w64, const float* w32,
+                          uint32_t* ocol, double* ow64, float* ow32) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  uint32_t j = pos[i];
+  ocol[j] = col[i];
+  ow64[j] = w64[i];
+  ow32[j] = w32[i];
+}
+
+__global__ void k_rows_after_compact(const uint32_t* row_ptr, const uint32_t* pos,
+                                     uint32_t nbins, uint32_t* out) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r <= nbins) out[r] = pos[row_ptr[r]];
+}
+
+// corrections.rs:57-93, geometry.rs:552-636, 697-717.
+__global__ void k_normalization(Geo g, const double* tth, const double* chi, int has_pol,
+                                double factor, double* norm) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t npix = (uint64_t)g.dim1 * g.dim2;
+  if (i >= npix) return;
+  uint32_t r = (uint32_t)(i / g.dim2), c = (uint32_t)(i % g.dim2);
+  double p1 = ((double)r + 0.5) * g.pixel1 - g.poni1;
+  double p2 = ((double)c + 0.5) * g.pixel2 - g.poni2;
+  double p3 = g.distance;
+  double cosa = p3 / sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+  double sa = cosa * cosa * cosa;
+  double pol = 1.0;
+  if (has_pol) {
+    double ct = cos(tth[i]);
+    double c2 = ct * ct;
+    pol = 0.5 * (1.0 + c2 - factor * cos(2.0 * chi[i]) * (1.0 - c2));
+  }
+  norm[i] = pol * sa;
+}
+
+// prepared.rs:229-262, row<FAST = false>: n = n + w * norm[j], row order.
+__global__ void k_denominators(const uint32_t* row_ptr, const uint32_t* col, const double* w64,
+                               int weighted, const double* norm, uint32_t nbins, double* den) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nbins) return;
+  double n = 0.0;
+  for (uint32_t k = row_ptr[r]; k < row_ptr[r + 1]; k++) {
+    if (weighted) n = n + w64[k] * norm[col[k]];
+    else n = n + norm[col[k]];
+  }
+  den[r] = n;
+}
+
+// ---------------------------------------------------------------- host
+
+struct Scratch {
+  DevBuf<uint8_t> tmp;
+  void* get(size_t bytes) {
+    if (tmp.n < bytes) tmp.alloc(bytes);
+    return tmp.p;
+  }
+};
+
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t s, Scratch& sc) {
+  size_t bytes = 0;
+  DIO_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s));
+  DIO_HIP(hipcub::DeviceScan::ExclusiveSum(sc.get(bytes), bytes, in, out, (int)n, s));
+}
+
+template <class T>
+T read1(const T* d, hipStream_t s) {
+  T v;
+  DIO_HIP(hipMemcpyAsync(&v, d, sizeof(T), hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+// Recompute flagged angles with the host libm (the reference's own atan2),
+// in parallel over host threads.
+void host_patch(const Geo& g, const std::vector<uint32_t>& idx,
+                const std::vector<std::pair<double, double>>& points, std::vector<double>& tv,
+                std::vector<double>& cv) {
+  size_t n = idx.size();
+  tv.resize(n);
+  cv.resize(n);
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nt = 1;
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; t++) {
+    pool.emplace_back([&, t]() {
+      for (size_t i = t; i < n; i += nt) {
+        AngleArgs a = angle_args(g, points[i].first, points[i].second);
+        tv[i] = ::atan2(a.ty, a.tx);
+        cv[i] = ::atan2(a.cy, a.cx);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
+// Angles of a rows x cols point grid at d = index + off, certified and
+// patched. Returns the number of patched points.
+uint64_t grid_angles(const Geo& g, uint32_t rows, uint32_t cols, double off, double* tth,
+                     double* chi, hipStream_t s) {
+  uint64_t n = (uint64_t)rows * cols;
+  DevBuf<uint32_t> flist(n), fcount(1);
+  DIO_HIP(hipMemsetAsync(fcount.p, 0, sizeof(uint32_t), s));
+  k_grid_angles<<<blocks_for(n, kThreads), kThreads, 0, s>>>(g, rows, cols, off, tth, chi,
+                                                            flist.p, fcount.p);
+  DIO_HIP(hipGetLastError());
+  uint32_t nf = read1(fcount.p, s);
+  if (nf == 0) return 0;
+  std::vector<uint32_t> idx(nf);
+  DIO_HIP(hipMemcpyAsync(idx.data(), flist.p, nf * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipStreamSynchronize(s));
+  std::vector<std::pair<double, double>> pts(nf);
+  for (uint32_t i = 0; i < nf; i++)
+    pts[i] = {(double)(idx[i] / cols) + off, (double)(idx[i] % cols) + off};
+  std::vector<double> tv, cv;
+  host_patch(g, idx, pts, tv, cv);
+  DevBuf<uint32_t> di(nf);
+  DevBuf<double> dt(nf), dc(nf);
+  DIO_HIP(hipMemcpyAsync(di.p, idx.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  DIO_HIP(hipMemcpyAsync(dt.p, tv.data(), nf * sizeof(double), hipMemcpyHostToDevice, s));
+  DIO_HIP(hipMemcpyAsync(dc.p, cv.data(), nf * sizeof(double), hipMemcpyHostToDevice, s));
+  k_patch<<<blocks_for(nf, kThreads), kThreads, 0, s>>>(di.p, dt.p, dc.p, nf, tth, chi);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+  return nf;
+}
+
+void device_minmax(const double* v, uint64_t n, hipStream_t s, double& lo, double& hi) {
+  unsigned nb = std::min<uint64_t>(1024, std::max<uint64_t>(1, blocks_for(n, kThreads)));
+  DevBuf<double> pmin(nb), pmax(nb);
+  k_minmax<<<nb, kThreads, 0, s>>>(v, n, pmin.p, pmax.p);
+  DIO_HIP(hipGetLastError());
+  std::vector<double> a(nb), b(nb);
+  DIO_HIP(hipMemcpyAsync(a.data(), pmin.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipMemcpyAsync(b.data(), pmax.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipStreamSynchronize(s));
+  lo = a[0];
+  hi = b[0];
+  for (unsigned i = 1; i < nb; i++) {
+    if (a[i] < lo) lo = a[i];
+    if (b[i] > hi) hi = b[i];
+  }
+}
+
+// Pixel-major entries -> CSR with rows in ascending pixel order.
+void assemble(uint32_t nbins, uint64_t nnz, DevBuf<uint32_t>& bin, DevBuf<uint32_t>& pix,
+              DevBuf<double>& w, hipStream_t s, Scratch& sc, Table& t) {
+  t.nbins = nbins;
+  t.nnz = nnz;
+  t.row_ptr.alloc(nbins + 1);
+  t.col.alloc(nnz);
+  t.w64.alloc(nnz);
+  t.w32.alloc(nnz);
+  DevBuf<uint32_t> hist(nbins + 1);
+  DIO_HIP(hipMemsetAsync(hist.p, 0, (nbins + 1) * sizeof(uint32_t), s));
+  if (nnz) {
+    k_bin_hist<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(bin.p, nnz, hist.p);
+    DIO_HIP(hipGetLastError());
+  }
+  exclusive_scan(hist.p, t.row_ptr.p, nbins + 1, s, sc);
+  if (!nnz) return;
+  DevBuf<uint32_t> order_in(nnz), order(nnz), keys(nnz);
+  k_iota<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(order_in.p, nnz);
+  int end_bit = 1;
+  while (end_bit < 32 && (1ull << end_bit) < nbins) end_bit++;
+  size_t bytes = 0;
+  DIO_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, bin.p, keys.p, order_in.p, order.p,
+                                             (int)nnz, 0, end_bit, s));
+  DIO_HIP(hipcub::DeviceRadixSort::SortPairs(sc.get(bytes), bytes, bin.p, keys.p, order_in.p,
+                                             order.p, (int)nnz, 0, end_bit, s));
+  k_gather_sorted<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(order.p, pix.p, w.p, nnz,
+                                                                t.col.p, t.w64.p, t.w32.p);
+  DIO_HIP(hipGetLastError());
+}
+
+void compact(const Table& in, const uint8_t* dmask, hipStream_t s, Scratch& sc, Table& out) {
+  out.nbins = in.nbins;
+  DevBuf<uint32_t> keep(in.nnz + 1), pos(in.nnz + 1);
+  DIO_HIP(hipMemsetAsync(keep.p, 0, (in.nnz + 1) * sizeof(uint32_t), s));
+  if (in.nnz)
+    k_keep<<<blocks_for(in.nnz, kThreads), kThreads, 0, s>>>(in.col.p, dmask, in.nnz, keep.p);
+  exclusive_scan(keep.p, pos.p, in.nnz + 1, s, sc);
+  uint32_t m = read1(pos.p + in.nnz, s);
+  out.nnz = m;
+  out.row_ptr.alloc(in.nbins + 1);
+  out.col.alloc(m);
+  out.w64.alloc(m);
+  out.w32.alloc(m);
+  k_rows_after_compact<<<blocks_for(in.nbins + 1, kThreads), kThreads, 0, s>>>(
+      in.row_ptr.p, pos.p, in.nbins, out.row_ptr.p);
+  if (in.nnz)
+    k_compact<<<blocks_for(in.nnz, kThreads), kThreads, 0, s>>>(
+        keep.p, pos.p, in.nnz, in.col.p, in.w64.p, in.w32.p, out.col.p, out.w64.p, out.w32.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+}
+
+// IntegrationGrid2d::new validity for the 1D sampling grid (grid.rs:48-108).
+bool sample_grid_ok(uint32_t n, double lo, double hi) {
+  double bounds[2][2] = {{lo, hi}, {-kPiD, kPiD}};
+  uint32_t counts[2] = {n, 1};
+  if (n == 0) return false;
+  for (int k = 0; k < 2; k++) {
+    double l = bounds[k][0], h = bounds[k][1];
+    double step = (h - l) / (double)counts[k];
+    if (!std::isfinite(l) || !std::isfinite(h) || !std::isfinite(step) || step <= 0.0 ||
+        l + step == l || h - step == h)
+      return false;
+  }
+  return lo >= 0.0;
+}
+
+// Beam pixels: sample, patch uncertain sample angles, bin, count, fill.
+struct Singular {
+  std::vector<uint32_t> pixels;
+  DevBuf<uint32_t> dpix, hits;
+  uint64_t patched = 0;
+};
+
+void singular_prepare(const Geo& g, const SampleGrid& G, Singular& S, uint32_t* cnt,
+                      hipStream_t s) {
+  uint32_t ns = (uint32_t)S.pixels.size();
+  if (!ns) return;
+  std::sort(S.pixels.begin(), S.pixels.end());
+  S.dpix.alloc(ns);
+  DIO_HIP(hipMemcpyAsync(S.dpix.p, S.pixels.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         s));
+  uint64_t total = (uint64_t)ns * kSamplesPerPixel;
+  DevBuf<double> st(total), sc(total);
+  DevBuf<uint32_t> flist(total), fcount(1);
+  DIO_HIP(hipMemsetAsync(fcount.p, 0, sizeof(uint32_t), s));
+  k_sing_samples<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, S.dpix.p, ns, st.p, sc.p,
+                                                                 flist.p, fcount.p);
+  DIO_HIP(hipGetLastError());
+  uint32_t nf = read1(fcount.p, s);
+  if (nf) {
+    std::vector<uint32_t> idx(nf);
+    DIO_HIP(hipMemcpyAsync(idx.data(), flist.p, nf * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DIO_HIP(hipStreamSynchronize(s));
+    std::vector<std::pair<double, double>> pts(nf);
+    const double dy[4] = {0.25, 0.5, 0.75, 0.5};
+    const double dx[4] = {0.5, 0.25, 0.5, 0.75};
+    for (uint32_t i = 0; i < nf; i++) {
+      uint32_t slot = idx[i] / kSamplesPerPixel, smp = idx[i] % kSamplesPerPixel;
+      uint32_t pixel = S.pixels[slot];
+      uint32_t row = smp >> 9, col = (smp >> 2) & 127u, o = smp & 3u;
+      pts[i] = {(double)(pixel / g.dim2) + ((double)row + dy[o]) / 128.0,
+                (double)(pixel % g.dim2) + ((double)col + dx[o]) / 128.0};
+    }
+    std::vector<double> tv, cv;
+    host_patch(g, idx, pts, tv, cv);
+    DevBuf<uint32_t> di(nf);
+    DevBuf<double> dt(nf), dc(nf);
+    DIO_HIP(hipMemcpyAsync(di.p, idx.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    DIO_HIP(hipMemcpyAsync(dt.p, tv.data(), nf * sizeof(double), hipMemcpyHostToDevice, s));
+    DIO_HIP(hipMemcpyAsync(dc.p, cv.data(), nf * sizeof(double), hipMemcpyHostToDevice, s));
+    k_patch<<<blocks_for(nf, kThreads), kThreads, 0, s>>>(di.p, dt.p, dc.p, nf, st.p, sc.p);
+    DIO_HIP(hipGetLastError());
+    S.patched = nf;
+  }
+  S.hits.alloc((uint64_t)ns * G.n);
+  DIO_HIP(hipMemsetAsync(S.hits.p, 0, S.hits.bytes(), s));
+  k_sing_bin<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, G, ns, st.p, sc.p, S.hits.p);
+  k_sing_count<<<ns, kThreads, 0, s>>>(S.hits.p, G.n, ns, S.dpix.p, cnt);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+}
+
+void build_1d(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, Built& out) {
+  uint32_t dim1 = d.dim1, dim2 = d.dim2;
+  uint64_t npix = (uint64_t)dim1 * dim2;
+  uint32_t n = d.n_radial;
+  // Pixel-centre angles (calc_tth_all / calc_azimuth_all).
+  DevBuf<double> ttp(npix), chp(npix), radp;
+  out.uncertain += grid_angles(g, dim1, dim2, 0.5, ttp.p, chp.p, s);
+  const double* rad = ttp.p;
+  if (g.unit != 0) {
+    radp.alloc(npix);
+    k_to_unit<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(g, ttp.p, radp.p, npix);
+    rad = radp.p;
+  }
+  DevBuf<uint32_t> cnt(npix + 1), off(npix + 1);
+  DIO_HIP(hipMemsetAsync(cnt.p, 0, (npix + 1) * sizeof(uint32_t), s));
+  DevBuf<uint32_t> ebin, epix;
+  DevBuf<double> ew;
+  uint64_t nnz = 0;
+  if (d.method == DIO_COUNTING) {
+    double min, max, step;
+    if (d.has_radial_range) {
+      min = d.radial_min;
+      max = d.radial_max + 10.0 * kEps;
+    } else {
+      device_minmax(rad, npix, s, min, max);
+      max = max + 10.0 * kEps;
+    }
+    step = (max - min) / (double)n;
+    DevBuf<uint32_t> pbin(npix);
+    k_count_bins<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(rad, (uint32_t)npix, n, min, max,
+                                                                step, d.has_radial_range, cnt.p,
+                                                                pbin.p);
+    exclusive_scan(cnt.p, off.p, npix + 1, s, sc);
+    nnz = read1(off.p + npix, s);
+    ebin.alloc(nnz);
+    epix.alloc(nnz);
+    ew.alloc(nnz);
+    k_count_fill<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(cnt.p, pbin.p, off.p,
+                                                                (uint32_t)npix, ebin.p, epix.p,
+                                                                ew.p);
+    DIO_HIP(hipGetLastError());
+    out.weighted = false;
+    out.hist_min = min;
+    out.hist_step = step;
+  } else if (d.method == DIO_PIXEL_SPLIT) {
+    uint64_t nc = (uint64_t)(dim1 + 1) * (dim2 + 1);
+    DevBuf<double> tc(nc), ac(nc), rcb;
+    out.uncertain += grid_angles(g, dim1 + 1, dim2 + 1, 0.0, tc.p, ac.p, s);
+    const double* rc = tc.p;
+    if (g.unit != 0) {
+      rcb.alloc(nc);
+      k_to_unit<<<blocks_for(nc, kThreads), kThreads, 0, s>>>(g, tc.p, rcb.p, nc);
+      rc = rcb.p;
+    }
+    double min, max;
+    if (d.has_radial_range) {
+      min = d.radial_min;
+      max = d.radial_max + 10.0 * kEps;
+    } else {
+      device_minmax(rc, nc, s, min, max);
+      max += 10.0 * kEps;
+    }
+    double step = (max - min) / (double)n;
+    PsParams P;
+    P.g = g;
+    P.rc = rc;
+    P.ac = ac.p;
+    P.rad = rad;
+    P.chi = chp.p;
+    P.n = n;
+    P.min = min;
+    P.step = step;
+    P.glo = min;
+    P.ghi = min + (double)n * step;
+    P.grid_ok = sample_grid_ok(n, P.glo, P.ghi) ? 1 : 0;
+    P.clamp_low = d.has_radial_range ? 0 : 1;
+    DevBuf<uint32_t> sing(kMaxSingular), nsing(1);
+    DIO_HIP(hipMemsetAsync(nsing.p, 0, sizeof(uint32_t), s));
+    k_ps_count<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(P, (uint32_t)npix, cnt.p, sing.p,
+                                                              nsing.p);
+    DIO_HIP(hipGetLastError());
+    uint32_t ns = read1(nsing.p, s);
+    if (ns > kMaxSingular) fail(DIO_ERR_UNSUPPORTED, "too many beam pixels for sampling");
+    Singular S;
+    S.pixels.resize(ns);
+    if (ns) {
+      DIO_HIP(hipMemcpyAsync(S.pixels.data(), sing.p, ns * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, s));
+      DIO_HIP(hipStreamSynchronize(s));
+    }
+    SampleGrid G{n, P.glo, P.ghi, P.clamp_low};
+    singular_prepare(g, G, S, cnt.p, s);
+    out.uncertain += S.patched;
+    out.singular = ns;
+    exclusive_scan(cnt.p, off.p, npix + 1, s, sc);
+    uint64_t total = read1(off.p + npix, s);
+    nnz = total;
+    ebin.alloc(nnz);
+    epix.alloc(nnz);
+    ew.alloc(nnz);
+    k_ps_fill<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(P, (uint32_t)npix, off.p, ebin.p,
+                                                             epix.p, ew.p);
+    if (ns)
+      k_sing_fill<<<blocks_for(ns, 64), 64, 0, s>>>(S.hits.p, n, ns, S.dpix.p, off.p, ebin.p,
+                                                    epix.p, ew.p);
+    DIO_HIP(hipGetLastError());
+    DIO_HIP(hipStreamSynchronize(s));
+    out.hist_min = min;
+    out.hist_step = step;
+  } else {
+    fail(DIO_ERR_UNSUPPORTED, "method not available for 1D tables on this device build");
+  }
+  if (nnz > 0x7fffffffull) fail(DIO_ERR_UNSUPPORTED, "table exceeds 2^31 entries");
+  out.n_radial = n;
+  out.n_azimuthal = 1;
+  assemble(n, nnz, ebin, epix, ew, s, sc, out.full);
+  // Corrections use the (patched) pixel-centre angles.
+  out.norm.alloc(npix);
+  k_normalization<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(
+      g, ttp.p, chp.p, d.has_polarization, d.polarization_factor, out.norm.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+Geo make_geo(const dio_poni& p, const dio_plan_desc& d) {
+  Geo g;
+  g.pixel1 = p.pixel1;
+  g.pixel2 = p.pixel2;
+  g.poni1 = p.poni1;
+  g.poni2 = p.poni2;
+  g.distance = p.distance;
+  // geometry.rs:640-660: host libm sin/cos of the rotations.
+  g.s1 = ::sin(p.rot1);
+  g.c1 = ::cos(p.rot1);
+  g.s2 = ::sin(p.rot2);
+  g.c2 = ::cos(p.rot2);
+  g.s3 = ::sin(p.rot3);
+  g.c3 = ::cos(p.rot3);
+  g.wavelength = p.wavelength;
+  g.unit = d.unit;
+  g.q_scale = 0.0;
+  g.dim1 = d.dim1;
+  g.dim2 = d.dim2;
+  return g;
+}
+
+void build_tables(const Geo& g, const dio_plan_desc& d, hipStream_t s, Built& out) {
+  Scratch sc;
+  if (d.ndim != 1) fail(DIO_ERR_UNSUPPORTED, "2D tables are not available in this build");
+  build_1d(g, d, s, sc, out);
+  uint64_t npix = (uint64_t)d.dim1 * d.dim2;
+  bool any_mask = false;
+  if (d.mask) {
+    for (uint64_t i = 0; i < npix && !any_mask; i++) any_mask = d.mask[i] != 0;
+  }
+  if (any_mask) {
+    out.mask.alloc(npix);
+    DIO_HIP(hipMemcpyAsync(out.mask.p, d.mask, npix, hipMemcpyHostToDevice, s));
+    compact(out.full, out.mask.p, s, sc, out.exec);
+    out.has_exec = true;
+  }
+  const Table& t = out.exec_table();
+  out.den.alloc(t.nbins);
+  k_denominators<<<blocks_for(t.nbins, 64), 64, 0, s>>>(t.row_ptr.p, t.col.p, t.w64.p,
+                                                        out.weighted ? 1 : 0, out.norm.p,
+                                                        t.nbins, out.den.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace dio

@@ -1,0 +1,67 @@
+// Shared host-side plumbing: errors, device buffers, launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "../../include/dioptrin_hip.h"
+
+namespace dio {
+
+// Typed error carried inside the library and turned into a status code and
+// thread-local message at the C boundary (mirrors CudaError, cuda.rs:218-237).
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(DIO_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define DIO_HIP(call) ::dio::hip_check((call), #call)
+
+// Owning device allocation.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) DIO_HIP(hipMalloc(&p, count * sizeof(T)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+inline unsigned blocks_for(size_t n, unsigned threads) {
+  return static_cast<unsigned>((n + threads - 1) / threads);
+}
+
+}  // namespace dio

@@ -1,0 +1,467 @@
+"""Python front end mirroring the reference's Integrator / Cuda1dPlan API.
+
+Reference interfaces (paths relative to the reference repository root):
+  Integrator(...)                dioptrin.pyi:337-460, src/python/integrator.rs
+  Integrator.integrate1d         dioptrin.pyi:597-636, src/integration/methods.rs:24-190
+  Integrator.integrate2d         dioptrin.pyi:660-733
+  Integrator.batch1d / batch2d   dioptrin.pyi:845-1063, src/python/integrator.rs:1867, 2423
+  Integrator.prepare_cuda1d/2d   dioptrin.pyi:490-560, src/integration/cuda.rs:1089-1157
+  Cuda1dPlan.integrate1d/batch1d dioptrin.pyi:1358-1433
+
+Every computation runs in libdioptrin_hip.so on the GPU; this module only
+marshals arguments, caches plans and wraps results.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence, Union
+
+import numpy as np
+
+from . import _native as N
+
+_NP_DTYPES = {np.dtype(np.uint16): "uint16", np.dtype(np.uint32): "uint32",
+              np.dtype(np.int32): "int32", np.dtype(np.float32): "float32",
+              np.dtype(np.float64): "float64"}
+
+
+@dataclass
+class IntegrationResult:
+    """Result of one frame (python/results.rs: radial, intensity, errors,
+    azimuthal_std; 2D adds azimuthal and the grid shape)."""
+    radial: Optional[np.ndarray]
+    intensity: np.ndarray
+    errors: Optional[np.ndarray] = None
+    azimuthal_std: Optional[np.ndarray] = None
+    azimuthal: Optional[np.ndarray] = None
+    n_radial: Optional[int] = None
+    n_azimuthal: Optional[int] = None
+
+
+@dataclass
+class BatchResult:
+    """One entry of a batch (python BatchResult: result or error per item)."""
+    filename: str
+    result: Optional[IntegrationResult] = None
+    error: Optional[str] = None
+
+    def is_ok(self) -> bool:
+        return self.error is None
+
+    def is_err(self) -> bool:
+        return self.error is not None
+
+
+def load_poni(path: str) -> dict:
+    """Reads the PONI v1/v2 keys used by the geometry (poni.rs:79-135)."""
+    keys = {"pixelsize1": "pixel1", "pixelsize2": "pixel2", "distance": "distance",
+            "poni1": "poni1", "poni2": "poni2", "rot1": "rot1", "rot2": "rot2", "rot3": "rot3",
+            "wavelength": "wavelength"}
+    out = {k: 0.0 for k in keys.values()}
+    import json
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if not line or line.startswith("#") or ":" not in line:
+                continue
+            k, v = line.split(":", 1)
+            k, v = k.strip().lower(), v.strip()
+            if k == "detector_config":
+                cfg = json.loads(v)
+                for name in ("pixel1", "pixel2"):
+                    if name in cfg:
+                        out[name] = float(cfg[name])
+            elif k in keys:
+                out[keys[k]] = float(v)
+    if out["pixel1"] == 0.0 or out["pixel2"] == 0.0:
+        raise ValueError(f"{path}: missing pixel sizes (pixel1/pixel2) in PONI file")
+    return out
+
+
+class _Context:
+    _cache: dict = {}
+
+    @classmethod
+    def get(cls, device: int):
+        if device not in cls._cache:
+            h = C.c_void_p()
+            N.check(N.lib().dio_open(device, C.byref(h)))
+            cls._cache[device] = h
+        return cls._cache[device]
+
+
+def device_info(device: int = 0) -> dict:
+    """Device description (cuda_info counterpart, cuda.rs:309-326)."""
+    info = N.DeviceInfo()
+    N.check(N.lib().dio_device_info_get(_Context.get(device), C.byref(info)))
+    return {"name": info.name.decode(), "arch": info.arch.decode(),
+            "compute_units": info.compute_units, "xcds": info.xcds,
+            "total_memory_bytes": info.total_memory, "runtime_version": info.runtime_version}
+
+
+class GpuPlan:
+    """A lookup table built and kept on one GPU (Cuda1dPlan / Cuda2dPlan
+    counterpart). A snapshot: later Integrator changes do not affect it."""
+
+    def __init__(self, poni: dict, shape, method="pixel_split", n_ss=None, unit="2th_rad",
+                 n_radial=1000, n_azimuthal=None, radial_range=None, azimuthal_range=None,
+                 polarization_factor=None, mask=None, precision="f32_compensated", device=0):
+        if method not in N.METHODS:
+            raise ValueError(f"unknown method {method!r}")
+        if unit not in N.UNITS:
+            raise ValueError(f"unknown unit {unit!r}")
+        if precision not in N.PRECISIONS:
+            raise ValueError(f"unknown precision {precision!r}; expected one of "
+                             f"{sorted(N.PRECISIONS)}")
+        dim1, dim2 = int(shape[0]), int(shape[1])
+        self.shape = (dim1, dim2)
+        self.ndim = 1 if n_azimuthal is None else 2
+        self.precision = precision
+        self.device = device
+        p = N.Poni(*(float(poni.get(k, 0.0)) for k in (
+            "pixel1", "pixel2", "distance", "poni1", "poni2", "rot1", "rot2", "rot3",
+            "wavelength")))
+        d = N.PlanDesc()
+        d.dim1, d.dim2 = dim1, dim2
+        d.method = N.METHODS[method]
+        d.n_ss = int(n_ss or (5 if method == "supersampled" else 0))
+        d.unit = N.UNITS[unit]
+        d.ndim = self.ndim
+        d.n_radial = int(n_radial)
+        d.n_azimuthal = int(n_azimuthal or 1)
+        if radial_range is not None:
+            d.has_radial_range = 1
+            d.radial_min, d.radial_max = map(float, radial_range)
+        if azimuthal_range is not None:
+            d.has_azimuthal_range = 1
+            d.azimuthal_min, d.azimuthal_max = map(float, azimuthal_range)
+        if polarization_factor is not None:
+            d.has_polarization = 1
+            d.polarization_factor = float(polarization_factor)
+        self._mask = None
+        if mask is not None:
+            m = np.ascontiguousarray(mask, dtype=np.uint8)
+            if m.shape != self.shape:
+                raise ValueError(f"mask shape {m.shape} does not match image shape {self.shape}")
+            self._mask = m
+            d.mask = m.ctypes.data
+        d.precision = N.PRECISIONS[precision]
+        h = C.c_void_p()
+        N.check(N.lib().dio_plan_create(_Context.get(device), C.byref(p), C.byref(d), C.byref(h)))
+        self._h = h
+        self._mask = None
+        info = N.PlanInfo()
+        N.check(N.lib().dio_plan_info_get(h, C.byref(info)))
+        self.info = {f: getattr(info, f) for f, _ in N.PlanInfo._fields_}
+        self.n_radial = info.n_radial
+        self.n_azimuthal = info.n_azimuthal if self.ndim == 2 else None
+        self.nbins = info.nbins
+        self.radial = np.empty(self.n_radial)
+        N.check(N.lib().dio_plan_radial(h, self.radial.ctypes.data))
+        self.azimuthal = None
+        if self.ndim == 2:
+            self.azimuthal = np.empty(self.n_azimuthal)
+            N.check(N.lib().dio_plan_azimuthal(h, self.azimuthal.ctypes.data))
+
+    # -- properties mirroring Cuda1dPlan
+    @property
+    def n_points(self):
+        return self.n_radial
+
+    @property
+    def table_entries(self):
+        return self.info["nnz_masked"]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().dio_plan_destroy(h)
+            self._h = None
+
+    # -- table access for parity checks
+    def export_csr(self, masked=False):
+        nnz = self.info["nnz_masked"] if masked else self.info["nnz"]
+        rp = np.empty(self.nbins + 1, np.uint32)
+        col = np.empty(max(nnz, 1), np.uint32)
+        w64 = np.empty(max(nnz, 1), np.float64)
+        w32 = np.empty(max(nnz, 1), np.float32)
+        N.check(N.lib().dio_plan_export_csr(self._h, int(masked), rp.ctypes.data, col.ctypes.data,
+                                            w64.ctypes.data, w32.ctypes.data, max(nnz, 1)))
+        return rp, col[:nnz], w64[:nnz], w32[:nnz]
+
+    def denominators(self):
+        out = np.empty(self.nbins)
+        N.check(N.lib().dio_plan_denominators(self._h, out.ctypes.data))
+        return out
+
+    def normalization(self):
+        out = np.empty(self.shape[0] * self.shape[1])
+        N.check(N.lib().dio_plan_normalization(self._h, out.ctypes.data))
+        return out
+
+    # -- integration
+    def _stack(self, images) -> np.ndarray:
+        if isinstance(images, np.ndarray) and images.ndim == 3:
+            arr = images
+        else:
+            images = list(images)
+            if not images:
+                return np.empty((0,) + self.shape, np.float32)
+            dts = {np.asarray(im).dtype for im in images}
+            if len(dts) != 1:
+                raise TypeError("all images of one call must have the same dtype")
+            arr = np.stack([np.asarray(im) for im in images])
+        if arr.dtype not in _NP_DTYPES:
+            raise TypeError(f"unsupported dtype {arr.dtype}; expected float64, float32, "
+                            f"uint16, uint32 or int32")
+        if arr.shape[1:] != self.shape:
+            raise ValueError(f"frames have shape {arr.shape[1:]}, but the plan expects "
+                             f"{self.shape}")
+        return np.ascontiguousarray(arr)
+
+    def integrate_stack(self, stack: np.ndarray, errors=False, azimuthal_std=False):
+        """Raw arrays for a (n, dim1, dim2) stack: intensity, sigma, std."""
+        stack = self._stack(stack)
+        n = stack.shape[0]
+        flags = (N.WITH_ERRORS if errors else 0) | (N.WITH_AZIMUTHAL_STD if azimuthal_std else 0)
+        I = np.empty((n, self.nbins))
+        S = np.empty((n, self.nbins)) if errors else None
+        D = np.empty((n, self.nbins)) if azimuthal_std else None
+        if n:
+            N.check(N.lib().dio_integrate(self._h, stack.ctypes.data, N.DTYPES[_NP_DTYPES[
+                stack.dtype]], n, flags, I.ctypes.data, None if S is None else S.ctypes.data,
+                None if D is None else D.ctypes.data))
+        return I, S, D
+
+    def _result(self, i, s, d):
+        if self.ndim == 2:
+            shape = (self.n_azimuthal, self.n_radial)
+            return IntegrationResult(self.radial, i.reshape(shape),
+                                     None if s is None else s.reshape(shape), None,
+                                     self.azimuthal, self.n_radial, self.n_azimuthal)
+        return IntegrationResult(self.radial, i, s, d)
+
+    def integrate1d(self, image, *, errors=False, azimuthal_std=False) -> IntegrationResult:
+        I, S, D = self.integrate_stack(np.asarray(image)[None], errors, azimuthal_std)
+        return self._result(I[0], None if S is None else S[0], None if D is None else D[0])
+
+    integrate2d = integrate1d
+
+    def batch1d(self, images, *, errors=False, azimuthal_std=False) -> list:
+        I, S, D = self.integrate_stack(images, errors, azimuthal_std)
+        return [self._result(I[k], None if S is None else S[k], None if D is None else D[k])
+                for k in range(I.shape[0])]
+
+    batch2d = batch1d
+
+    # -- device-resident path (torch tensors as device memory)
+    def integrate_device(self, frames, out_i, out_s=None, out_d=None, errors=False,
+                         azimuthal_std=False, stream=None):
+        """Enqueues integration of device frames (torch tensor (n, dim1, dim2))
+        into device outputs (n, nbins) float64 tensors on `stream`."""
+        import torch
+        dt = _NP_DTYPES[np.dtype(str(frames.dtype).replace("torch.", ""))]
+        flags = (N.WITH_ERRORS if errors else 0) | (N.WITH_AZIMUTHAL_STD if azimuthal_std else 0)
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        N.check(N.lib().dio_integrate_device(
+            self._h, frames.data_ptr(), N.DTYPES[dt], frames.shape[0], flags, out_i.data_ptr(),
+            None if out_s is None else out_s.data_ptr(),
+            None if out_d is None else out_d.data_ptr(), s))
+
+    def autotune(self, dtype="uint32", n_frames=64, errors=False) -> int:
+        t = C.c_uint32()
+        N.check(N.lib().dio_plan_autotune(self._h, N.DTYPES[dtype], n_frames,
+                                          N.WITH_ERRORS if errors else 0, C.byref(t)))
+        return t.value
+
+    def set_tile(self, frames_per_tile: int):
+        N.check(N.lib().dio_plan_set_tile(self._h, frames_per_tile))
+
+    def time_gather(self, frames, errors=False, reps=20) -> float:
+        """Average ms of one launch of the segment kernel over one tile."""
+        dt = _NP_DTYPES[np.dtype(str(frames.dtype).replace("torch.", ""))]
+        ms = C.c_double()
+        N.check(N.lib().dio_plan_time_gather(self._h, frames.data_ptr(), N.DTYPES[dt],
+                                             frames.shape[0], N.WITH_ERRORS if errors else 0,
+                                             None, None, reps, C.byref(ms)))
+        return ms.value
+
+    def capture_graph(self, frames, out_i, out_s=None, out_d=None, errors=False,
+                      azimuthal_std=False):
+        return GpuGraph(self, frames, out_i, out_s, out_d, errors, azimuthal_std)
+
+
+class GpuGraph:
+    """hipGraph of one device-resident integration, replayed by launch()."""
+
+    def __init__(self, plan: GpuPlan, frames, out_i, out_s, out_d, errors, azimuthal_std):
+        dt = _NP_DTYPES[np.dtype(str(frames.dtype).replace("torch.", ""))]
+        flags = (N.WITH_ERRORS if errors else 0) | (N.WITH_AZIMUTHAL_STD if azimuthal_std else 0)
+        h = C.c_void_p()
+        N.check(N.lib().dio_graph_create(
+            plan._h, frames.data_ptr(), N.DTYPES[dt], frames.shape[0], flags, out_i.data_ptr(),
+            None if out_s is None else out_s.data_ptr(),
+            None if out_d is None else out_d.data_ptr(), C.byref(h)))
+        self._h, self.plan = h, plan
+        self._keep = (frames, out_i, out_s, out_d)
+
+    def launch(self, stream=None):
+        N.check(N.lib().dio_graph_launch(self._h, stream))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().dio_graph_destroy(h)
+            self._h = None
+
+
+class Integrator:
+    """Radial integration configured like the reference Integrator.
+
+    Every integration builds (once per shape and key, like the reference's
+    single-entry caches, cache.rs:42-96) a GpuPlan and runs on the GPU.
+    """
+
+    def __init__(self, poni_path: Union[str, dict], method: Optional[str] = None,
+                 polarization_factor: Optional[float] = None, n_ss: Optional[int] = None,
+                 unit: Optional[str] = None, radial_range=None, *,
+                 precision: str = "f32_compensated", device: int = 0):
+        self.poni = dict(poni_path) if isinstance(poni_path, dict) else load_poni(poni_path)
+        self.method = method or "pixel_split"
+        if self.method not in N.METHODS:
+            raise ValueError(f"unknown method {self.method!r}")
+        if n_ss is not None and n_ss <= 0:
+            raise ValueError("n_ss must be positive")
+        self.n_ss = n_ss if n_ss is not None else 5
+        self.unit = unit or "2th_rad"
+        if self.unit not in N.UNITS:
+            raise ValueError(f"unknown unit {self.unit!r}")
+        if polarization_factor is not None and not -1.0 <= polarization_factor <= 1.0:
+            raise ValueError("polarization_factor must lie in [-1, 1]")
+        self.polarization_factor = polarization_factor
+        if radial_range is not None:
+            lo, hi = map(float, radial_range)
+            if not (np.isfinite(lo) and np.isfinite(hi) and lo < hi):
+                raise ValueError("radial_range must be finite with min < max")
+            radial_range = (lo, hi)
+        self.radial_range = radial_range
+        self.precision = precision
+        self.device = device
+        self.mask = None
+        self._plans: dict = {}
+
+    @classmethod
+    def from_poni_dict(cls, poni_dict: dict, *args, **kwargs):
+        return cls(poni_dict, *args, **kwargs)
+
+    # -- setters mirroring the reference (each invalidates cached plans)
+    def set_method(self, method: str, n_ss: Optional[int] = None):
+        if method not in N.METHODS:
+            raise ValueError(f"unknown method {method!r}")
+        self.method = method
+        if n_ss is not None:
+            self.n_ss = n_ss
+        self._plans.clear()
+
+    def set_polarization_factor(self, factor: Optional[float]):
+        self.polarization_factor = factor
+        self._plans.clear()
+
+    def set_unit(self, unit: str):
+        if unit not in N.UNITS:
+            raise ValueError(f"unknown unit {unit!r}")
+        self.unit = unit
+        self._plans.clear()
+
+    def set_mask(self, mask: Optional[np.ndarray]):
+        self.mask = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        self._plans.clear()
+
+    def clear_mask(self):
+        self.set_mask(None)
+
+    def get_unit(self) -> str:
+        return self.unit
+
+    def _plan(self, shape, n_radial, n_azimuthal=None, azimuthal_range=None, precision=None):
+        key = (tuple(shape), n_radial, n_azimuthal, azimuthal_range, precision or self.precision)
+        plan = self._plans.get(key)
+        if plan is None:
+            if self.mask is not None and self.mask.shape != tuple(shape):
+                raise ValueError(f"image shape {tuple(shape)} does not match mask shape "
+                                 f"{self.mask.shape}")
+            plan = GpuPlan(self.poni, shape, self.method, self.n_ss, self.unit, n_radial,
+                           n_azimuthal, self.radial_range, azimuthal_range,
+                           self.polarization_factor, self.mask, precision or self.precision,
+                           self.device)
+            self._plans = {key: plan}  # single entry, like the reference caches
+        return plan
+
+    def prepare_gpu1d(self, shape, n_points: int, *, precision: str = "f32_compensated",
+                      device: Optional[int] = None) -> GpuPlan:
+        """prepare_cuda1d counterpart: an independent plan snapshot."""
+        return GpuPlan(self.poni, shape, self.method, self.n_ss, self.unit, n_points, None,
+                       self.radial_range, None, self.polarization_factor, self.mask, precision,
+                       self.device if device is None else device)
+
+    def prepare_gpu2d(self, shape, n_radial: int, n_azimuthal: int, *, azimuthal_range=None,
+                      precision: str = "f32_compensated", device: Optional[int] = None) -> GpuPlan:
+        return GpuPlan(self.poni, shape, self.method, self.n_ss, self.unit, n_radial,
+                       n_azimuthal, self.radial_range, azimuthal_range, self.polarization_factor,
+                       self.mask, precision, self.device if device is None else device)
+
+    def integrate1d(self, image: np.ndarray, n_points: int, errors: bool = False, *,
+                    azimuthal_std: bool = False) -> IntegrationResult:
+        if n_points <= 0:
+            raise ValueError("n_points must be positive")
+        image = np.asarray(image)
+        return self._plan(image.shape, n_points).integrate1d(image, errors=errors,
+                                                             azimuthal_std=azimuthal_std)
+
+    def integrate1d_with_errors(self, image, n_points):
+        return self.integrate1d(image, n_points, True)
+
+    def integrate1d_with_azimuthal_std(self, image, n_points, errors=False):
+        return self.integrate1d(image, n_points, errors, azimuthal_std=True)
+
+    def integrate2d(self, image: np.ndarray, n_radial: int, n_azimuthal: int,
+                    errors: bool = False, azimuthal_range=None) -> IntegrationResult:
+        image = np.asarray(image)
+        return self._plan(image.shape, n_radial, n_azimuthal, azimuthal_range).integrate1d(
+            image, errors=errors)
+
+    def batch1d(self, images, n_points: int, errors: bool = False, num_workers=None,
+                num_threads=None, *, azimuthal_std: bool = False) -> list:
+        """batch1d over arrays: per-frame results in input order; a frame that
+        cannot be integrated reports its error without failing the others."""
+        if n_points <= 0:
+            raise ValueError("n_points must be positive")
+        return self._batch(images, lambda shape: self._plan(shape, n_points), errors,
+                           azimuthal_std)
+
+    def batch2d(self, images, n_radial: int, n_azimuthal: int, errors: bool = False,
+                azimuthal_range=None, num_workers=None, num_threads=None) -> list:
+        return self._batch(images, lambda shape: self._plan(shape, n_radial, n_azimuthal,
+                                                           azimuthal_range), errors, False)
+
+    def _batch(self, images, plan_for, errors, azimuthal_std):
+        images = list(images) if not (isinstance(images, np.ndarray) and images.ndim == 3) \
+            else [images[i] for i in range(images.shape[0])]
+        out: list = [None] * len(images)
+        groups: dict = {}
+        for i, im in enumerate(images):
+            im = np.asarray(im)
+            groups.setdefault((im.shape, im.dtype.str), []).append(i)
+        for (shape, _), idx in groups.items():
+            try:
+                plan = plan_for(shape)
+                res = plan.batch1d([images[i] for i in idx], errors=errors,
+                                   azimuthal_std=azimuthal_std)
+                for i, r in zip(idx, res):
+                    out[i] = BatchResult(f"array[{i}]", r, None)
+            except (ValueError, TypeError) as e:
+                for i in idx:
+                    out[i] = BatchResult(f"array[{i}]", None, str(e))
+        return out

@@ -1,0 +1,168 @@
+/* dioptrin_hip: C ABI of the MI355X-native Dioptrin integration hot path.
+ *
+ * Plain C types only: handles, pointers and sizes. Every entry returns a
+ * status (0 = ok) and leaves a thread-local message for dio_last_error().
+ * Status codes map one to one onto the reference's CudaError categories
+ * (src/integration/cuda.rs:218-237).
+ *
+ * Reference interfaces each entry replaces (paths relative to the reference
+ * repository root) are cited per declaration.
+ */
+#ifndef DIOPTRIN_HIP_H
+#define DIOPTRIN_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (CudaError: NotCompiled/Unavailable, Unsupported,
+ * InvalidInput, Device). */
+enum {
+  DIO_OK = 0,
+  DIO_ERR_UNAVAILABLE = 1,
+  DIO_ERR_UNSUPPORTED = 2,
+  DIO_ERR_INVALID = 3,
+  DIO_ERR_DEVICE = 4
+};
+
+/* IntegrationMethod (src/integration.rs:227-237). */
+enum { DIO_COUNTING = 0, DIO_PIXEL_SPLIT = 1, DIO_PYFAI_FULL = 2, DIO_SUPERSAMPLED = 3 };
+/* RadialUnit (src/integration.rs:115-126). */
+enum { DIO_UNIT_TTH_RAD = 0, DIO_UNIT_TTH_DEG = 1, DIO_UNIT_Q_A = 2, DIO_UNIT_Q_NM = 3 };
+/* Accumulation precision (CudaPrecision, src/integration/cuda.rs:64-73), plus
+ * DIO_F32_WIDE: float32 storage with float64 register accumulation, which on
+ * CDNA costs one full-rate DFMA per term. */
+enum { DIO_F32 = 0, DIO_F32_COMPENSATED = 1, DIO_F64 = 2, DIO_F32_WIDE = 3 };
+/* Frame element types (CudaPixel, src/integration/cuda.rs:354-367). */
+enum { DIO_U16 = 0, DIO_U32 = 1, DIO_I32 = 2, DIO_F32T = 3, DIO_F64T = 4 };
+/* Output selection flags. */
+enum { DIO_WITH_ERRORS = 1, DIO_WITH_AZIMUTHAL_STD = 2 };
+
+typedef struct dio_ctx dio_ctx;
+typedef struct dio_plan dio_plan;
+typedef struct dio_graph dio_graph;
+
+/* Poni (src/poni.rs:13-24): metres and radians. */
+typedef struct {
+  double pixel1, pixel2, distance, poni1, poni2, rot1, rot2, rot3, wavelength;
+} dio_poni;
+
+/* What a plan is built from: the Integrator settings that key the
+ * reference's table caches (OneDKey/TwoDKey, src/integration/cache.rs:23-36)
+ * plus the image shape, mask and corrections of its snapshot. */
+typedef struct {
+  uint32_t dim1, dim2;          /* image rows, columns */
+  int32_t method;               /* DIO_COUNTING .. DIO_SUPERSAMPLED */
+  uint32_t n_ss;                /* subpixels per side for DIO_SUPERSAMPLED */
+  int32_t unit;                 /* DIO_UNIT_* */
+  uint32_t ndim;                /* 1 or 2 */
+  uint32_t n_radial;            /* bins (1D) or radial bins (2D) */
+  uint32_t n_azimuthal;         /* 2D only */
+  int32_t has_radial_range;
+  double radial_min, radial_max;
+  int32_t has_azimuthal_range;  /* 2D only */
+  double azimuthal_min, azimuthal_max;
+  int32_t has_polarization;
+  double polarization_factor;
+  const uint8_t *mask;          /* host, dim1*dim2, nonzero = masked; may be NULL */
+  int32_t precision;            /* DIO_F32 .. DIO_F32_WIDE */
+} dio_plan_desc;
+
+typedef struct {
+  uint32_t nbins;               /* rows of the table (n_radial * n_azimuthal in 2D) */
+  uint32_t n_radial, n_azimuthal;
+  uint64_t nnz;                 /* entries of the unmasked table */
+  uint64_t nnz_masked;          /* entries of the execution table */
+  uint32_t weighted;            /* 0 for counting tables (unit weights) */
+  uint32_t n_segments;          /* gather work units */
+  uint64_t uncertain_angles;    /* angles recomputed with host libm */
+  uint32_t singular_pixels;     /* beam pixels integrated by sampling */
+  double build_ms;              /* plan build wall time */
+  double hist_min, hist_step;   /* 1D radial axis */
+  uint64_t device_bytes;        /* device memory held by the plan */
+} dio_plan_info;
+
+typedef struct {
+  char name[256];
+  char arch[64];
+  int32_t compute_units;
+  int32_t xcds;
+  uint64_t total_memory;
+  int32_t runtime_version;
+} dio_device_info;
+
+/* Thread-local text of the last failure on this thread. */
+const char *dio_last_error(void);
+const char *dio_version(void);
+
+/* Device context, cached per ordinal (device.rs:142-176 Cuda::open/device). */
+int dio_open(int device, dio_ctx **out);
+int dio_close(dio_ctx *ctx);
+/* cuda_info (src/integration/cuda.rs:309-326). */
+int dio_device_info_get(dio_ctx *ctx, dio_device_info *out);
+
+/* Integrator::prepare_cuda1d / prepare_cuda2d (src/integration/cuda.rs:1089,
+ * 1111): builds the lookup table ON THE DEVICE (corner geometry, splitting,
+ * count-then-fill CSR, mask compaction, FP64 denominators) and keeps it
+ * resident. The plan is a snapshot like Cuda1dPlan (cuda.rs:378-386). */
+int dio_plan_create(dio_ctx *ctx, const dio_poni *poni, const dio_plan_desc *desc,
+                    dio_plan **out);
+int dio_plan_destroy(dio_plan *plan);
+/* Cuda1dPlan::{n_points, table_entries, ...} (cuda.rs:411-470). */
+int dio_plan_info_get(dio_plan *plan, dio_plan_info *out);
+/* Radial bin centres (Cuda1dPlan::radial, cuda.rs:440; histogram::bins). */
+int dio_plan_radial(dio_plan *plan, double *out);
+/* Azimuthal bin centres of a 2D plan (Cuda2dPlan::azimuthal). */
+int dio_plan_azimuthal(dio_plan *plan, double *out);
+/* Copies the table for bit-exact checks: masked = 0 gives the builder's
+ * Lut1d/Lut2d (counting_csr.rs:13-40), masked = 1 the execution table
+ * (prepared.rs:163-201). Any output pointer may be NULL. Capacities are in
+ * elements; row_ptr needs nbins + 1. */
+int dio_plan_export_csr(dio_plan *plan, int masked, uint32_t *row_ptr, uint32_t *col_idx,
+                        double *w64, float *w32, uint64_t capacity);
+/* Strict FP64 denominators of the execution table (prepared.rs:229-262). */
+int dio_plan_denominators(dio_plan *plan, double *out);
+/* Per-pixel normalization (polarization * solid angle, corrections.rs:57-93). */
+int dio_plan_normalization(dio_plan *plan, double *out);
+
+/* Cuda1dPlan::integrate_frames / integrate_stack and Cuda2dPlan
+ * equivalents (cuda.rs:480-548, 770-863): host frames in, host results out.
+ * frames: n_frames contiguous frames of dtype. intensity/sigma/std:
+ * n_frames * nbins float64 each (sigma/std only with their flag).
+ * Synchronous; frames are borrowed for the call. */
+int dio_integrate(dio_plan *plan, const void *frames, int dtype, uint64_t n_frames,
+                  uint32_t flags, double *intensity, double *sigma, double *std_out);
+
+/* Device-resident batch: frames and outputs already in device memory
+ * (the batch1d/batch2d hot path). Enqueued on `stream` (a hipStream_t,
+ * NULL = the plan's stream); returns without synchronizing. */
+int dio_integrate_device(dio_plan *plan, const void *d_frames, int dtype, uint64_t n_frames,
+                         uint32_t flags, double *d_intensity, double *d_sigma, double *d_std,
+                         void *stream);
+
+/* Frames per tile of the fused batch kernel, from a short device sweep
+ * (autotuner). The choice is stored in the plan and used by later calls. */
+int dio_plan_autotune(dio_plan *plan, int dtype, uint64_t n_frames_hint, uint32_t flags,
+                      uint32_t *frames_per_tile);
+int dio_plan_set_tile(dio_plan *plan, uint32_t frames_per_tile);
+
+/* Steady-state replay: captures one dio_integrate_device call into a
+ * hipGraph and replays it. */
+int dio_graph_create(dio_plan *plan, const void *d_frames, int dtype, uint64_t n_frames,
+                     uint32_t flags, double *d_intensity, double *d_sigma, double *d_std,
+                     dio_graph **out);
+int dio_graph_launch(dio_graph *graph, void *stream);
+int dio_graph_destroy(dio_graph *graph);
+
+/* Kernel-duration probe for roofline measurement: times `reps` launches of
+ * the dominant gather kernel with HIP events on the plan's stream. */
+int dio_plan_time_gather(dio_plan *plan, const void *d_frames, int dtype, uint64_t n_frames,
+                         uint32_t flags, double *d_intensity, double *d_sigma, int reps,
+                         double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,715 @@
+/* FP64 CPU oracle of the Dioptrin integration hot path (test infrastructure
+ * only; see oracle.h). Build: -O2 -ffp-contract=off -fno-fast-math. */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_PI 3.14159265358979311599796346854418516159057617187500
+#define OR_TAU 6.28318530717958623199592693708837032318115234375000
+#define OR_EPS DBL_EPSILON
+
+/* ---------------------------------------------------------------- helpers */
+
+/* Rust `f64 as isize` / `as usize`: saturating, NaN -> 0. */
+static long long sat_isize(double v) {
+  if (v != v) return 0;
+  if (v >= 9223372036854775807.0) return 9223372036854775807LL;
+  if (v <= -9223372036854775808.0) return (-9223372036854775807LL - 1);
+  return (long long)v;
+}
+static unsigned long long sat_usize(double v) {
+  if (!(v > 0.0)) return 0; /* negative, zero or NaN */
+  if (v >= 18446744073709551615.0) return 18446744073709551615ULL;
+  return (unsigned long long)v;
+}
+
+/* Rust f64::rem_euclid: r = a % b; r < 0 ? r + |b| : r. */
+static double rem_euclid(double a, double b) {
+  double r = fmod(a, b);
+  return r < 0.0 ? r + fabs(b) : r;
+}
+
+/* pyfai_1d.rs:180-186: x.rem_euclid(TAU), returning values already in
+ * [0, TAU) unchanged. */
+static double wrap_tau(double x) {
+  if (x >= 0.0 && x < OR_TAU) return x;
+  return rem_euclid(x, OR_TAU);
+}
+
+/* src/math/mod.rs:8-32: first element, then strict comparisons. */
+static double vmin(const double *v, size_t n) {
+  double m = v[0];
+  for (size_t i = 1; i < n; i++)
+    if (v[i] < m) m = v[i];
+  return m;
+}
+static double vmax(const double *v, size_t n) {
+  double m = v[0];
+  for (size_t i = 1; i < n; i++)
+    if (v[i] > m) m = v[i];
+  return m;
+}
+
+/* Growable per-bin entry list. */
+typedef struct {
+  uint64_t *col;
+  double *w;
+  size_t len, cap;
+} bucket;
+
+static void bucket_push(bucket *b, uint64_t col, double w) {
+  if (b->len == b->cap) {
+    size_t cap = b->cap ? 2 * b->cap : 8;
+    b->col = (uint64_t *)realloc(b->col, cap * sizeof(uint64_t));
+    b->w = (double *)realloc(b->w, cap * sizeof(double));
+    b->cap = cap;
+  }
+  b->col[b->len] = col;
+  b->w[b->len] = w;
+  b->len++;
+}
+
+/* pixel_split.rs:332-356: flatten per-bin lists to CSR in bin order. */
+static void buckets_to_lut(bucket *bk, size_t n, int weighted, double min, double step,
+                           or_lut *out) {
+  out->nbins = n;
+  out->row_ptr = (uint64_t *)malloc((n + 1) * sizeof(uint64_t));
+  out->row_ptr[0] = 0;
+  for (size_t i = 0; i < n; i++) out->row_ptr[i + 1] = out->row_ptr[i] + bk[i].len;
+  size_t nnz = out->row_ptr[n];
+  out->nnz = nnz;
+  out->col = (uint64_t *)malloc((nnz ? nnz : 1) * sizeof(uint64_t));
+  out->w = weighted ? (double *)malloc((nnz ? nnz : 1) * sizeof(double)) : NULL;
+  size_t k = 0;
+  for (size_t i = 0; i < n; i++) {
+    for (size_t e = 0; e < bk[i].len; e++, k++) {
+      out->col[k] = bk[i].col[e];
+      if (weighted) out->w[k] = bk[i].w[e];
+    }
+    free(bk[i].col);
+    free(bk[i].w);
+  }
+  out->hist_min = min;
+  out->hist_step = step;
+}
+
+void or_lut_free(or_lut *lut) {
+  free(lut->row_ptr);
+  free(lut->col);
+  free(lut->w);
+  memset(lut, 0, sizeof(*lut));
+}
+
+/* --------------------------------------------------------------- geometry */
+
+typedef struct {
+  double s1, c1, s2, c2, s3, c3;
+} trig;
+
+/* geometry.rs:640-660 (TrigonometricConstants::new). */
+static trig make_trig(const or_poni *p) {
+  trig t;
+  t.s1 = sin(p->rot1);
+  t.c1 = cos(p->rot1);
+  t.s2 = sin(p->rot2);
+  t.c2 = cos(p->rot2);
+  t.s3 = sin(p->rot3);
+  t.c3 = cos(p->rot3);
+  return t;
+}
+
+/* geometry.rs:663-681, evaluated left to right exactly as written. */
+static double f_x1(double p1, double p2, double p3, const trig *t) {
+  return p1 * t->c2 * t->c3 + p2 * (t->c3 * t->s1 * t->s2 - t->c1 * t->s3) -
+         p3 * (t->c1 * t->c3 * t->s2 + t->s1 * t->s3);
+}
+static double f_x2(double p1, double p2, double p3, const trig *t) {
+  return p1 * t->c2 * t->s3 + p2 * (t->c1 * t->c3 + t->s1 * t->s2 * t->s3) -
+         p3 * (-(t->c3 * t->s1) + t->c1 * t->s2 * t->s3);
+}
+static double f_x3(double p1, double p2, double p3, const trig *t) {
+  return p1 * t->s2 - p2 * t->c2 * t->s1 + p3 * t->c1 * t->c2;
+}
+/* geometry.rs:684-692. */
+static double f_tth(double x, double y, double z) { return atan2(sqrt(y * y + x * x), z); }
+static double f_azimuth(double x, double y) { return atan2(x, y); }
+
+/* geometry.rs:288-365 (det reference -> poni reference -> sample frame):
+ * p1 = d1 * pixel1 - poni1, p2 = d2 * pixel2 - poni2, p3 = distance. */
+static void point_xyz(const or_poni *p, const trig *t, double d1, double d2, double *x, double *y,
+                      double *z) {
+  double p1 = d1 * p->pixel1 - p->poni1;
+  double p2 = d2 * p->pixel2 - p->poni2;
+  *x = f_x1(p1, p2, p->distance, t);
+  *y = f_x2(p1, p2, p->distance, t);
+  *z = f_x3(p1, p2, p->distance, t);
+}
+
+void or_positions(const or_poni *p, const double *d1, const double *d2, size_t n, double *x,
+                  double *y, double *z) {
+  trig t = make_trig(p);
+  for (size_t i = 0; i < n; i++) point_xyz(p, &t, d1[i], d2[i], &x[i], &y[i], &z[i]);
+}
+
+/* geometry.rs:519-549 (calc_tth_and_azimuth). */
+void or_tth_chi(const or_poni *p, const double *d1, const double *d2, size_t n, double *tth,
+                double *chi) {
+  trig t = make_trig(p);
+  for (size_t i = 0; i < n; i++) {
+    double x, y, z;
+    point_xyz(p, &t, d1[i], d2[i], &x, &y, &z);
+    if (tth) tth[i] = f_tth(x, y, z);
+    if (chi) chi[i] = f_azimuth(x, y);
+  }
+}
+
+/* geometry.rs:229-248, 265-278: corner grid d1 = i, d2 = j. */
+void or_corners(const or_poni *p, size_t dim1, size_t dim2, double *tth, double *chi) {
+  trig t = make_trig(p);
+  size_t k = 0;
+  for (size_t i = 0; i <= dim1; i++)
+    for (size_t j = 0; j <= dim2; j++, k++) {
+      double x, y, z;
+      point_xyz(p, &t, (double)i, (double)j, &x, &y, &z);
+      tth[k] = f_tth(x, y, z);
+      chi[k] = f_azimuth(x, y);
+    }
+}
+
+/* geometry.rs:308-320, 445-487: centres d1 = i + 0.5, d2 = j + 0.5. */
+void or_centres(const or_poni *p, size_t dim1, size_t dim2, double *tth, double *chi) {
+  trig t = make_trig(p);
+  size_t k = 0;
+  for (size_t i = 0; i < dim1; i++)
+    for (size_t j = 0; j < dim2; j++, k++) {
+      double x, y, z;
+      point_xyz(p, &t, (double)i + 0.5, (double)j + 0.5, &x, &y, &z);
+      if (tth) tth[k] = f_tth(x, y, z);
+      if (chi) chi[k] = f_azimuth(x, y);
+    }
+}
+
+/* geometry.rs:552-617 (flat detector) and 697-712: cos(a)^3 with
+ * cos(a) = p3 / sqrt(p1^2 + p2^2 + p3^2). */
+void or_solid_angle(const or_poni *p, size_t dim1, size_t dim2, double *out) {
+  size_t k = 0;
+  for (size_t i = 0; i < dim1; i++)
+    for (size_t j = 0; j < dim2; j++, k++) {
+      double p1 = ((double)i + 0.5) * p->pixel1 - p->poni1;
+      double p2 = ((double)j + 0.5) * p->pixel2 - p->poni2;
+      double p3 = p->distance;
+      double cosa = p3 / sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+      out[k] = cosa * cosa * cosa;
+    }
+}
+
+/* geometry.rs:625-636, 715-717. */
+void or_polarization(double factor, const double *tth, const double *chi, size_t n, double *out) {
+  for (size_t i = 0; i < n; i++) {
+    double c = cos(tth[i]);
+    double c2 = c * c;
+    out[i] = 0.5 * (1.0 + c2 - factor * cos(2.0 * chi[i]) * (1.0 - c2));
+  }
+}
+
+/* corrections.rs:57-93: normalization = polarization * solid angle. */
+void or_normalization(const or_poni *p, size_t dim1, size_t dim2, int has_pol, double factor,
+                      double *out) {
+  size_t n = dim1 * dim2;
+  double *sa = (double *)malloc(n * sizeof(double));
+  or_solid_angle(p, dim1, dim2, sa);
+  if (has_pol) {
+    double *tth = (double *)malloc(n * sizeof(double));
+    double *chi = (double *)malloc(n * sizeof(double));
+    or_centres(p, dim1, dim2, tth, chi);
+    or_polarization(factor, tth, chi, n, out);
+    for (size_t i = 0; i < n; i++) out[i] = out[i] * sa[i];
+    free(tth);
+    free(chi);
+  } else {
+    for (size_t i = 0; i < n; i++) out[i] = 1.0 * sa[i];
+  }
+  free(sa);
+}
+
+/* integration.rs:132-187 (RadialConverter::convert). Degrees use Rust's
+ * f64::to_degrees, a product with the correctly rounded 180/pi. */
+void or_convert_unit(int unit, double wavelength, const double *tth, size_t n, double *out) {
+  double q_scale = 0.0;
+  if (unit == OR_UNIT_Q_A || unit == OR_UNIT_Q_NM)
+    q_scale = 4.0 * OR_PI / (wavelength * (unit == OR_UNIT_Q_A ? 1e10 : 1e9));
+  for (size_t i = 0; i < n; i++) {
+    switch (unit) {
+      case OR_UNIT_TTH_RAD: out[i] = tth[i]; break;
+      case OR_UNIT_TTH_DEG: out[i] = tth[i] * (180.0 / OR_PI); break;
+      default: out[i] = q_scale * sin(tth[i] * 0.5); break;
+    }
+  }
+}
+
+void or_bins(size_t n, double min, double step, double *out) {
+  for (size_t i = 0; i < n; i++) out[i] = min + ((double)i + 0.5) * step;
+}
+
+/* ---------------------------------------------------- pixel splitting, 1D */
+
+/* pixel_split.rs:14-25: shoelace sum, then 0.5 *. */
+static double polygon_area(const double (*poly)[2], int n) {
+  if (n < 3) return 0.0;
+  double area = 0.0;
+  for (int i = 0; i < n; i++) {
+    const double *a = poly[i];
+    const double *b = poly[(i + 1) % n];
+    area += a[0] * b[1] - b[0] * a[1];
+  }
+  return 0.5 * area;
+}
+
+/* pixel_split.rs:28-45. */
+static void intersect_x(const double *p0, const double *p1, double xe, double *out) {
+  double dx = p1[0] - p0[0];
+  if (fabs(dx) < OR_EPS) {
+    out[0] = xe;
+    out[1] = p0[1];
+    return;
+  }
+  double t = (xe - p0[0]) / dx;
+  out[0] = xe;
+  out[1] = p0[1] + t * (p1[1] - p0[1]);
+}
+static void intersect_y(const double *p0, const double *p1, double ye, double *out) {
+  double dy = p1[1] - p0[1];
+  if (fabs(dy) < OR_EPS) {
+    out[0] = p0[0];
+    out[1] = ye;
+    return;
+  }
+  double t = (ye - p0[1]) / dy;
+  out[0] = p0[0] + t * (p1[0] - p0[0]);
+  out[1] = ye;
+}
+
+/* One Sutherland-Hodgman stage (pixel_split.rs:48-137). axis 0 = x, 1 = y;
+ * keep_ge: inside means coordinate >= edge, else <= edge. */
+static int clip_stage(const double (*in)[2], int n, int axis, int keep_ge, double edge,
+                      double (*out)[2]) {
+  if (n == 0) return 0;
+  int m = 0;
+  const double *prev = in[n - 1];
+  int prev_in = keep_ge ? (prev[axis] >= edge) : (prev[axis] <= edge);
+  for (int i = 0; i < n; i++) {
+    const double *cur = in[i];
+    int cur_in = keep_ge ? (cur[axis] >= edge) : (cur[axis] <= edge);
+    if (cur_in) {
+      if (!prev_in) {
+        if (axis == 0) intersect_x(prev, cur, edge, out[m]);
+        else intersect_y(prev, cur, edge, out[m]);
+        m++;
+      }
+      out[m][0] = cur[0];
+      out[m][1] = cur[1];
+      m++;
+    } else if (prev_in) {
+      if (axis == 0) intersect_x(prev, cur, edge, out[m]);
+      else intersect_y(prev, cur, edge, out[m]);
+      m++;
+    }
+    prev = cur;
+    prev_in = cur_in;
+  }
+  return m;
+}
+
+/* pixel_split.rs:140-163: left, right, bottom, top; stops at an empty
+ * stage. Returns the vertex count of the clipped polygon in `res`. */
+static int clip_rect(const double (*poly)[2], int n, double xmin, double xmax, double ymin,
+                     double ymax, double (*res)[2]) {
+  double a[24][2], b[24][2];
+  int m = clip_stage(poly, n, 0, 1, xmin, a);
+  if (m == 0) return 0;
+  m = clip_stage((const double(*)[2])a, m, 0, 0, xmax, b);
+  if (m == 0) return 0;
+  m = clip_stage((const double(*)[2])b, m, 1, 1, ymin, a);
+  if (m == 0) return 0;
+  m = clip_stage((const double(*)[2])a, m, 1, 0, ymax, res);
+  return m;
+}
+
+/* pixel_split.rs:166-175. */
+static double unwrap_angle(double angle, double reference) {
+  double a = angle;
+  double delta = a - reference;
+  if (delta > OR_PI) a -= 2.0 * OR_PI;
+  else if (delta < -OR_PI) a += 2.0 * OR_PI;
+  return a;
+}
+
+/* pyfai_1d.rs:188-201: beam pixel test on (radial, chi) corners. */
+static int singular(const double (*poly)[2]) {
+  double winding = 0.0;
+  for (int i = 0; i < 4; i++) {
+    if (poly[i][0] == 0.0) return 1;
+    double delta = wrap_tau(poly[(i + 1) % 4][1] - poly[i][1] + OR_PI) - OR_PI;
+    if (fabs(delta) >= OR_PI - 64.0 * OR_EPS) return 1;
+    winding += delta;
+  }
+  return fabs(winding) > OR_PI;
+}
+
+/* integration/grid.rs:48-108 (IntegrationGrid2d::new) validity, for the
+ * 1D sampling grid (n radial, 1 azimuthal, chi in (-pi, pi)). */
+static int sample_grid_ok(size_t n, double lo, double hi) {
+  double bounds[2][2] = {{lo, hi}, {-OR_PI, OR_PI}};
+  size_t counts[2] = {n, 1};
+  if (n == 0) return 0;
+  for (int k = 0; k < 2; k++) {
+    double l = bounds[k][0], h = bounds[k][1];
+    double step = (h - l) / (double)counts[k];
+    if (!isfinite(l) || !isfinite(h) || !isfinite(step) || step <= 0.0 || l + step == l ||
+        h - step == h)
+      return 0;
+  }
+  if (lo < 0.0) return 0;
+  return 1;
+}
+
+/* pyfai_1d.rs:208-252: 128 x 128 x 4 detector-plane samples of one pixel,
+ * binned on the grid (r0, r1) x (-pi, pi) with n radial bins. Appends
+ * (pixel, hits / 65536) to the bins in ascending bin order. */
+static void sample_pixel(const or_poni *p, const trig *t, size_t dim2, size_t pixel, int unit,
+                         size_t n, double r0, double r1, int clamp_low, bucket *bk) {
+  const int N = 128;
+  const double off[4][2] = {{0.25, 0.5}, {0.5, 0.25}, {0.75, 0.5}, {0.5, 0.75}};
+  const double count = (double)(N * N * 4);
+  unsigned *hits = (unsigned *)calloc(n, sizeof(unsigned));
+  double row0 = (double)(pixel / dim2), col0 = (double)(pixel % dim2);
+  double a0 = -OR_PI, a1 = OR_PI;
+  double q_scale = 0.0;
+  if (unit == OR_UNIT_Q_A || unit == OR_UNIT_Q_NM)
+    q_scale = 4.0 * OR_PI / (p->wavelength * (unit == OR_UNIT_Q_A ? 1e10 : 1e9));
+  for (int row = 0; row < N; row++)
+    for (int col = 0; col < N; col++)
+      for (int o = 0; o < 4; o++) {
+        double d1 = row0 + ((double)row + off[o][0]) / (double)N;
+        double d2 = col0 + ((double)col + off[o][1]) / (double)N;
+        double x, y, z;
+        point_xyz(p, t, d1, d2, &x, &y, &z);
+        double tth = f_tth(x, y, z);
+        double a = f_azimuth(x, y);
+        double r;
+        switch (unit) {
+          case OR_UNIT_TTH_RAD: r = tth; break;
+          case OR_UNIT_TTH_DEG: r = tth * (180.0 / OR_PI); break;
+          default: r = q_scale * sin(tth * 0.5); break;
+        }
+        a = a0 + rem_euclid(a - a0, OR_TAU);
+        if ((r < r0 && !clamp_low) || r >= r1 || a >= a1) continue;
+        unsigned long long ir = r < r0 ? 0 : sat_usize((r - r0) / (r1 - r0) * (double)n);
+        unsigned long long ia = sat_usize((a - a0) / (a1 - a0) * 1.0);
+        if (ir < n && ia < 1) hits[ia * n + ir]++;
+      }
+  for (size_t b = 0; b < n; b++)
+    if (hits[b]) bucket_push(&bk[b], pixel, (double)hits[b] / count);
+  free(hits);
+}
+
+/* pixel_split.rs:190-328 (lut1d_ps_fractions), serial and in pixel order,
+ * which is the order the reference's fold/reduce assembles. */
+static void ps_fractions(const or_poni *p, size_t dim1, size_t dim2, const double *radial_c,
+                         const double *chi_c, int unit, size_t n, double min, double step,
+                         const double *rc, const double *ac, int clamp_low, bucket *bk) {
+  trig t = make_trig(p);
+  double glo = min, ghi = min + (double)n * step;
+  int grid_ok = sample_grid_ok(n, glo, ghi);
+  size_t w = dim2 + 1;
+  size_t npix = dim1 * dim2;
+  for (size_t idx = 0; idx < npix; idx++) {
+    size_t b = idx / dim2 * w + idx % dim2;
+    size_t i00 = b, i01 = b + 1, i11 = b + w + 1, i10 = b + w;
+    double r00 = rc[i00], r01 = rc[i01], r10 = rc[i10], r11 = rc[i11];
+    if (grid_ok) {
+      double raw[4][2] = {{rc[i00], ac[i00]}, {rc[i01], ac[i01]}, {rc[i11], ac[i11]},
+                          {rc[i10], ac[i10]}};
+      if (singular((const double(*)[2])raw)) {
+        sample_pixel(p, &t, dim2, idx, unit, n, glo, ghi, clamp_low, bk);
+        continue;
+      }
+    }
+    double ca = chi_c[idx];
+    double a00 = unwrap_angle(ac[i00], ca);
+    double a01 = unwrap_angle(ac[i01], ca);
+    double a10 = unwrap_angle(ac[i10], ca);
+    double a11 = unwrap_angle(ac[i11], ca);
+    double poly[4][2] = {{r00, a00}, {r01, a01}, {r11, a11}, {r10, a10}};
+    double area_total = fabs(polygon_area((const double(*)[2])poly, 4));
+    if (area_total <= 0.0) {
+      double pos = (radial_c[idx] - min) / step;
+      if (pos < 0.0) continue;
+      long long bin = sat_isize(floor(pos));
+      if (bin >= 0 && (unsigned long long)bin < n) bucket_push(&bk[bin], idx, 1.0);
+      continue;
+    }
+    double rmin = r00, rmax = r00;
+    double rv[3] = {r01, r10, r11};
+    for (int k = 0; k < 3; k++) {
+      if (rv[k] < rmin) rmin = rv[k];
+      if (rv[k] > rmax) rmax = rv[k];
+    }
+    long long start = sat_isize(floor((rmin - min) / step));
+    long long end = sat_isize(floor((rmax - min) / step));
+    if (end < 0 || start >= (long long)n) continue;
+    if (start < 0) start = 0;
+    if (end >= (long long)n) end = (long long)n - 1;
+    double amin = a00, amax = a00;
+    double av[3] = {a01, a10, a11};
+    for (int k = 0; k < 3; k++) {
+      if (av[k] < amin) amin = av[k];
+      if (av[k] > amax) amax = av[k];
+    }
+    for (long long bb = start; bb <= end; bb++) {
+      double bin_min = min + (double)bb * step;
+      double bin_max = bin_min + step;
+      double clipped[24][2];
+      int m = clip_rect((const double(*)[2])poly, 4, bin_min, bin_max, amin, amax, clipped);
+      double area = fabs(polygon_area((const double(*)[2])clipped, m));
+      if (area > 0.0) bucket_push(&bk[bb], idx, area / area_total);
+    }
+  }
+}
+
+/* pixel_split.rs:389-455 (build_lut1d_ps_csr and _with_range). */
+int or_build_lut1d_ps(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, int unit,
+                      int has_range, double rmin, double rmax, or_lut *out) {
+  if (n_points == 0 || dim1 == 0 || dim2 == 0) return 1;
+  size_t nc = (dim1 + 1) * (dim2 + 1), npix = dim1 * dim2;
+  double *tc = (double *)malloc(nc * sizeof(double));
+  double *ac = (double *)malloc(nc * sizeof(double));
+  double *rc = (double *)malloc(nc * sizeof(double));
+  double *tth = (double *)malloc(npix * sizeof(double));
+  double *chi = (double *)malloc(npix * sizeof(double));
+  double *rad = (double *)malloc(npix * sizeof(double));
+  or_corners(p, dim1, dim2, tc, ac);
+  or_convert_unit(unit, p->wavelength, tc, nc, rc);
+  or_centres(p, dim1, dim2, tth, chi);
+  or_convert_unit(unit, p->wavelength, tth, npix, rad);
+  double min, max, step;
+  if (has_range) {
+    /* histogram.rs:184-188 */
+    min = rmin;
+    max = rmax + 10.0 * OR_EPS;
+  } else {
+    min = vmin(rc, nc);
+    max = vmax(rc, nc);
+    max += 10.0 * OR_EPS;
+  }
+  step = (max - min) / (double)n_points;
+  bucket *bk = (bucket *)calloc(n_points, sizeof(bucket));
+  ps_fractions(p, dim1, dim2, rad, chi, unit, n_points, min, step, rc, ac, !has_range, bk);
+  buckets_to_lut(bk, n_points, 1, min, step, out);
+  free(bk);
+  free(tc);
+  free(ac);
+  free(rc);
+  free(tth);
+  free(chi);
+  free(rad);
+  return 0;
+}
+
+/* counting_csr.rs:54-134 (build_lut1d_csr and _with_range) on given radial
+ * values, with the bin rules of histogram.rs:126-141, 184-196. */
+int or_counting_from_radial(const double *rad, size_t npix, size_t n_points, int has_range,
+                            double rmin, double rmax, or_lut *out) {
+  if (n_points == 0 || npix == 0) return 1;
+  double min, max, step;
+  if (has_range) {
+    min = rmin;
+    max = rmax + 10.0 * OR_EPS;
+  } else {
+    min = vmin(rad, npix);
+    max = vmax(rad, npix) + 10.0 * OR_EPS;
+  }
+  step = (max - min) / (double)n_points;
+  bucket *bk = (bucket *)calloc(n_points, sizeof(bucket));
+  for (size_t i = 0; i < npix; i++) {
+    double r = rad[i];
+    if (has_range && !(r >= min && r <= max)) continue;
+    unsigned long long bin = sat_usize((r - min) / step);
+    if (bin > n_points - 1) bin = n_points - 1;
+    bucket_push(&bk[bin], i, 1.0);
+  }
+  buckets_to_lut(bk, n_points, 0, min, step, out);
+  free(bk);
+  return 0;
+}
+
+int or_build_lut1d_counting(const or_poni *p, size_t dim1, size_t dim2, size_t n_points,
+                            int unit, int has_range, double rmin, double rmax, or_lut *out) {
+  if (n_points == 0 || dim1 == 0 || dim2 == 0) return 1;
+  size_t npix = dim1 * dim2;
+  double *tth = (double *)malloc(npix * sizeof(double));
+  double *rad = (double *)malloc(npix * sizeof(double));
+  or_centres(p, dim1, dim2, tth, NULL);
+  or_convert_unit(unit, p->wavelength, tth, npix, rad);
+  int rc = or_counting_from_radial(rad, npix, n_points, has_range, rmin, rmax, out);
+  free(tth);
+  free(rad);
+  return rc;
+}
+
+/* pixel_split.rs:190-328 on an explicit (min, step), as the reference's
+ * tests call it (lut1d_ps_fractions). */
+int or_ps_fractions(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, int unit,
+                    double min, double step, int clamp_low, or_lut *out) {
+  if (n_points == 0 || dim1 == 0 || dim2 == 0) return 1;
+  size_t nc = (dim1 + 1) * (dim2 + 1), npix = dim1 * dim2;
+  double *tc = (double *)malloc(nc * sizeof(double));
+  double *ac = (double *)malloc(nc * sizeof(double));
+  double *rc = (double *)malloc(nc * sizeof(double));
+  double *tth = (double *)malloc(npix * sizeof(double));
+  double *chi = (double *)malloc(npix * sizeof(double));
+  double *rad = (double *)malloc(npix * sizeof(double));
+  or_corners(p, dim1, dim2, tc, ac);
+  or_convert_unit(unit, p->wavelength, tc, nc, rc);
+  or_centres(p, dim1, dim2, tth, chi);
+  or_convert_unit(unit, p->wavelength, tth, npix, rad);
+  bucket *bk = (bucket *)calloc(n_points, sizeof(bucket));
+  ps_fractions(p, dim1, dim2, rad, chi, unit, n_points, min, step, rc, ac, clamp_low, bk);
+  buckets_to_lut(bk, n_points, 1, min, step, out);
+  free(bk);
+  free(tc);
+  free(ac);
+  free(rc);
+  free(tth);
+  free(chi);
+  free(rad);
+  return 0;
+}
+
+/* --------------------------------------------------- execution and gather */
+
+/* prepared.rs:163-193: entries of masked pixels dropped, order kept. An
+ * all-zero mask keeps the table as is (copied here). */
+void or_fixed_table(const or_lut *in, const uint8_t *mask, or_lut *out) {
+  out->nbins = in->nbins;
+  out->hist_min = in->hist_min;
+  out->hist_step = in->hist_step;
+  out->row_ptr = (uint64_t *)malloc((in->nbins + 1) * sizeof(uint64_t));
+  out->col = (uint64_t *)malloc((in->nnz ? in->nnz : 1) * sizeof(uint64_t));
+  out->w = in->w ? (double *)malloc((in->nnz ? in->nnz : 1) * sizeof(double)) : NULL;
+  size_t k = 0;
+  out->row_ptr[0] = 0;
+  for (size_t r = 0; r < in->nbins; r++) {
+    for (uint64_t e = in->row_ptr[r]; e < in->row_ptr[r + 1]; e++) {
+      uint64_t j = in->col[e];
+      if (mask && mask[j]) continue;
+      out->col[k] = j;
+      if (in->w) out->w[k] = in->w[e];
+      k++;
+    }
+    out->row_ptr[r + 1] = k;
+  }
+  out->nnz = k;
+}
+
+/* prepared.rs:229-262 with row<FAST=false, NORM> (1011-1121): sequential
+ * n = n + w * norm[j] in row order (n + norm[j] for unweighted tables). */
+void or_denominators(const or_lut *lut, const double *norm, double *den) {
+  for (size_t r = 0; r < lut->nbins; r++) {
+    double n = 0.0;
+    for (uint64_t e = lut->row_ptr[r]; e < lut->row_ptr[r + 1]; e++) {
+      uint64_t j = lut->col[e];
+      if (lut->w) n = n + lut->w[e] * norm[j];
+      else n = n + norm[j];
+    }
+    den[r] = n;
+  }
+}
+
+/* prepared.rs:985-997 (normalize_bin) and 1011-1121 (strict row). */
+void or_gather(const or_lut *lut, const double *den, const double *signal, int with_var,
+               double *intensity, double *sigma) {
+  for (size_t r = 0; r < lut->nbins; r++) {
+    double s = 0.0, v = 0.0;
+    for (uint64_t e = lut->row_ptr[r]; e < lut->row_ptr[r + 1]; e++) {
+      double x = signal[lut->col[e]];
+      if (lut->w) {
+        double w = lut->w[e];
+        s = s + w * x;
+        if (with_var) v = v + w * w * fmax(x, 0.0);
+      } else {
+        s += x;
+        if (with_var) v += fmax(x, 0.0);
+      }
+    }
+    double n = den[r];
+    intensity[r] = n == 0.0 ? 0.0 : s / n;
+    if (with_var && sigma) {
+      double vv = n == 0.0 ? 0.0 : v / (n * n);
+      sigma[r] = sqrt(fmax(vv, 0.0));
+    }
+  }
+}
+
+/* azimuthal.rs:48-118. Invalid contributions are skipped; origin is the
+ * first valid corrected value; empty rows give (0, 0, NaN). */
+void or_azimuthal_std(const or_lut *lut, const uint8_t *mask, const double *norm,
+                      const double *signal, int errors, double *intensity, double *sigma,
+                      double *std_out) {
+  for (size_t r = 0; r < lut->nbins; r++) {
+    int found = 0;
+    double origin = 0.0, sum_signal = 0.0, sum_weight = 0.0, sum_shifted = 0.0, sum_var = 0.0;
+    for (int pass = 0; pass < 2; pass++) {
+      for (uint64_t e = lut->row_ptr[r]; e < lut->row_ptr[r + 1]; e++) {
+        uint64_t j = lut->col[e];
+        if (mask && mask[j]) continue;
+        double n = norm[j];
+        double c = lut->w ? lut->w[e] : 1.0;
+        double s = signal[j];
+        double x = s / n;
+        double w = c * n;
+        if (!(isfinite(s) && isfinite(n) && n > 0.0 && isfinite(c) && c > 0.0 && isfinite(x) &&
+              isfinite(w) && w > 0.0))
+          continue;
+        if (pass == 0) {
+          origin = x;
+          found = 1;
+          break;
+        }
+        sum_signal += c * s;
+        sum_weight += w;
+        sum_shifted += w * (x - origin);
+        if (errors) sum_var += c * c * fmax(s, 0.0);
+      }
+      if (!found) break;
+    }
+    if (!found) {
+      intensity[r] = 0.0;
+      if (sigma) sigma[r] = 0.0;
+      std_out[r] = NAN;
+      continue;
+    }
+    double mean_shifted = sum_shifted / sum_weight;
+    double m2 = 0.0;
+    for (uint64_t e = lut->row_ptr[r]; e < lut->row_ptr[r + 1]; e++) {
+      uint64_t j = lut->col[e];
+      if (mask && mask[j]) continue;
+      double n = norm[j];
+      double c = lut->w ? lut->w[e] : 1.0;
+      double s = signal[j];
+      double x = s / n;
+      double w = c * n;
+      if (!(isfinite(s) && isfinite(n) && n > 0.0 && isfinite(c) && c > 0.0 && isfinite(x) &&
+            isfinite(w) && w > 0.0))
+        continue;
+      double delta = (x - origin) - mean_shifted;
+      m2 += w * delta * delta;
+    }
+    intensity[r] = sum_signal / sum_weight;
+    if (sigma) sigma[r] = sqrt(fmax(sum_var, 0.0)) / sum_weight;
+    std_out[r] = sqrt(fmax(m2 / sum_weight, 0.0));
+  }
+}

@@ -1,0 +1,104 @@
+/* FP64 CPU oracle of the Dioptrin integration hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: tests, the smoke
+ * check and the CPU-baseline leg of the benchmark link it to compare the HIP
+ * path against. Nothing in the product (dioptrin_amd/) may call it.
+ *
+ * It restates, in plain C with IEEE double arithmetic, no FMA contraction and
+ * the host libm, the reference algorithms (paths relative to the reference
+ * repository root):
+ *   geometry        src/geometry.rs:229-320, 371-636, 639-718
+ *   bin rules       src/math/histogram.rs:120-203, src/math/mod.rs:8-32,
+ *                   src/lut/common.rs:33-65
+ *   units           src/integration.rs:115-208
+ *   corrections     src/integration/corrections.rs:57-93
+ *   counting 1D     src/lut/counting_csr.rs:54-134
+ *   pixel split 1D  src/lut/pixel_split.rs:14-455
+ *   beam pixels     src/lut/pyfai_1d.rs:180-252, integration/grid.rs:48-160
+ *   masked table    src/lut/prepared.rs:163-262
+ *   gather          src/lut/prepared.rs:985-1121
+ *   azimuthal std   src/lut/azimuthal.rs:48-118
+ * Each function's comment cites the lines it follows.
+ */
+#ifndef DIOPTRIN_ORACLE_H
+#define DIOPTRIN_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  double pixel1, pixel2, distance, poni1, poni2, rot1, rot2, rot3, wavelength;
+} or_poni;
+
+enum { OR_UNIT_TTH_RAD = 0, OR_UNIT_TTH_DEG = 1, OR_UNIT_Q_A = 2, OR_UNIT_Q_NM = 3 };
+
+typedef struct {
+  size_t nbins;
+  size_t nnz;
+  uint64_t *row_ptr; /* nbins + 1 */
+  uint64_t *col;     /* nnz */
+  double *w;         /* nnz, NULL for unweighted (counting) tables */
+  double hist_min, hist_step;
+} or_lut;
+
+/* Sample-frame angles of detector points (d1 = row, d2 = column, in pixel
+ * units): tth = atan2(sqrt(y^2 + x^2), z), chi = atan2(x, y). */
+void or_tth_chi(const or_poni *p, const double *d1, const double *d2, size_t n,
+                double *tth, double *chi);
+/* Sample-frame coordinates (x, y, z) of detector points. */
+void or_positions(const or_poni *p, const double *d1, const double *d2, size_t n,
+                  double *x, double *y, double *z);
+/* Corner grid angles, (dim1+1)*(dim2+1) values each. */
+void or_corners(const or_poni *p, size_t dim1, size_t dim2, double *tth, double *chi);
+/* Pixel-centre angles, dim1*dim2 values each. */
+void or_centres(const or_poni *p, size_t dim1, size_t dim2, double *tth, double *chi);
+void or_solid_angle(const or_poni *p, size_t dim1, size_t dim2, double *out);
+void or_polarization(double factor, const double *tth, const double *chi, size_t n, double *out);
+/* normalization = polarization * solid angle (polarization 1 if !has_pol). */
+void or_normalization(const or_poni *p, size_t dim1, size_t dim2, int has_pol, double factor,
+                      double *out);
+void or_convert_unit(int unit, double wavelength, const double *tth, size_t n, double *out);
+
+/* Table builders. Return 0 on success, nonzero on an invalid request. */
+int or_build_lut1d_ps(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, int unit,
+                      int has_range, double rmin, double rmax, or_lut *out);
+int or_build_lut1d_counting(const or_poni *p, size_t dim1, size_t dim2, size_t n_points,
+                            int unit, int has_range, double rmin, double rmax, or_lut *out);
+int or_counting_from_radial(const double *rad, size_t npix, size_t n_points, int has_range,
+                            double rmin, double rmax, or_lut *out);
+int or_ps_fractions(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, int unit,
+                    double min, double step, int clamp_low, or_lut *out);
+void or_lut_free(or_lut *lut);
+
+/* Masked execution table: entries of masked pixels removed, order kept. */
+void or_fixed_table(const or_lut *in, const uint8_t *mask, or_lut *out);
+/* Strict per-bin denominators sum_k w_k * norm[col_k] in row order. */
+void or_denominators(const or_lut *lut, const double *norm, double *den);
+/* Strict gather: I = sum(w x)/den, sigma = sqrt(max(sum(w^2 max(x,0))/den^2, 0)). */
+void or_gather(const or_lut *lut, const double *den, const double *signal, int with_var,
+               double *intensity, double *sigma);
+/* Weighted population spread of corrected pixels (unmasked table, mask per
+ * entry). sigma may be NULL when errors are not requested. */
+void or_azimuthal_std(const or_lut *lut, const uint8_t *mask, const double *norm,
+                      const double *signal, int errors, double *intensity, double *sigma,
+                      double *std_out);
+/* Bin centres min + (i + 0.5) * step. */
+void or_bins(size_t n, double min, double step, double *out);
+
+/* Multithreaded restatement of the reference's default CPU batch1d path:
+ * pixel-major CSC scatter into 4 lane histograms, 4-frame tiles, frames over
+ * a thread pool (src/lut/csc.rs:198-700, src/python/integrator.rs:715-760).
+ * Returns an opaque handle. */
+void *or_csc_new(const or_lut *lut, size_t npix, const uint8_t *mask, const double *norm);
+void or_csc_free(void *csc);
+/* frames: n_frames x npix uint32 frames; out_i: n_frames x nbins. */
+void or_csc_batch_u32(const void *csc, const uint32_t *frames, size_t n_frames, int threads,
+                      double *out_i);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

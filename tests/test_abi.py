@@ -1,0 +1,69 @@
+"""The C-ABI library loads and exports every entry point its header declares
+(no compute calls: these run without a GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dioptrin_hip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dio_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "dio_plan_create" in names and "dio_integrate_device" in names
+    from dioptrin_amd import _native
+    assert sorted(_native.EXPORTS) == names
+
+
+def test_library_exports_every_declared_symbol():
+    from dioptrin_amd import _native
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", _native.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (dio_[a-z0-9_]+)", out))
+    assert set(declared()) <= exported
+
+
+def test_library_is_built_for_gfx950():
+    from dioptrin_amd import _native
+    data = open(_native.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_library_reports_version_and_no_device_cleanly():
+    from dioptrin_amd import _native
+    L = _native.lib()
+    assert b"gfx950" in L.dio_version()
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    rc = L.dio_open(0, ctypes.byref(h))
+    assert rc == _native.DIO_ERR_UNAVAILABLE
+    assert L.dio_last_error()
+
+
+def test_invalid_configuration_fails_before_the_device():
+    # Configuration errors come before any device work (cuda.rs:1001-1057).
+    from dioptrin_amd import _native as N
+    L = N.lib()
+    p = N.Poni(1e-4, 1e-4, 0.1, 0.0, 0.0, 0.0, 0.0, 0.0, 1e-10)
+    d = N.PlanDesc()
+    d.dim1, d.dim2, d.ndim, d.n_radial = 0, 10, 1, 10
+    rc = L.dio_plan_create(None, ctypes.byref(p), ctypes.byref(d), ctypes.byref(ctypes.c_void_p()))
+    assert rc == N.DIO_ERR_INVALID
