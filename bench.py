@@ -47,7 +47,7 @@ def cpu_baseline(frames_dev, poni, w, seconds=12.0):
     norm = O.normalization(O.Poni(**poni), *w["shape"], w["polarization"])
     csc = O.CscBaseline(lut, w["shape"][0] * w["shape"][1], None, norm)
     nf = 4 * threads
-    host = frames_dev[:nf].cpu().numpy().view(np.uint32).reshape(nf, -1)
+    host = frames_dev[:nf].numpy().reshape(nf, -1)
     done, t0 = 0, time.time()
     while True:
         csc.batch_u32(host, threads)
@@ -78,9 +78,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import torch
-
     from dioptrin_amd import GpuPlan, device_info, synth
+    from dioptrin_amd.device import DeviceArray, synchronize
 
     w = synth.WORKLOADS[WORKLOAD]
     poni = synth.workload_poni(WORKLOAD)
@@ -89,12 +88,12 @@ def main():
     log(f"[bench] device {device_info(0)['name']} {device_info(0)['arch']}")
     t0 = time.time()
     frames = synth.frames_device(WORKLOAD, nfr, seed=2026)
-    torch.cuda.synchronize()
+    synchronize()
     log(f"[bench] generated {nfr} frames in {time.time() - t0:.1f} s")
 
     t0 = time.time()
     plan = GpuPlan(poni, w["shape"], n_radial=w["n_radial"], precision=args.precision)
-    torch.cuda.synchronize()
+    synchronize()
     build_wall = (time.time() - t0) * 1e3
     info = plan.info
     log(f"[bench] table: nnz {info['nnz']} segments {info['n_segments']} build "
@@ -105,8 +104,8 @@ def main():
     log(f"[bench] frames per tile {tile}")
 
     bins = plan.nbins
-    out_i = torch.empty((nfr, bins), dtype=torch.float64, device="cuda")
-    out_s = torch.empty((nfr, bins), dtype=torch.float64, device="cuda") if args.errors else None
+    out_i = DeviceArray((nfr, bins), np.float64)
+    out_s = DeviceArray((nfr, bins), np.float64) if args.errors else None
 
     if args.no_graph:
         def step():
@@ -119,11 +118,11 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    torch.cuda.synchronize()
+    synchronize()
     el = time.perf_counter() - t0
     ms_per_step = el * 1e3 / args.steps
     fps = nfr * args.steps / el
@@ -136,12 +135,12 @@ def main():
     for k in range(4):
         plan1.integrate_device(frames[k:k + 1], out_i[k:k + 1], None if out_s is None else
                                out_s[k:k + 1], errors=args.errors)
-    torch.cuda.synchronize()
+    synchronize()
     t0 = time.perf_counter()
     for k in range(n1):
         plan1.integrate_device(frames[k:k + 1], out_i[k:k + 1], None if out_s is None else
                                out_s[k:k + 1], errors=args.errors)
-        torch.cuda.synchronize()
+        synchronize()
     per_frame_fps = n1 / (time.perf_counter() - t0)
     plan.set_tile(tile)
     log(f"[bench] per-frame: {per_frame_fps:.0f} frames/s")

@@ -25,7 +25,8 @@ EXPORTS = [
     "dio_plan_azimuthal", "dio_plan_export_csr", "dio_plan_denominators",
     "dio_plan_normalization", "dio_integrate", "dio_integrate_device", "dio_plan_autotune",
     "dio_plan_set_tile", "dio_graph_create", "dio_graph_launch", "dio_graph_destroy",
-    "dio_plan_time_gather",
+    "dio_plan_time_gather", "dio_device_alloc", "dio_device_free", "dio_memcpy",
+    "dio_synchronize", "dio_synthetic_poisson_u32",
 ]
 
 
@@ -108,6 +109,11 @@ def lib() -> C.CDLL:
             "dio_graph_destroy": (i32, [vp]),
             "dio_plan_time_gather": (i32, [vp, vp, i32, u64, u32, dp, dp, i32,
                                            C.POINTER(C.c_double)]),
+            "dio_device_alloc": (i32, [vp, u64, C.POINTER(vp)]),
+            "dio_device_free": (i32, [vp, vp]),
+            "dio_memcpy": (i32, [vp, vp, vp, u64, i32]),
+            "dio_synchronize": (i32, [vp]),
+            "dio_synthetic_poisson_u32": (i32, [vp, vp, u64, vp, u64, u64, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -137,7 +137,11 @@ void run_device(dio_plan* p, dio::GatherScratch& sc, const void* frames, int dty
 
 }  // namespace
 
+void dio_set_last_error(const char* msg) { g_last_error = msg; }
+
 extern "C" {
+
+int dio_ctx_device(dio_ctx* ctx) { return ctx->device; }
 
 const char* dio_last_error(void) { return g_last_error.c_str(); }
 const char* dio_version(void) { return "dioptrin_hip 0.1.0 (gfx950)"; }

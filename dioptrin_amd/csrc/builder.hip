@@ -395,6 +395,339 @@ __global__ void k_count_fill(const uint32_t* cnt, const uint32_t* pbin, const ui
   w[at] = 1.0;
 }
 
+// ------------------------------------------------------------ 2D grids
+
+// integration/grid.rs:26-222 (IntegrationGrid2d) with its clamp flags.
+struct Grid2 {
+  uint32_t nr, na;
+  double r0, r1, a0, a1;
+  int clamp_radial_end, clamp_index_rounding, clamp_radial_outside, clamp_azimuthal_outside;
+  __host__ __device__ double rstep() const { return (r1 - r0) / (double)nr; }
+  __host__ __device__ double astep() const { return (a1 - a0) / (double)na; }
+};
+
+// grid.rs:176-221 (IntegrationGrid2d::bin); -1 when the value has no bin.
+__host__ __device__ inline long long grid_bin(const Grid2& g, double radial, double az) {
+  if (!std::isfinite(radial) || !std::isfinite(az)) return -1;
+  if (!g.clamp_radial_outside &&
+      (radial < g.r0 || radial > g.r1 || (radial == g.r1 && !g.clamp_radial_end)))
+    return -1;
+  double offset;
+  if (g.clamp_azimuthal_outside) {
+    offset = az - g.a0;
+    double hi = g.a1 - g.a0;
+    offset = offset < 0.0 ? 0.0 : (offset > hi ? hi : offset);
+  } else if (g.clamp_index_rounding) {
+    double wrapped = (az >= g.a0 && az < g.a1) ? az : g.a0 + rem_euclid(az - g.a0, kTauD);
+    if (wrapped >= g.a1 && g.a1 - g.a0 < kTauD) return -1;
+    offset = wrapped - g.a0;
+  } else {
+    double a = g.a0 + rem_euclid(az - g.a0, kTauD);
+    if (a >= g.a1) return -1;
+    offset = a - g.a0;
+  }
+  unsigned long long r = sat_usize((radial - g.r0) / g.rstep());
+  unsigned long long a = sat_usize(offset / g.astep());
+  if (g.clamp_radial_end || g.clamp_index_rounding || g.clamp_radial_outside)
+    if (r > g.nr - 1) r = g.nr - 1;
+  if (g.clamp_index_rounding || g.clamp_azimuthal_outside)
+    if (a > g.na - 1) a = g.na - 1;
+  if (r < g.nr && a < g.na) return (long long)(a * g.nr + r);
+  return -1;
+}
+
+struct Ps2Params {
+  Geo g;
+  Grid2 G;
+  const double* rc;
+  const double* ac;
+  const double* rad;
+  const double* chi;
+};
+
+// shared_grid.rs:152-200 (split_polygon): the three periodic copies of the
+// polygon, bins (ar, rr) in order; a bin hit by several copies gets the sum
+// of its weights in copy order (-2pi, 0, +2pi), as the reference merges them.
+template <class Emit>
+__device__ void split2d(const Ps2Params& P, const double (*poly)[2], double area, Emit& emit) {
+  const Grid2& G = P.G;
+  double r0 = G.r0, a0 = G.a0, dr = G.rstep(), da = G.astep();
+  double rmin = INFINITY, rmax = -INFINITY;
+  for (int i = 0; i < 4; i++) {
+    rmin = fmin(rmin, poly[i][0]);
+    rmax = fmax(rmax, poly[i][0]);
+  }
+  long long rs = sat_isize(floor((rmin - r0) / dr));
+  long long re = sat_isize(floor((rmax - r0) / dr));
+  if (rs < 0) rs = 0;
+  if (re > (long long)G.nr - 1) re = (long long)G.nr - 1;
+  const double shifts[3] = {-kTauD, 0.0, kTauD};
+  long long as[3], ae[3];
+  long long lo = 0x7fffffffffffffffLL, hi = -1;
+  for (int k = 0; k < 3; k++) {
+    double amin = INFINITY, amax = -INFINITY;
+    for (int i = 0; i < 4; i++) {
+      double a = poly[i][1] + shifts[k];
+      amin = fmin(amin, a);
+      amax = fmax(amax, a);
+    }
+    as[k] = sat_isize(floor((amin - a0) / da));
+    ae[k] = sat_isize(floor((amax - a0) / da));
+    if (as[k] < 0) as[k] = 0;
+    if (ae[k] > (long long)G.na - 1) ae[k] = (long long)G.na - 1;
+    if (as[k] <= ae[k]) {
+      lo = as[k] < lo ? as[k] : lo;
+      hi = ae[k] > hi ? ae[k] : hi;
+    }
+  }
+  for (long long ar = lo; ar <= hi; ar++) {
+    bool any = false;
+    for (int k = 0; k < 3; k++) any |= (as[k] <= ar && ar <= ae[k]);
+    if (!any) continue;
+    for (long long rr = rs; rr <= re; rr++) {
+      double r = r0 + (double)rr * dr;
+      double a = a0 + (double)ar * da;
+      double acc = 0.0;
+      bool found = false;
+      for (int k = 0; k < 3; k++) {
+        if (!(as[k] <= ar && ar <= ae[k])) continue;
+        double ps[4][2];
+        for (int i = 0; i < 4; i++) {
+          ps[i][0] = poly[i][0];
+          ps[i][1] = poly[i][1] + shifts[k];
+        }
+        double w = clipped_area(ps, r, r + dr, a, a + da) / area;
+        if (w > 0.0) {
+          acc = found ? acc + w : w;
+          found = true;
+        }
+      }
+      if (found) emit((uint32_t)((uint64_t)ar * G.nr + (uint64_t)rr), acc);
+    }
+  }
+}
+
+// shared_grid.rs:58-130 for one pixel of a PixelSplit 2D table. Returns false
+// for a pixel around or on the polar axis (integrated by sampling).
+template <class Emit>
+__device__ bool ps2d_pixel(const Ps2Params& P, uint32_t idx, Emit& emit) {
+  uint32_t dim2 = P.g.dim2;
+  uint64_t w = (uint64_t)dim2 + 1;
+  uint64_t b = (uint64_t)(idx / dim2) * w + idx % dim2;
+  uint64_t ci[4] = {b, b + 1, b + w + 1, b + w};
+  bool axis_on_edge = false, zero_r = false;
+  double winding = 0.0;
+  for (int i = 0; i < 4; i++) {
+    double delta = P.ac[ci[(i + 1) & 3]] - P.ac[ci[i]];
+    delta = rem_euclid(delta + kPiD, kTauD) - kPiD;
+    axis_on_edge |= fabs(delta) >= kPiD - 64.0 * kEps;
+    winding += delta;
+    zero_r |= P.rc[ci[i]] == 0.0;
+  }
+  if (fabs(winding) > kPiD || axis_on_edge || zero_r) return false;
+  double center = P.G.a0 + rem_euclid(P.chi[idx] - P.G.a0, kTauD);
+  double poly[4][2];
+  for (int i = 0; i < 4; i++) {
+    poly[i][0] = P.rc[ci[i]];
+    poly[i][1] = center + rem_euclid(P.ac[ci[i]] - center + kPiD, kTauD) - kPiD;
+  }
+  double area = fabs(polygon_area(poly, 4));
+  if (area > 0.0) {
+    split2d(P, poly, area, emit);
+  } else {
+    long long bin = grid_bin(P.G, P.rad[idx], P.chi[idx]);
+    if (bin >= 0) emit((uint32_t)bin, 1.0);
+  }
+  return true;
+}
+
+__global__ void k_ps2d_count(Ps2Params P, uint32_t npix, uint32_t* cnt, uint32_t* sing,
+                             uint32_t* nsing) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npix) return;
+  CountEmit e;
+  if (!ps2d_pixel(P, idx, e)) {
+    uint32_t s = atomicAdd(nsing, 1u);
+    if (s < kMaxSingular) sing[s] = idx;
+    cnt[idx] = 0;
+    return;
+  }
+  cnt[idx] = e.n;
+}
+
+__global__ void k_ps2d_fill(Ps2Params P, uint32_t npix, const uint32_t* off, uint32_t* bin,
+                            uint32_t* pix, double* w) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npix) return;
+  FillEmit e{bin, pix, w, off[idx], idx};
+  ps2d_pixel(P, idx, e);
+}
+
+// shared_grid.rs:202-232: 2D sample coordinates d1 = row + (i + o1)/128,
+// d2 = ((col * 128 + j) + o2)/128.
+__host__ __device__ inline void sample_point_2d(uint32_t dim2, uint32_t pixel, uint32_t s,
+                                                double& d1, double& d2) {
+  const double o1[4] = {0.25, 0.5, 0.75, 0.5};
+  const double o2[4] = {0.5, 0.25, 0.5, 0.75};
+  uint32_t i = s >> 9, j = (s >> 2) & 127u, o = s & 3u;
+  uint32_t row = pixel / dim2, col = pixel % dim2;
+  d1 = (double)row + ((double)i + o1[o]) / 128.0;
+  d2 = ((double)((uint64_t)col * 128 + j) + o2[o]) / 128.0;
+}
+
+__global__ void k_sing_samples2d(Geo g, const uint32_t* sing, uint32_t nsing, double* stth,
+                                 double* schi, uint32_t* flist, uint32_t* fcount) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)nsing * kSamplesPerPixel) return;
+  uint32_t slot = (uint32_t)(i / kSamplesPerPixel), s = (uint32_t)(i % kSamplesPerPixel);
+  double d1, d2;
+  sample_point_2d(g.dim2, sing[slot], s, d1, d2);
+  Angles a = angles_cert(g, d1, d2);
+  stth[i] = a.tth;
+  schi[i] = a.chi;
+  if (a.uncertain) flist[atomicAdd(fcount, 1u)] = (uint32_t)i;
+}
+
+__global__ void k_sing_bin2d(Geo g, Grid2 G, uint32_t nsing, const double* stth,
+                             const double* schi, uint32_t* hits) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)nsing * kSamplesPerPixel) return;
+  uint32_t slot = (uint32_t)(i / kSamplesPerPixel);
+  long long b = grid_bin(G, tth_to_unit(g, stth[i]), schi[i]);
+  if (b >= 0) atomicAdd(&hits[(uint64_t)slot * G.nr * G.na + (uint64_t)b], 1u);
+}
+
+// Counting 2D (shared_grid.rs:52-57): the bin of the pixel centre.
+__global__ void k_count2d_bins(Grid2 G, const double* rad, const double* chi, uint32_t npix,
+                               uint32_t* cnt, uint32_t* pbin) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  long long b = grid_bin(G, rad[i], chi[i]);
+  cnt[i] = b >= 0;
+  if (b >= 0) pbin[i] = (uint32_t)b;
+}
+
+// ------------------------------------------------------------ supersampling
+
+// Bin rule of one subpixel (supersampled.rs:517-595, 1381-1477): 1D clamped
+// (automatic range) or in-range (explicit range), 2D the grid's bin.
+struct SsBin {
+  Geo g;
+  int twod;
+  uint32_t n1;
+  double min1, max1, step1;
+  int clamped1;
+  Grid2 G;
+  uint32_t nss;
+  int inv;  // 1: rf + (i + 0.5) * inv_ss coordinates; 0: (i + 0.5) / n_ss
+};
+
+__host__ __device__ inline long long ss_bin(const SsBin& B, double tth, double chi) {
+  double r = tth_to_unit(B.g, tth);
+  if (B.twod) return grid_bin(B.G, r, chi);
+  if (!B.clamped1 && !(r >= B.min1 && r <= B.max1)) return -1;
+  unsigned long long u = sat_usize((r - B.min1) / B.step1);
+  if (u > B.n1 - 1) u = B.n1 - 1;
+  return (long long)u;
+}
+
+// supersampled.rs:39-43 (/ n_ss) and 310-311, 438-440 (* inv_ss).
+__host__ __device__ inline void ss_point(const SsBin& B, uint64_t sub, double& d1, double& d2) {
+  uint32_t nss = B.nss, nss2 = nss * nss;
+  uint64_t pixel = sub / nss2;
+  uint32_t k = (uint32_t)(sub % nss2), i = k / nss, j = k % nss;
+  uint64_t row = pixel / B.g.dim2, col = pixel % B.g.dim2;
+  double gc = (double)(col * nss + j);
+  if (B.inv) {
+    double inv_ss = 1.0 / (double)nss;
+    d1 = (double)row + ((double)i + 0.5) * inv_ss;
+    d2 = (gc + 0.5) * inv_ss;
+  } else {
+    d1 = (double)row + ((double)i + 0.5) / (double)nss;
+    d2 = (gc + 0.5) / (double)nss;
+  }
+}
+
+__device__ inline double next_up(double v) { return nextafter(v, INFINITY); }
+__device__ inline double next_down(double v) { return nextafter(v, -INFINITY); }
+
+// Bin of every subpixel. An uncertain angle matters only if one of its two
+// possible roundings changes the bin; such subpixels are listed for the host.
+__global__ void k_ss_bins(SsBin B, uint64_t nsub, uint32_t* bins, uint32_t* flist,
+                          uint32_t* fcount, uint32_t fcap) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsub) return;
+  double d1, d2;
+  ss_point(B, t, d1, d2);
+  Angles a = angles_cert(B.g, d1, d2);
+  long long b = ss_bin(B, a.tth, a.chi);
+  bool ambiguous = false;
+  if (a.uncertain) {
+    double tc[3] = {a.tth, next_up(a.tth), next_down(a.tth)};
+    double cc[3] = {a.chi, next_up(a.chi), next_down(a.chi)};
+    int nt = (a.uncertain & 1) ? 3 : 1, nc = (B.twod && (a.uncertain & 2)) ? 3 : 1;
+    for (int x = 0; x < nt; x++)
+      for (int y = 0; y < nc; y++) ambiguous |= ss_bin(B, tc[x], cc[y]) != b;
+  }
+  bins[t] = b < 0 ? 0xffffffffu : (uint32_t)b;
+  if (ambiguous) {
+    uint32_t pos = atomicAdd(fcount, 1u);
+    if (pos < fcap) flist[pos] = (uint32_t)t;
+  }
+}
+
+__global__ void k_ss_patch(const uint32_t* idx, const uint32_t* val, uint32_t n, uint32_t* bins) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) bins[idx[i]] = val[i];
+}
+
+// Distinct bins among a pixel's subpixels, in first-occurrence order.
+__global__ void k_ss_count(const uint32_t* bins, uint32_t npix, uint32_t nss2, uint32_t* cnt) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  const uint32_t* sb = bins + (uint64_t)p * nss2;
+  uint32_t n = 0;
+  for (uint32_t k = 0; k < nss2; k++) {
+    uint32_t b = sb[k];
+    if (b == 0xffffffffu) continue;
+    bool seen = false;
+    for (uint32_t q = 0; q < k && !seen; q++) seen = sb[q] == b;
+    if (!seen) n++;
+  }
+  cnt[p] = n;
+}
+
+__global__ void k_ss_fill(const uint32_t* bins, uint32_t npix, uint32_t nss2, const uint32_t* off,
+                          uint32_t* bin, uint32_t* pix, double* w) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  const uint32_t* sb = bins + (uint64_t)p * nss2;
+  uint64_t at = off[p];
+  for (uint32_t k = 0; k < nss2; k++) {
+    uint32_t b = sb[k];
+    if (b == 0xffffffffu) continue;
+    bool seen = false;
+    for (uint32_t q = 0; q < k && !seen; q++) seen = sb[q] == b;
+    if (seen) continue;
+    uint32_t c = 0;
+    for (uint32_t q = k; q < nss2; q++) c += sb[q] == b;
+    bin[at] = b;
+    pix[at] = p;
+    w[at] = (double)c / (double)nss2;
+    at++;
+  }
+}
+
+__global__ void k_list_angles(Geo g, const double* d1, const double* d2, uint32_t n, double* tth,
+                              double* chi, uint32_t* flist, uint32_t* fcount) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Angles a = angles_cert(g, d1[i], d2[i]);
+  tth[i] = a.tth;
+  chi[i] = a.chi;
+  if (a.uncertain) flist[atomicAdd(fcount, 1u)] = i;
+}
+
 // --------------------------------------------------------- CSR assembly
 
 __global__ void k_iota(uint32_t* v, uint64_t n) {
@@ -644,8 +977,9 @@ struct Singular {
   uint64_t patched = 0;
 };
 
-void singular_prepare(const Geo& g, const SampleGrid& G, Singular& S, uint32_t* cnt,
-                      hipStream_t s) {
+void singular_prepare(const Geo& g, const SampleGrid& G, const Grid2* G2, Singular& S,
+                      uint32_t* cnt, hipStream_t s) {
+  uint32_t nbins = G2 ? G2->nr * G2->na : G.n;
   uint32_t ns = (uint32_t)S.pixels.size();
   if (!ns) return;
   std::sort(S.pixels.begin(), S.pixels.end());
@@ -656,8 +990,12 @@ void singular_prepare(const Geo& g, const SampleGrid& G, Singular& S, uint32_t* 
   DevBuf<double> st(total), sc(total);
   DevBuf<uint32_t> flist(total), fcount(1);
   DIO_HIP(hipMemsetAsync(fcount.p, 0, sizeof(uint32_t), s));
-  k_sing_samples<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, S.dpix.p, ns, st.p, sc.p,
-                                                                 flist.p, fcount.p);
+  if (G2)
+    k_sing_samples2d<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, S.dpix.p, ns, st.p,
+                                                                     sc.p, flist.p, fcount.p);
+  else
+    k_sing_samples<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, S.dpix.p, ns, st.p, sc.p,
+                                                                   flist.p, fcount.p);
   DIO_HIP(hipGetLastError());
   uint32_t nf = read1(fcount.p, s);
   if (nf) {
@@ -670,6 +1008,10 @@ void singular_prepare(const Geo& g, const SampleGrid& G, Singular& S, uint32_t* 
     for (uint32_t i = 0; i < nf; i++) {
       uint32_t slot = idx[i] / kSamplesPerPixel, smp = idx[i] % kSamplesPerPixel;
       uint32_t pixel = S.pixels[slot];
+      if (G2) {
+        sample_point_2d(g.dim2, pixel, smp, pts[i].first, pts[i].second);
+        continue;
+      }
       uint32_t row = smp >> 9, col = (smp >> 2) & 127u, o = smp & 3u;
       pts[i] = {(double)(pixel / g.dim2) + ((double)row + dy[o]) / 128.0,
                 (double)(pixel % g.dim2) + ((double)col + dx[o]) / 128.0};
@@ -685,10 +1027,14 @@ void singular_prepare(const Geo& g, const SampleGrid& G, Singular& S, uint32_t* 
     DIO_HIP(hipGetLastError());
     S.patched = nf;
   }
-  S.hits.alloc((uint64_t)ns * G.n);
+  S.hits.alloc((uint64_t)ns * nbins);
   DIO_HIP(hipMemsetAsync(S.hits.p, 0, S.hits.bytes(), s));
-  k_sing_bin<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, G, ns, st.p, sc.p, S.hits.p);
-  k_sing_count<<<ns, kThreads, 0, s>>>(S.hits.p, G.n, ns, S.dpix.p, cnt);
+  if (G2)
+    k_sing_bin2d<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, *G2, ns, st.p, sc.p,
+                                                                 S.hits.p);
+  else
+    k_sing_bin<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, G, ns, st.p, sc.p, S.hits.p);
+  k_sing_count<<<ns, kThreads, 0, s>>>(S.hits.p, nbins, ns, S.dpix.p, cnt);
   DIO_HIP(hipGetLastError());
   DIO_HIP(hipStreamSynchronize(s));
 }
@@ -784,7 +1130,7 @@ void build_1d(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, 
       DIO_HIP(hipStreamSynchronize(s));
     }
     SampleGrid G{n, P.glo, P.ghi, P.clamp_low};
-    singular_prepare(g, G, S, cnt.p, s);
+    singular_prepare(g, G, nullptr, S, cnt.p, s);
     out.uncertain += S.patched;
     out.singular = ns;
     exclusive_scan(cnt.p, off.p, npix + 1, s, sc);
@@ -810,6 +1156,367 @@ void build_1d(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, 
   out.n_azimuthal = 1;
   assemble(n, nnz, ebin, epix, ew, s, sc, out.full);
   // Corrections use the (patched) pixel-centre angles.
+  out.norm.alloc(npix);
+  k_normalization<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(
+      g, ttp.p, chp.p, d.has_polarization, d.polarization_factor, out.norm.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+}
+
+// Angles of an explicit point list, certified and patched.
+uint64_t list_angles(const Geo& g, const std::vector<double>& d1, const std::vector<double>& d2,
+                     DevBuf<double>& tth, DevBuf<double>& chi, hipStream_t s) {
+  uint32_t n = (uint32_t)d1.size();
+  tth.alloc(n);
+  chi.alloc(n);
+  if (!n) return 0;
+  DevBuf<double> a(n), b(n);
+  DevBuf<uint32_t> flist(n), fcount(1);
+  DIO_HIP(hipMemcpyAsync(a.p, d1.data(), n * 8, hipMemcpyHostToDevice, s));
+  DIO_HIP(hipMemcpyAsync(b.p, d2.data(), n * 8, hipMemcpyHostToDevice, s));
+  DIO_HIP(hipMemsetAsync(fcount.p, 0, 4, s));
+  k_list_angles<<<blocks_for(n, kThreads), kThreads, 0, s>>>(g, a.p, b.p, n, tth.p, chi.p,
+                                                            flist.p, fcount.p);
+  DIO_HIP(hipGetLastError());
+  uint32_t nf = read1(fcount.p, s);
+  if (!nf) return 0;
+  std::vector<uint32_t> idx(nf);
+  DIO_HIP(hipMemcpyAsync(idx.data(), flist.p, nf * 4, hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipStreamSynchronize(s));
+  std::vector<std::pair<double, double>> pts(nf);
+  for (uint32_t i = 0; i < nf; i++) pts[i] = {d1[idx[i]], d2[idx[i]]};
+  std::vector<double> tv, cv;
+  host_patch(g, idx, pts, tv, cv);
+  DevBuf<uint32_t> di(nf);
+  DevBuf<double> dt(nf), dc(nf);
+  DIO_HIP(hipMemcpyAsync(di.p, idx.data(), nf * 4, hipMemcpyHostToDevice, s));
+  DIO_HIP(hipMemcpyAsync(dt.p, tv.data(), nf * 8, hipMemcpyHostToDevice, s));
+  DIO_HIP(hipMemcpyAsync(dc.p, cv.data(), nf * 8, hipMemcpyHostToDevice, s));
+  k_patch<<<blocks_for(nf, kThreads), kThreads, 0, s>>>(di.p, dt.p, dc.p, nf, tth.p, chi.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+  return nf;
+}
+
+// histogram.rs:151-181: range over pixel centres and the hull grid
+// (lut/common.rs:33-65) plus 3 eps, for radial (unit) and azimuth.
+void ss_ranges(const Geo& g, uint32_t nss, const double* rad, const double* chi, uint64_t npix,
+               hipStream_t s, double& rlo, double& rhi, double& alo, double& ahi,
+               uint64_t& patched) {
+  uint64_t dim1 = g.dim1, dim2 = g.dim2;
+  double border = 0.5 / (double)nss, step = 1.0 / (double)nss;
+  std::vector<double> h1, h2;
+  for (uint64_t i = 0; i < dim1 * nss; i++) { h1.push_back(border + (double)i * step); h2.push_back(border); }
+  for (uint64_t i = 0; i < dim1 * nss; i++) {
+    h1.push_back(border + (double)i * step);
+    h2.push_back((double)dim2 - border);
+  }
+  for (uint64_t i = 1; i < dim2 * nss; i++) { h1.push_back(border); h2.push_back(border + (double)i * step); }
+  for (uint64_t i = 1; i < dim2 * nss; i++) {
+    h1.push_back((double)dim1 - border);
+    h2.push_back(border + (double)i * step);
+  }
+  DevBuf<double> ht, ha, hr;
+  patched += list_angles(g, h1, h2, ht, ha, s);
+  const double* hrad = ht.p;
+  if (g.unit != 0) {
+    hr.alloc(h1.size());
+    k_to_unit<<<blocks_for(h1.size(), kThreads), kThreads, 0, s>>>(g, ht.p, hr.p, h1.size());
+    hrad = hr.p;
+  }
+  double a, b, c, d;
+  device_minmax(rad, npix, s, a, b);
+  device_minmax(hrad, h1.size(), s, c, d);
+  rlo = std::fmin(a, c);
+  rhi = std::fmax(b, d) + 3.0 * kEps;
+  if (chi) {
+    device_minmax(chi, npix, s, a, b);
+    device_minmax(ha.p, h1.size(), s, c, d);
+    alo = std::fmin(a, c);
+    ahi = std::fmax(b, d) + 3.0 * kEps;
+  }
+}
+
+// Supersampled entries: per-subpixel bins, host-resolved ambiguous ones,
+// then per-pixel distinct bins with weight count / n_ss^2.
+uint64_t ss_entries(const SsBin& B, uint64_t npix, hipStream_t s, Scratch& sc,
+                    DevBuf<uint32_t>& cnt, DevBuf<uint32_t>& off, DevBuf<uint32_t>& ebin,
+                    DevBuf<uint32_t>& epix, DevBuf<double>& ew, uint64_t& patched) {
+  uint32_t nss2 = B.nss * B.nss;
+  uint64_t nsub = npix * nss2;
+  if (nsub > 0xfffffff0ull) fail(DIO_ERR_UNSUPPORTED, "too many subpixels for 32-bit indices");
+  DevBuf<uint32_t> bins(nsub), flist(1u << 20), fcount(1);
+  DIO_HIP(hipMemsetAsync(fcount.p, 0, 4, s));
+  k_ss_bins<<<blocks_for(nsub, kThreads), kThreads, 0, s>>>(B, nsub, bins.p, flist.p, fcount.p,
+                                                           1u << 20);
+  DIO_HIP(hipGetLastError());
+  uint32_t nf = read1(fcount.p, s);
+  if (nf > (1u << 20)) fail(DIO_ERR_DEVICE, "too many ambiguous subpixels");
+  if (nf) {
+    std::vector<uint32_t> idx(nf), val(nf);
+    DIO_HIP(hipMemcpyAsync(idx.data(), flist.p, nf * 4, hipMemcpyDeviceToHost, s));
+    DIO_HIP(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < nf; i++) {
+      double d1, d2;
+      ss_point(B, idx[i], d1, d2);
+      AngleArgs a = angle_args(B.g, d1, d2);
+      long long b = ss_bin(B, ::atan2(a.ty, a.tx), ::atan2(a.cy, a.cx));
+      val[i] = b < 0 ? 0xffffffffu : (uint32_t)b;
+    }
+    DevBuf<uint32_t> di(nf), dv(nf);
+    DIO_HIP(hipMemcpyAsync(di.p, idx.data(), nf * 4, hipMemcpyHostToDevice, s));
+    DIO_HIP(hipMemcpyAsync(dv.p, val.data(), nf * 4, hipMemcpyHostToDevice, s));
+    k_ss_patch<<<blocks_for(nf, kThreads), kThreads, 0, s>>>(di.p, dv.p, nf, bins.p);
+    DIO_HIP(hipStreamSynchronize(s));
+    patched += nf;
+  }
+  k_ss_count<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(bins.p, (uint32_t)npix, nss2,
+                                                            cnt.p);
+  exclusive_scan(cnt.p, off.p, npix + 1, s, sc);
+  uint64_t nnz = read1(off.p + npix, s);
+  ebin.alloc(nnz);
+  epix.alloc(nnz);
+  ew.alloc(nnz);
+  k_ss_fill<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(bins.p, (uint32_t)npix, nss2, off.p,
+                                                           ebin.p, epix.p, ew.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+  return nnz;
+}
+
+// integration/grid.rs:48-108 (IntegrationGrid2d::new): validity and the
+// correction of azimuthal starts outside [-2pi, 2pi].
+Grid2 grid_new(uint32_t nr, uint32_t na, double rlo, double rhi, double alo, double ahi) {
+  double lo[2] = {rlo, alo}, hi[2] = {rhi, ahi};
+  uint32_t cnt[2] = {nr, na};
+  const char* names[2] = {"radial", "azimuthal"};
+  for (int k = 0; k < 2; k++) {
+    double step = (hi[k] - lo[k]) / (double)cnt[k];
+    if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]) || !std::isfinite(step) ||
+        step <= 0.0 || lo[k] + step == lo[k] || hi[k] - step == hi[k])
+      fail(DIO_ERR_INVALID, std::string(names[k]) +
+                                " range must have finite, increasing, resolvable bin edges");
+  }
+  if (rlo < 0.0) fail(DIO_ERR_INVALID, "radial range must be nonnegative");
+  if (ahi - alo > kTauD + 1e-12) fail(DIO_ERR_INVALID, "azimuthal span must be at most 2pi");
+  if (!(alo >= -kTauD && alo <= kTauD)) {
+    double start = rem_euclid(alo, kTauD);
+    if (start >= kPiD) start -= kTauD;
+    ahi = start + (ahi - alo);
+    alo = start;
+  }
+  Grid2 g{};
+  g.nr = nr;
+  g.na = na;
+  g.r0 = rlo;
+  g.r1 = rhi;
+  g.a0 = alo;
+  g.a1 = ahi;
+  return g;
+}
+
+// Default and explicit-range 2D tables of Counting and PixelSplit
+// (integration/methods.rs:365-388, grid.rs:579-675, 978-1028,
+// lut/shared_grid.rs:13-150).
+void build_2d(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, Built& out) {
+  uint32_t dim1 = d.dim1, dim2 = d.dim2;
+  uint64_t npix = (uint64_t)dim1 * dim2;
+  uint32_t nr = d.n_radial, na = d.n_azimuthal;
+  uint64_t nb64 = (uint64_t)nr * na;
+  DevBuf<double> ttp(npix), chp(npix), radp;
+  out.uncertain += grid_angles(g, dim1, dim2, 0.5, ttp.p, chp.p, s);
+  const double* rad = ttp.p;
+  if (g.unit != 0) {
+    radp.alloc(npix);
+    k_to_unit<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(g, ttp.p, radp.p, npix);
+    rad = radp.p;
+  }
+  bool ps = d.method == DIO_PIXEL_SPLIT;
+  if (!ps && d.method != DIO_COUNTING)
+    fail(DIO_ERR_UNSUPPORTED, "method not available for 2D tables on this device build");
+  uint64_t nc = (uint64_t)(dim1 + 1) * (dim2 + 1);
+  DevBuf<double> tc, ac, rcb;
+  const double* rc = nullptr;
+  if (ps) {
+    tc.alloc(nc);
+    ac.alloc(nc);
+    out.uncertain += grid_angles(g, dim1 + 1, dim2 + 1, 0.0, tc.p, ac.p, s);
+    rc = tc.p;
+    if (g.unit != 0) {
+      rcb.alloc(nc);
+      k_to_unit<<<blocks_for(nc, kThreads), kThreads, 0, s>>>(g, tc.p, rcb.p, nc);
+      rc = rcb.p;
+    }
+  }
+  double rlo, rhi, alo, ahi;
+  if (d.has_radial_range) {
+    rlo = d.radial_min;
+    rhi = d.radial_max;
+  } else {
+    device_minmax(ps ? rc : rad, ps ? nc : npix, s, rlo, rhi);
+    rhi = rhi + 10.0 * kEps;
+  }
+  if (d.has_azimuthal_range) {
+    alo = d.azimuthal_min;
+    ahi = d.azimuthal_max;
+  } else {
+    device_minmax(ps ? ac.p : chp.p, ps ? nc : npix, s, alo, ahi);
+    ahi = ahi + 10.0 * kEps;
+    if (!(ahi - alo > 1e-12)) {
+      alo = -kPiD;
+      ahi = kPiD;
+    }
+  }
+  Grid2 G = grid_new(nr, na, rlo, rhi, alo, ahi);
+  G.clamp_index_rounding = 1;
+  if (!d.has_radial_range && !ps) G.clamp_radial_end = 1;
+  DevBuf<uint32_t> cnt(npix + 1), off(npix + 1);
+  DIO_HIP(hipMemsetAsync(cnt.p, 0, (npix + 1) * sizeof(uint32_t), s));
+  DevBuf<uint32_t> ebin, epix;
+  DevBuf<double> ew;
+  uint64_t nnz = 0;
+  if (!ps) {
+    DevBuf<uint32_t> pbin(npix);
+    k_count2d_bins<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(G, rad, chp.p, (uint32_t)npix,
+                                                                  cnt.p, pbin.p);
+    exclusive_scan(cnt.p, off.p, npix + 1, s, sc);
+    nnz = read1(off.p + npix, s);
+    ebin.alloc(nnz);
+    epix.alloc(nnz);
+    ew.alloc(nnz);
+    k_count_fill<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(cnt.p, pbin.p, off.p,
+                                                                (uint32_t)npix, ebin.p, epix.p,
+                                                                ew.p);
+    DIO_HIP(hipGetLastError());
+    out.weighted = false;
+  } else {
+    Ps2Params P{g, G, rc, ac.p, rad, chp.p};
+    DevBuf<uint32_t> sing(kMaxSingular), nsing(1);
+    DIO_HIP(hipMemsetAsync(nsing.p, 0, sizeof(uint32_t), s));
+    k_ps2d_count<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(P, (uint32_t)npix, cnt.p,
+                                                                sing.p, nsing.p);
+    DIO_HIP(hipGetLastError());
+    uint32_t ns = read1(nsing.p, s);
+    if (ns > kMaxSingular) fail(DIO_ERR_UNSUPPORTED, "too many beam pixels for sampling");
+    Singular S;
+    S.pixels.resize(ns);
+    if (ns) {
+      DIO_HIP(hipMemcpyAsync(S.pixels.data(), sing.p, ns * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, s));
+      DIO_HIP(hipStreamSynchronize(s));
+    }
+    SampleGrid unused{0, 0.0, 0.0, 0};
+    singular_prepare(g, unused, &G, S, cnt.p, s);
+    out.uncertain += S.patched;
+    out.singular = ns;
+    exclusive_scan(cnt.p, off.p, npix + 1, s, sc);
+    nnz = read1(off.p + npix, s);
+    ebin.alloc(nnz);
+    epix.alloc(nnz);
+    ew.alloc(nnz);
+    k_ps2d_fill<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(P, (uint32_t)npix, off.p, ebin.p,
+                                                               epix.p, ew.p);
+    if (ns)
+      k_sing_fill<<<blocks_for(ns, 64), 64, 0, s>>>(S.hits.p, (uint32_t)nb64, ns, S.dpix.p,
+                                                    off.p, ebin.p, epix.p, ew.p);
+    DIO_HIP(hipGetLastError());
+    DIO_HIP(hipStreamSynchronize(s));
+  }
+  if (nnz > 0x7fffffffull) fail(DIO_ERR_UNSUPPORTED, "table exceeds 2^31 entries");
+  out.n_radial = nr;
+  out.n_azimuthal = na;
+  out.hist_min = G.r0;
+  out.hist_step = G.rstep();
+  out.az_min = G.a0;
+  out.az_step = G.astep();
+  assemble((uint32_t)nb64, nnz, ebin, epix, ew, s, sc, out.full);
+  out.norm.alloc(npix);
+  k_normalization<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(
+      g, ttp.p, chp.p, d.has_polarization, d.polarization_factor, out.norm.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+}
+
+// Supersampled tables, 1D (supersampled.rs:517-595) and the default or
+// explicit-range 2D grid (methods.rs:365-388, grid.rs:579-675, 978-1028,
+// supersampled.rs:1437-1477).
+void build_ss(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, Built& out) {
+  uint32_t dim1 = d.dim1, dim2 = d.dim2;
+  uint64_t npix = (uint64_t)dim1 * dim2;
+  DevBuf<double> ttp(npix), chp(npix), radp;
+  out.uncertain += grid_angles(g, dim1, dim2, 0.5, ttp.p, chp.p, s);
+  const double* rad = ttp.p;
+  if (g.unit != 0) {
+    radp.alloc(npix);
+    k_to_unit<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(g, ttp.p, radp.p, npix);
+    rad = radp.p;
+  }
+  SsBin B{};
+  B.g = g;
+  B.nss = d.n_ss;
+  B.inv = 1;
+  uint32_t nbins;
+  if (d.ndim == 1) {
+    double min, max, dummy;
+    if (d.has_radial_range) {
+      min = d.radial_min;
+      max = d.radial_max + 10.0 * kEps;
+    } else {
+      ss_ranges(g, d.n_ss, rad, nullptr, npix, s, min, max, dummy, dummy, out.uncertain);
+    }
+    B.twod = 0;
+    B.n1 = d.n_radial;
+    B.min1 = min;
+    B.max1 = max;
+    B.step1 = (max - min) / (double)d.n_radial;
+    B.clamped1 = d.has_radial_range ? 0 : 1;
+    nbins = d.n_radial;
+    out.n_radial = d.n_radial;
+    out.n_azimuthal = 1;
+    out.hist_min = min;
+    out.hist_step = B.step1;
+  } else {
+    double rlo = d.radial_min, rhi = d.radial_max, alo = d.azimuthal_min, ahi = d.azimuthal_max;
+    if (!d.has_radial_range || !d.has_azimuthal_range) {
+      double a, b, c, e;
+      ss_ranges(g, d.n_ss, rad, chp.p, npix, s, a, b, c, e, out.uncertain);
+      if (!d.has_radial_range) {
+        rlo = a;
+        rhi = b;
+      }
+      if (!d.has_azimuthal_range) {
+        alo = c;
+        ahi = e;
+        if (!(ahi - alo > 1e-12)) {
+          alo = -kPiD;
+          ahi = kPiD;
+        }
+      }
+    }
+    Grid2 G = grid_new(d.n_radial, d.n_azimuthal, rlo, rhi, alo, ahi);
+    G.clamp_index_rounding = 1;
+    if (!d.has_radial_range) G.clamp_radial_outside = 1;
+    if (!d.has_azimuthal_range) G.clamp_azimuthal_outside = 1;
+    B.twod = 1;
+    B.G = G;
+    // The fast builder (and its * inv_ss coordinates) serves only the default
+    // clamped grid (supersampled.rs:1448-1452, 750-753); the others go through
+    // build_counted_csr (/ n_ss).
+    B.inv = (G.clamp_radial_outside && G.clamp_azimuthal_outside && G.astep() > 1e-9) ? 1 : 0;
+    nbins = d.n_radial * d.n_azimuthal;
+    out.n_radial = d.n_radial;
+    out.n_azimuthal = d.n_azimuthal;
+    out.hist_min = G.r0;
+    out.hist_step = G.rstep();
+    out.az_min = G.a0;
+    out.az_step = G.astep();
+  }
+  DevBuf<uint32_t> cnt(npix + 1), off(npix + 1), ebin, epix;
+  DIO_HIP(hipMemsetAsync(cnt.p, 0, (npix + 1) * sizeof(uint32_t), s));
+  DevBuf<double> ew;
+  uint64_t nnz = ss_entries(B, npix, s, sc, cnt, off, ebin, epix, ew, out.uncertain);
+  if (nnz > 0x7fffffffull) fail(DIO_ERR_UNSUPPORTED, "table exceeds 2^31 entries");
+  assemble(nbins, nnz, ebin, epix, ew, s, sc, out.full);
   out.norm.alloc(npix);
   k_normalization<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(
       g, ttp.p, chp.p, d.has_polarization, d.polarization_factor, out.norm.p);
@@ -843,8 +1550,9 @@ Geo make_geo(const dio_poni& p, const dio_plan_desc& d) {
 
 void build_tables(const Geo& g, const dio_plan_desc& d, hipStream_t s, Built& out) {
   Scratch sc;
-  if (d.ndim != 1) fail(DIO_ERR_UNSUPPORTED, "2D tables are not available in this build");
-  build_1d(g, d, s, sc, out);
+  if (d.method == DIO_SUPERSAMPLED) build_ss(g, d, s, sc, out);
+  else if (d.ndim == 2) build_2d(g, d, s, sc, out);
+  else build_1d(g, d, s, sc, out);
   uint64_t npix = (uint64_t)d.dim1 * d.dim2;
   bool any_mask = false;
   if (d.mask) {

@@ -21,6 +21,13 @@ struct Error : std::runtime_error {
 
 [[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
 
+}  // namespace dio
+
+// Sets the thread-local text returned by dio_last_error().
+void dio_set_last_error(const char* msg);
+
+namespace dio {
+
 inline void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) fail(DIO_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }

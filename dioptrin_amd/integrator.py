@@ -256,19 +256,17 @@ class GpuPlan:
 
     batch2d = batch1d
 
-    # -- device-resident path (torch tensors as device memory)
+    # -- device-resident path (DeviceArray frames and outputs)
     def integrate_device(self, frames, out_i, out_s=None, out_d=None, errors=False,
                          azimuthal_std=False, stream=None):
-        """Enqueues integration of device frames (torch tensor (n, dim1, dim2))
-        into device outputs (n, nbins) float64 tensors on `stream`."""
-        import torch
-        dt = _NP_DTYPES[np.dtype(str(frames.dtype).replace("torch.", ""))]
+        """Enqueues integration of device frames (DeviceArray (n, dim1, dim2))
+        into device outputs ((n, nbins) float64 DeviceArrays) on `stream`
+        (a hipStream_t handle; None = the plan's stream)."""
+        dt = _NP_DTYPES[frames.dtype]
         flags = (N.WITH_ERRORS if errors else 0) | (N.WITH_AZIMUTHAL_STD if azimuthal_std else 0)
-        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         N.check(N.lib().dio_integrate_device(
-            self._h, frames.data_ptr(), N.DTYPES[dt], frames.shape[0], flags, out_i.data_ptr(),
-            None if out_s is None else out_s.data_ptr(),
-            None if out_d is None else out_d.data_ptr(), s))
+            self._h, frames.ptr, N.DTYPES[dt], frames.shape[0], flags, out_i.ptr,
+            None if out_s is None else out_s.ptr, None if out_d is None else out_d.ptr, stream))
 
     def autotune(self, dtype="uint32", n_frames=64, errors=False) -> int:
         t = C.c_uint32()
@@ -281,9 +279,9 @@ class GpuPlan:
 
     def time_gather(self, frames, errors=False, reps=20) -> float:
         """Average ms of one launch of the segment kernel over one tile."""
-        dt = _NP_DTYPES[np.dtype(str(frames.dtype).replace("torch.", ""))]
+        dt = _NP_DTYPES[frames.dtype]
         ms = C.c_double()
-        N.check(N.lib().dio_plan_time_gather(self._h, frames.data_ptr(), N.DTYPES[dt],
+        N.check(N.lib().dio_plan_time_gather(self._h, frames.ptr, N.DTYPES[dt],
                                              frames.shape[0], N.WITH_ERRORS if errors else 0,
                                              None, None, reps, C.byref(ms)))
         return ms.value
@@ -297,13 +295,13 @@ class GpuGraph:
     """hipGraph of one device-resident integration, replayed by launch()."""
 
     def __init__(self, plan: GpuPlan, frames, out_i, out_s, out_d, errors, azimuthal_std):
-        dt = _NP_DTYPES[np.dtype(str(frames.dtype).replace("torch.", ""))]
+        dt = _NP_DTYPES[frames.dtype]
         flags = (N.WITH_ERRORS if errors else 0) | (N.WITH_AZIMUTHAL_STD if azimuthal_std else 0)
         h = C.c_void_p()
         N.check(N.lib().dio_graph_create(
-            plan._h, frames.data_ptr(), N.DTYPES[dt], frames.shape[0], flags, out_i.data_ptr(),
-            None if out_s is None else out_s.data_ptr(),
-            None if out_d is None else out_d.data_ptr(), C.byref(h)))
+            plan._h, frames.ptr, N.DTYPES[dt], frames.shape[0], flags, out_i.ptr,
+            None if out_s is None else out_s.ptr, None if out_d is None else out_d.ptr,
+            C.byref(h)))
         self._h, self.plan = h, plan
         self._keep = (frames, out_i, out_s, out_d)
 

@@ -128,19 +128,12 @@ def frame_pilatus(seed: int, shape=(1043, 981), poni=None) -> np.ndarray:
     return rng.poisson(rate).astype(np.float32)
 
 
-def frames_device(name: str, n: int, seed: int, device="cuda"):
-    """n uint32 frames generated on the device with torch (Poisson of the
-    ring pattern plus a per-frame background U[10, 500], per-frame seeds)."""
-    import torch
+def frames_device(name: str, n: int, seed: int, device: int = 0):
+    """n uint32 frames generated on the device (Poisson of the ring pattern
+    plus a per-frame background U[10, 500], per-frame seeded streams)."""
+    from .device import synthetic_poisson_u32
     w = WORKLOADS[name]
     poni = workload_poni(name)
-    rate = torch.from_numpy(ring_rate(w["shape"], poni, seed=1234, background=0.0).astype(
-        np.float32)).to(device)
-    out = torch.empty((n,) + tuple(w["shape"]), dtype=torch.int32, device=device)
-    g = torch.Generator(device=device)
-    rng = np.random.default_rng(seed)
-    for k in range(n):
-        g.manual_seed(seed * 100003 + k)
-        bg = float(rng.uniform(10.0, 500.0))
-        out[k] = torch.poisson(rate + bg, generator=g).to(torch.int32)
-    return out.view(torch.uint32) if hasattr(torch, "uint32") else out
+    rate = ring_rate(w["shape"], poni, seed=1234, background=0.0).astype(np.float32)
+    bg = np.random.default_rng(seed).uniform(10.0, 500.0, size=n)
+    return synthetic_poisson_u32(rate, bg, seed, device)

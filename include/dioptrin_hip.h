@@ -162,6 +162,23 @@ int dio_plan_time_gather(dio_plan *plan, const void *d_frames, int dtype, uint64
                          uint32_t flags, double *d_intensity, double *d_sigma, int reps,
                          double *avg_ms);
 
+/* Device memory and synchronisation for callers without their own device
+ * allocator (the reference's plans own their device buffers the same way,
+ * src/integration/cuda/device.rs:1354-1460). */
+int dio_device_alloc(dio_ctx *ctx, uint64_t bytes, void **d_ptr);
+int dio_device_free(dio_ctx *ctx, void *d_ptr);
+/* kind: 0 host to device, 1 device to host, 2 device to device. */
+int dio_memcpy(dio_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind);
+int dio_synchronize(dio_ctx *ctx);
+
+/* Seeded synthetic detector frames for benchmarks and tests (no reference
+ * counterpart): d_out[f][p] = Poisson(rate[p] + background[f]) as uint32,
+ * rate given per pixel on the device (float32), background per frame on the
+ * host; stream (f, p) draws from a counter-based generator keyed by seed. */
+int dio_synthetic_poisson_u32(dio_ctx *ctx, const float *d_rate, uint64_t npix,
+                              const double *background, uint64_t n_frames, uint64_t seed,
+                              uint32_t *d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -306,3 +306,50 @@ def ps_fractions(p: Poni, dim1, dim2, n_points, min_, step, clamp_low=False,
     if rc != 0:
         raise ValueError("invalid table request")
     return _take(c)
+
+
+def build_lut2d(p: Poni, dim1, dim2, n_radial, n_azimuthal, method="pixel_split",
+                unit="2th_rad", radial_range=None, azimuthal_range=None):
+    """Default 2D table (counting or pixel_split); returns (Lut, (r0, r1, a0, a1))."""
+    c = _Lut()
+    edges = np.empty(4)
+    rr = radial_range or (0.0, 0.0)
+    ar = azimuthal_range or (0.0, 0.0)
+    rc = lib().or_build_lut2d(C.byref(p.c()), C.c_size_t(dim1), C.c_size_t(dim2),
+                              C.c_int({"counting": 0, "pixel_split": 1}[method]),
+                              C.c_int(UNITS[unit]), C.c_size_t(n_radial), C.c_size_t(n_azimuthal),
+                              C.c_int(radial_range is not None), C.c_double(rr[0]),
+                              C.c_double(rr[1]), C.c_int(azimuthal_range is not None),
+                              C.c_double(ar[0]), C.c_double(ar[1]), C.byref(c), _d(edges))
+    if rc != 0:
+        raise ValueError("invalid 2D table request")
+    return _take(c), tuple(edges)
+
+
+def build_lut1d_ss(p: Poni, dim1, dim2, n_points, n_ss, unit="2th_rad", radial_range=None) -> Lut:
+    c = _Lut()
+    has = radial_range is not None
+    lo, hi = radial_range if has else (0.0, 0.0)
+    rc = lib().or_build_lut1d_ss(C.byref(p.c()), C.c_size_t(dim1), C.c_size_t(dim2),
+                                 C.c_size_t(n_points), C.c_size_t(n_ss), C.c_int(UNITS[unit]),
+                                 C.c_int(has), C.c_double(lo), C.c_double(hi), C.byref(c))
+    if rc != 0:
+        raise ValueError("invalid table request")
+    return _take(c)
+
+
+def build_lut2d_ss(p: Poni, dim1, dim2, n_radial, n_azimuthal, n_ss, unit="2th_rad",
+                   radial_range=None, azimuthal_range=None):
+    c = _Lut()
+    edges = np.empty(4)
+    rr = radial_range or (0.0, 0.0)
+    ar = azimuthal_range or (0.0, 0.0)
+    rc = lib().or_build_lut2d_ss(C.byref(p.c()), C.c_size_t(dim1), C.c_size_t(dim2),
+                                 C.c_size_t(n_radial), C.c_size_t(n_azimuthal), C.c_size_t(n_ss),
+                                 C.c_int(UNITS[unit]), C.c_int(radial_range is not None),
+                                 C.c_double(rr[0]), C.c_double(rr[1]),
+                                 C.c_int(azimuthal_range is not None), C.c_double(ar[0]),
+                                 C.c_double(ar[1]), C.byref(c), _d(edges))
+    if rc != 0:
+        raise ValueError("invalid 2D table request")
+    return _take(c), tuple(edges)

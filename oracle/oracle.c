@@ -589,6 +589,439 @@ int or_ps_fractions(const or_poni *p, size_t dim1, size_t dim2, size_t n_points,
   return 0;
 }
 
+/* ------------------------------------------------------------- 2D grids */
+
+/* integration/grid.rs:26-222 (IntegrationGrid2d). */
+typedef struct {
+  size_t nr, na;
+  double r0, r1, a0, a1;
+  int clamp_radial_end, clamp_index_rounding, clamp_radial_outside, clamp_azimuthal_outside;
+} grid2d;
+
+/* grid.rs:48-108: validity checks and the correction of distant starts. */
+static int grid_new(grid2d *g, size_t nr, size_t na, double rlo, double rhi, double alo,
+                    double ahi) {
+  memset(g, 0, sizeof(*g));
+  if (nr == 0 || na == 0) return 1;
+  double lo[2] = {rlo, alo}, hi[2] = {rhi, ahi};
+  size_t cnt[2] = {nr, na};
+  for (int k = 0; k < 2; k++) {
+    double step = (hi[k] - lo[k]) / (double)cnt[k];
+    if (!isfinite(lo[k]) || !isfinite(hi[k]) || !isfinite(step) || step <= 0.0 ||
+        lo[k] + step == lo[k] || hi[k] - step == hi[k])
+      return 1;
+  }
+  if (rlo < 0.0) return 1;
+  if (ahi - alo > OR_TAU + 1e-12) return 1;
+  if (!(alo >= -OR_TAU && alo <= OR_TAU)) {
+    double start = rem_euclid(alo, OR_TAU);
+    if (start >= OR_PI) start -= OR_TAU;
+    ahi = start + (ahi - alo);
+    alo = start;
+  }
+  g->nr = nr;
+  g->na = na;
+  g->r0 = rlo;
+  g->r1 = rhi;
+  g->a0 = alo;
+  g->a1 = ahi;
+  return 0;
+}
+
+static double grid_rstep(const grid2d *g) { return (g->r1 - g->r0) / (double)g->nr; }
+static double grid_astep(const grid2d *g) { return (g->a1 - g->a0) / (double)g->na; }
+
+/* grid.rs:176-221 (IntegrationGrid2d::bin); -1 for no bin. */
+static long long grid_bin(const grid2d *g, double radial, double az) {
+  double r0 = g->r0, r1 = g->r1, a0 = g->a0, a1 = g->a1;
+  if (!isfinite(radial) || !isfinite(az)) return -1;
+  if (!g->clamp_radial_outside &&
+      (radial < r0 || radial > r1 || (radial == r1 && !g->clamp_radial_end)))
+    return -1;
+  double offset;
+  if (g->clamp_azimuthal_outside) {
+    offset = az - a0;
+    double hi = a1 - a0;
+    offset = offset < 0.0 ? 0.0 : (offset > hi ? hi : offset);
+  } else if (g->clamp_index_rounding) {
+    double wrapped = (az >= a0 && az < a1) ? az : a0 + rem_euclid(az - a0, OR_TAU);
+    if (wrapped >= a1 && a1 - a0 < OR_TAU) return -1;
+    offset = wrapped - a0;
+  } else {
+    double a = a0 + rem_euclid(az - a0, OR_TAU);
+    if (a >= a1) return -1;
+    offset = a - a0;
+  }
+  unsigned long long r = sat_usize((radial - r0) / grid_rstep(g));
+  unsigned long long a = sat_usize(offset / grid_astep(g));
+  if (g->clamp_radial_end || g->clamp_index_rounding || g->clamp_radial_outside)
+    if (r > g->nr - 1) r = g->nr - 1;
+  if (g->clamp_index_rounding || g->clamp_azimuthal_outside)
+    if (a > g->na - 1) a = g->na - 1;
+  if (r < g->nr && a < g->na) return (long long)(a * g->nr + r);
+  return -1;
+}
+
+/* shared_grid.rs:202-232: 128 x 128 x 4 samples of a beam pixel on a 2D grid;
+ * weight 1/65536 per sample, summed per bin (sums of 2^-16 are exact). */
+static void sample_pixel_2d(const or_poni *p, const trig *t, size_t dim2, size_t pixel, int unit,
+                            const grid2d *g, bucket *bk) {
+  const int N = 128;
+  const double off[4][2] = {{0.25, 0.5}, {0.5, 0.25}, {0.75, 0.5}, {0.5, 0.75}};
+  size_t nb = g->nr * g->na;
+  unsigned *hits = (unsigned *)calloc(nb, sizeof(unsigned));
+  size_t row = pixel / dim2, col = pixel % dim2;
+  double cv[1];
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++)
+      for (int o = 0; o < 4; o++) {
+        double d1 = (double)row + ((double)i + off[o][0]) / (double)N;
+        double d2 = ((double)(col * N + j) + off[o][1]) / (double)N;
+        double x, y, z;
+        point_xyz(p, t, d1, d2, &x, &y, &z);
+        double tth = f_tth(x, y, z);
+        double az = f_azimuth(x, y);
+        or_convert_unit(unit, p->wavelength, &tth, 1, cv);
+        long long b = grid_bin(g, cv[0], az);
+        if (b >= 0) hits[b]++;
+      }
+  for (size_t b = 0; b < nb; b++)
+    if (hits[b]) bucket_push(&bk[b], pixel, (double)hits[b] / 65536.0);
+  free(hits);
+}
+
+/* shared_grid.rs:152-200 (split_polygon). Duplicate bins of the periodic
+ * copies are merged by adding in emission order (shift, ar, rr). */
+static void split_polygon_2d(const double (*poly)[2], double area, const grid2d *g, size_t pixel,
+                             bucket *bk) {
+  double r0 = g->r0, a0 = g->a0, dr = grid_rstep(g), da = grid_astep(g);
+  double rmin = INFINITY, rmax = -INFINITY;
+  for (int i = 0; i < 4; i++) {
+    rmin = fmin(rmin, poly[i][0]);
+    rmax = fmax(rmax, poly[i][0]);
+  }
+  long long rs = sat_isize(floor((rmin - r0) / dr));
+  long long re = sat_isize(floor((rmax - r0) / dr));
+  if (rs < 0) rs = 0;
+  if (re > (long long)g->nr - 1) re = (long long)g->nr - 1;
+  size_t cap = 16, len = 0;
+  size_t *bins = (size_t *)malloc(cap * sizeof(size_t));
+  double *ws = (double *)malloc(cap * sizeof(double));
+  const double shifts[3] = {-OR_TAU, 0.0, OR_TAU};
+  for (int s = 0; s < 3; s++) {
+    double ps[4][2];
+    double amin = INFINITY, amax = -INFINITY;
+    for (int i = 0; i < 4; i++) {
+      ps[i][0] = poly[i][0];
+      ps[i][1] = poly[i][1] + shifts[s];
+      amin = fmin(amin, ps[i][1]);
+      amax = fmax(amax, ps[i][1]);
+    }
+    long long as = sat_isize(floor((amin - a0) / da));
+    long long ae = sat_isize(floor((amax - a0) / da));
+    if (as < 0) as = 0;
+    if (ae > (long long)g->na - 1) ae = (long long)g->na - 1;
+    for (long long ar = as; ar <= ae; ar++)
+      for (long long rr = rs; rr <= re; rr++) {
+        double r = r0 + (double)rr * dr;
+        double a = a0 + (double)ar * da;
+        double clipped[24][2];
+        int m = clip_rect((const double(*)[2])ps, 4, r, r + dr, a, a + da, clipped);
+        double w = fabs(polygon_area((const double(*)[2])clipped, m)) / area;
+        if (w > 0.0) {
+          size_t bin = (size_t)ar * g->nr + (size_t)rr;
+          size_t k;
+          for (k = 0; k < len; k++)
+            if (bins[k] == bin) break;
+          if (k < len) {
+            ws[k] += w;
+          } else {
+            if (len == cap) {
+              cap *= 2;
+              bins = (size_t *)realloc(bins, cap * sizeof(size_t));
+              ws = (double *)realloc(ws, cap * sizeof(double));
+            }
+            bins[len] = bin;
+            ws[len] = w;
+            len++;
+          }
+        }
+      }
+  }
+  for (size_t k = 0; k < len; k++) bucket_push(&bk[bins[k]], pixel, ws[k]);
+  free(bins);
+  free(ws);
+}
+
+/* integration/methods.rs:365-388, grid.rs:579-675, 978-1028 and
+ * shared_grid.rs:13-150: the default 2D table of Counting and PixelSplit
+ * (or an explicit radial and/or azimuthal range). */
+int or_build_lut2d(const or_poni *p, size_t dim1, size_t dim2, int method, int unit,
+                   size_t n_radial, size_t n_azimuthal, int has_rr, double rlo, double rhi,
+                   int has_ar, double alo, double ahi, or_lut *out, double *edges) {
+  if (dim1 == 0 || dim2 == 0 || (method != 0 && method != 1)) return 1;
+  size_t nc = (dim1 + 1) * (dim2 + 1), npix = dim1 * dim2;
+  double *tth = (double *)malloc(npix * sizeof(double));
+  double *chi = (double *)malloc(npix * sizeof(double));
+  double *rad = (double *)malloc(npix * sizeof(double));
+  or_centres(p, dim1, dim2, tth, chi);
+  or_convert_unit(unit, p->wavelength, tth, npix, rad);
+  double *tc = NULL, *ac = NULL, *rc = NULL;
+  if (method == 1) {
+    tc = (double *)malloc(nc * sizeof(double));
+    ac = (double *)malloc(nc * sizeof(double));
+    rc = (double *)malloc(nc * sizeof(double));
+    or_corners(p, dim1, dim2, tc, ac);
+    or_convert_unit(unit, p->wavelength, tc, nc, rc);
+  }
+  /* default radial edges (grid.rs:978-1028) */
+  double r_lo = rlo, r_hi = rhi;
+  if (!has_rr) {
+    if (method == 0) {
+      r_lo = vmin(rad, npix);
+      r_hi = vmax(rad, npix) + 10.0 * OR_EPS;
+    } else {
+      r_lo = vmin(rc, nc);
+      r_hi = vmax(rc, nc) + 10.0 * OR_EPS;
+    }
+  }
+  /* default azimuthal edges (grid.rs:629-675) */
+  double a_lo = alo, a_hi = ahi;
+  if (!has_ar) {
+    if (method == 0) {
+      a_lo = vmin(chi, npix);
+      a_hi = vmax(chi, npix) + 10.0 * OR_EPS;
+    } else {
+      a_lo = vmin(ac, nc);
+      a_hi = vmax(ac, nc) + 10.0 * OR_EPS;
+    }
+    if (!(a_hi - a_lo > 1e-12)) {
+      a_lo = -OR_PI;
+      a_hi = OR_PI;
+    }
+  }
+  grid2d g;
+  if (grid_new(&g, n_radial, n_azimuthal, r_lo, r_hi, a_lo, a_hi)) {
+    free(tth); free(chi); free(rad); free(tc); free(ac); free(rc);
+    return 2;
+  }
+  g.clamp_index_rounding = 1;
+  if (!has_rr && method == 0) g.clamp_radial_end = 1;
+  if (edges) {
+    edges[0] = g.r0; edges[1] = g.r1; edges[2] = g.a0; edges[3] = g.a1;
+  }
+  trig t = make_trig(p);
+  size_t nb = n_radial * n_azimuthal;
+  bucket *bk = (bucket *)calloc(nb, sizeof(bucket));
+  size_t w = dim2 + 1;
+  for (size_t pix = 0; pix < npix; pix++) {
+    if (method == 0) {
+      long long b = grid_bin(&g, rad[pix], chi[pix]);
+      if (b >= 0) bucket_push(&bk[b], pix, 1.0);
+      continue;
+    }
+    size_t b0 = pix / dim2 * w + pix % dim2;
+    size_t idx[4] = {b0, b0 + 1, b0 + w + 1, b0 + w};
+    int axis_on_edge = 0, zero_r = 0;
+    double winding = 0.0;
+    for (int i = 0; i < 4; i++) {
+      double delta = ac[idx[(i + 1) % 4]] - ac[idx[i]];
+      delta = rem_euclid(delta + OR_PI, OR_TAU) - OR_PI;
+      if (fabs(delta) >= OR_PI - 64.0 * OR_EPS) axis_on_edge = 1;
+      winding += delta;
+      if (rc[idx[i]] == 0.0) zero_r = 1;
+    }
+    if (fabs(winding) > OR_PI || axis_on_edge || zero_r) {
+      sample_pixel_2d(p, &t, dim2, pix, unit, &g, bk);
+      continue;
+    }
+    double center = g.a0 + rem_euclid(chi[pix] - g.a0, OR_TAU);
+    double poly[4][2];
+    for (int i = 0; i < 4; i++) {
+      poly[i][0] = rc[idx[i]];
+      poly[i][1] = center + rem_euclid(ac[idx[i]] - center + OR_PI, OR_TAU) - OR_PI;
+    }
+    double area = fabs(polygon_area((const double(*)[2])poly, 4));
+    if (area > 0.0) {
+      split_polygon_2d((const double(*)[2])poly, area, &g, pix, bk);
+    } else {
+      long long b = grid_bin(&g, rad[pix], chi[pix]);
+      if (b >= 0) bucket_push(&bk[b], pix, 1.0);
+    }
+  }
+  buckets_to_lut(bk, nb, method == 1, g.r0, grid_rstep(&g), out);
+  free(bk);
+  free(tth); free(chi); free(rad); free(tc); free(ac); free(rc);
+  return 0;
+}
+
+/* ----------------------------------------------------------- supersampling */
+
+/* lut/common.rs:33-65 (hull_grid): the outermost subpixel centres. */
+static size_t hull_grid(size_t dim1, size_t dim2, size_t n_ss, double **d1o, double **d2o) {
+  size_t cap = 2 * dim1 * n_ss + 2 * dim2 * n_ss;
+  double *d1 = (double *)malloc(cap * sizeof(double));
+  double *d2 = (double *)malloc(cap * sizeof(double));
+  double border = 0.5 / (double)n_ss, step = 1.0 / (double)n_ss;
+  size_t k = 0;
+  for (size_t i = 0; i < dim1 * n_ss; i++) { d1[k] = border + (double)i * step; d2[k++] = border; }
+  for (size_t i = 0; i < dim1 * n_ss; i++) {
+    d1[k] = border + (double)i * step;
+    d2[k++] = (double)dim2 - border;
+  }
+  for (size_t i = 1; i < dim2 * n_ss; i++) { d1[k] = border; d2[k++] = border + (double)i * step; }
+  for (size_t i = 1; i < dim2 * n_ss; i++) {
+    d1[k] = (double)dim1 - border;
+    d2[k++] = border + (double)i * step;
+  }
+  *d1o = d1;
+  *d2o = d2;
+  return k;
+}
+
+/* histogram.rs:151-181: range over pixel centres and the hull grid, + 3 eps. */
+static void ss_ranges(const or_poni *p, size_t dim1, size_t dim2, size_t n_ss, int unit,
+                      const double *rad, const double *chi, size_t npix, double *rlo,
+                      double *rhi, double *alo, double *ahi) {
+  double *hd1, *hd2;
+  size_t nh = hull_grid(dim1, dim2, n_ss, &hd1, &hd2);
+  double *ht = (double *)malloc((nh ? nh : 1) * sizeof(double));
+  double *ha = (double *)malloc((nh ? nh : 1) * sizeof(double));
+  double *hr = (double *)malloc((nh ? nh : 1) * sizeof(double));
+  or_tth_chi(p, hd1, hd2, nh, ht, ha);
+  or_convert_unit(unit, p->wavelength, ht, nh, hr);
+  *rlo = fmin(vmin(rad, npix), vmin(hr, nh));
+  *rhi = fmax(vmax(rad, npix), vmax(hr, nh)) + 3.0 * OR_EPS;
+  if (chi) {
+    *alo = fmin(vmin(chi, npix), vmin(ha, nh));
+    *ahi = fmax(vmax(chi, npix), vmax(ha, nh)) + 3.0 * OR_EPS;
+  }
+  free(hd1); free(hd2); free(ht); free(ha); free(hr);
+}
+
+/* Per-pixel subpixel counts (supersampled.rs:21-106, 171-500, 731-1379):
+ * every subpixel is evaluated exactly; `inv` selects the fast builders'
+ * coordinates rf + (i + 0.5) * inv_ss (else (i + 0.5) / n_ss). A weight is
+ * count / n_ss^2. */
+static void ss_count(const or_poni *p, size_t dim1, size_t dim2, size_t n_ss, int unit, int inv,
+                     int twod, size_t n1, double min1, double max1, double step1, int clamped1,
+                     const grid2d *g, bucket *bk) {
+  trig t = make_trig(p);
+  double inv_ss = 1.0 / (double)n_ss;
+  size_t cap = n_ss * n_ss;
+  size_t *bins = (size_t *)malloc(cap * sizeof(size_t));
+  size_t *cnts = (size_t *)malloc(cap * sizeof(size_t));
+  double cv[1];
+  for (size_t row = 0; row < dim1; row++)
+    for (size_t col = 0; col < dim2; col++) {
+      size_t len = 0;
+      for (size_t i = 0; i < n_ss; i++) {
+        double d1 = inv ? (double)row + ((double)i + 0.5) * inv_ss
+                        : (double)row + ((double)i + 0.5) / (double)n_ss;
+        for (size_t j = 0; j < n_ss; j++) {
+          double d2 = inv ? ((double)(col * n_ss + j) + 0.5) * inv_ss
+                          : ((double)(col * n_ss + j) + 0.5) / (double)n_ss;
+          double x, y, z;
+          point_xyz(p, &t, d1, d2, &x, &y, &z);
+          double tth = f_tth(x, y, z);
+          or_convert_unit(unit, p->wavelength, &tth, 1, cv);
+          double r = cv[0];
+          long long b;
+          if (twod) {
+            b = grid_bin(g, r, f_azimuth(x, y));
+          } else if (clamped1) {
+            unsigned long long u = sat_usize((r - min1) / step1);
+            b = (long long)(u > n1 - 1 ? n1 - 1 : u);
+          } else {
+            if (!(r >= min1 && r <= max1)) continue;
+            unsigned long long u = sat_usize((r - min1) / step1);
+            b = (long long)(u > n1 - 1 ? n1 - 1 : u);
+          }
+          if (b < 0) continue;
+          size_t k;
+          for (k = 0; k < len; k++)
+            if (bins[k] == (size_t)b) break;
+          if (k < len) cnts[k]++;
+          else { bins[len] = (size_t)b; cnts[len] = 1; len++; }
+        }
+      }
+      for (size_t k = 0; k < len; k++)
+        bucket_push(&bk[bins[k]], row * dim2 + col, (double)cnts[k] / (double)(n_ss * n_ss));
+    }
+  free(bins);
+  free(cnts);
+}
+
+/* supersampled.rs:517-595 (build_lut1d_ss_csr and _with_range). */
+int or_build_lut1d_ss(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, size_t n_ss,
+                      int unit, int has_range, double rmin, double rmax, or_lut *out) {
+  if (n_points == 0 || dim1 == 0 || dim2 == 0 || n_ss == 0) return 1;
+  size_t npix = dim1 * dim2;
+  double *tth = (double *)malloc(npix * sizeof(double));
+  double *rad = (double *)malloc(npix * sizeof(double));
+  or_centres(p, dim1, dim2, tth, NULL);
+  or_convert_unit(unit, p->wavelength, tth, npix, rad);
+  double min, max, step, dummy;
+  if (has_range) {
+    min = rmin;
+    max = rmax + 10.0 * OR_EPS;
+  } else {
+    ss_ranges(p, dim1, dim2, n_ss, unit, rad, NULL, npix, &min, &max, &dummy, &dummy);
+  }
+  step = (max - min) / (double)n_points;
+  bucket *bk = (bucket *)calloc(n_points, sizeof(bucket));
+  ss_count(p, dim1, dim2, n_ss, unit, 1, 0, n_points, min, max, step, !has_range, NULL, bk);
+  buckets_to_lut(bk, n_points, 1, min, step, out);
+  free(bk);
+  free(tth);
+  free(rad);
+  return 0;
+}
+
+/* Default (or explicit-range) supersampled 2D table: methods.rs:365-388,
+ * grid.rs:579-675, 978-1028, supersampled.rs:1437-1477. */
+int or_build_lut2d_ss(const or_poni *p, size_t dim1, size_t dim2, size_t n_radial,
+                      size_t n_azimuthal, size_t n_ss, int unit, int has_rr, double rlo,
+                      double rhi, int has_ar, double alo, double ahi, or_lut *out,
+                      double *edges) {
+  if (dim1 == 0 || dim2 == 0 || n_ss == 0) return 1;
+  size_t npix = dim1 * dim2;
+  double *tth = (double *)malloc(npix * sizeof(double));
+  double *chi = (double *)malloc(npix * sizeof(double));
+  double *rad = (double *)malloc(npix * sizeof(double));
+  or_centres(p, dim1, dim2, tth, chi);
+  or_convert_unit(unit, p->wavelength, tth, npix, rad);
+  double s_rlo, s_rhi, s_alo, s_ahi;
+  ss_ranges(p, dim1, dim2, n_ss, unit, rad, chi, npix, &s_rlo, &s_rhi, &s_alo, &s_ahi);
+  double r_lo = has_rr ? rlo : s_rlo, r_hi = has_rr ? rhi : s_rhi;
+  double a_lo = alo, a_hi = ahi;
+  if (!has_ar) {
+    a_lo = s_alo;
+    a_hi = s_ahi;
+    if (!(a_hi - a_lo > 1e-12)) {
+      a_lo = -OR_PI;
+      a_hi = OR_PI;
+    }
+  }
+  grid2d g;
+  if (grid_new(&g, n_radial, n_azimuthal, r_lo, r_hi, a_lo, a_hi)) {
+    free(tth); free(chi); free(rad);
+    return 2;
+  }
+  g.clamp_index_rounding = 1;
+  if (!has_rr) g.clamp_radial_outside = 1;
+  if (!has_ar) g.clamp_azimuthal_outside = 1;
+  int fast = g.clamp_radial_outside && g.clamp_azimuthal_outside && grid_astep(&g) > 1e-9;
+  if (edges) { edges[0] = g.r0; edges[1] = g.r1; edges[2] = g.a0; edges[3] = g.a1; }
+  size_t nb = n_radial * n_azimuthal;
+  bucket *bk = (bucket *)calloc(nb, sizeof(bucket));
+  ss_count(p, dim1, dim2, n_ss, unit, fast, 1, 0, 0.0, 0.0, 0.0, 0, &g, bk);
+  buckets_to_lut(bk, nb, 1, g.r0, grid_rstep(&g), out);
+  free(bk);
+  free(tth); free(chi); free(rad);
+  return 0;
+}
+
 /* --------------------------------------------------- execution and gather */
 
 /* prepared.rs:163-193: entries of masked pixels dropped, order kept. An

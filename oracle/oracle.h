@@ -15,6 +15,9 @@
  *   counting 1D     src/lut/counting_csr.rs:54-134
  *   pixel split 1D  src/lut/pixel_split.rs:14-455
  *   beam pixels     src/lut/pyfai_1d.rs:180-252, integration/grid.rs:48-160
+ *   2D tables       src/lut/shared_grid.rs:13-232, integration/grid.rs:26-222,
+ *                   579-675, 978-1028, integration/methods.rs:365-388
+ *   supersampling   src/lut/supersampled.rs:21-595, 1381-1477, lut/common.rs:33-65
  *   masked table    src/lut/prepared.rs:163-262
  *   gather          src/lut/prepared.rs:985-1121
  *   azimuthal std   src/lut/azimuthal.rs:48-118
@@ -71,6 +74,19 @@ int or_counting_from_radial(const double *rad, size_t npix, size_t n_points, int
                             double rmin, double rmax, or_lut *out);
 int or_ps_fractions(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, int unit,
                     double min, double step, int clamp_low, or_lut *out);
+/* Default (or explicit-range) 2D shared-grid table of Counting (method 0) or
+ * PixelSplit (method 1); rows are az * n_radial + r. edges (may be NULL)
+ * receives r0, r1, a0, a1 of the grid. */
+int or_build_lut2d(const or_poni *p, size_t dim1, size_t dim2, int method, int unit,
+                   size_t n_radial, size_t n_azimuthal, int has_rr, double rlo, double rhi,
+                   int has_ar, double alo, double ahi, or_lut *out, double *edges);
+/* Supersampled tables (n_ss x n_ss subpixels, weight count / n_ss^2). */
+int or_build_lut1d_ss(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, size_t n_ss,
+                      int unit, int has_range, double rmin, double rmax, or_lut *out);
+int or_build_lut2d_ss(const or_poni *p, size_t dim1, size_t dim2, size_t n_radial,
+                      size_t n_azimuthal, size_t n_ss, int unit, int has_rr, double rlo,
+                      double rhi, int has_ar, double alo, double ahi, or_lut *out,
+                      double *edges);
 void or_lut_free(or_lut *lut);
 
 /* Masked execution table: entries of masked pixels removed, order kept. */

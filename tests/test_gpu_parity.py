@@ -207,23 +207,145 @@ def test_edge_cases_masked_inf_empty():
 
 
 def test_device_resident_path_and_graph_replay(pilatus):
-    import torch
+    from dioptrin_amd.device import DeviceArray, synchronize
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
                    polarization_factor=0.99, mask=d["mask"])
-    frames = torch.from_numpy(d["frames"]).cuda()
+    frames = DeviceArray.from_numpy(d["frames"])
     n = frames.shape[0]
-    out = torch.empty((n, plan.nbins), dtype=torch.float64, device="cuda")
+    out = DeviceArray((n, plan.nbins), np.float64)
     plan.integrate_device(frames, out)
-    torch.cuda.synchronize()
+    synchronize()
     host, _, _ = plan.integrate_stack(d["frames"])
-    assert np.array_equal(out.cpu().numpy(), host)
-    g_out = torch.zeros_like(out)
+    eager = out.numpy()
+    assert np.array_equal(eager, host)
+    g_out = DeviceArray((n, plan.nbins), np.float64)
     graph = plan.capture_graph(frames, g_out)
     for rep in range(3):
         g_out.zero_()
-        torch.cuda.synchronize()
+        synchronize()
         graph.launch()
-        torch.cuda.synchronize()
-        _ = (frames.float() * 2).sum()  # unrelated eager work between replays
-        assert torch.equal(g_out, out), rep
+        synchronize()
+        plan.integrate_stack(d["frames"][:1])  # unrelated eager work between replays
+        assert np.array_equal(g_out.numpy(), eager), rep
+
+
+def test_device_synthetic_frames_are_poisson():
+    from dioptrin_amd.device import synthetic_poisson_u32
+    rate = np.full((256, 256), 40.0, np.float32)
+    rate[:128] = 3.0
+    fr = synthetic_poisson_u32(rate, [0.0, 100.0], seed=7).numpy().astype(np.float64)
+    for f, bg in ((0, 0.0), (1, 100.0)):
+        for sl, lam in ((slice(0, 128), 3.0 + bg), (slice(128, 256), 40.0 + bg)):
+            x = fr[f, sl]
+            assert abs(x.mean() - lam) < 5 * math.sqrt(lam / x.size)
+            assert abs(x.var() / lam - 1.0) < 0.05
+    again = synthetic_poisson_u32(rate, [0.0, 100.0], seed=7).numpy()
+    assert np.array_equal(again.astype(np.float64), fr)
+
+
+# --------------------------------------------------------------- 2D tables
+
+GEOMETRIES_2D = [
+    # name, poni, shape, n_radial, n_azimuthal, method, unit, radial_range, azimuthal_range
+    ("beam_64x80_ps", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                           poni2=36 * 172e-6, wavelength=1e-10), (64, 80), 32, 36,
+     "pixel_split", "2th_rad", None, None),
+    ("beam_64x80_full_circle", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05,
+                                    poni1=24 * 172e-6, poni2=36 * 172e-6, wavelength=1e-10),
+     (64, 80), 24, 36, "pixel_split", "2th_deg", (0.0, 12.0), (-math.pi, math.pi)),
+    ("tilted_ps", dict(pixel1=1e-4, pixel2=1.2e-4, distance=0.05, poni1=-0.002, poni2=-0.001,
+                       rot1=0.01, rot2=-0.02, wavelength=1e-10), (15, 17), 20, 72,
+     "pixel_split", "2th_rad", None, (-math.pi, math.pi)),
+    ("counting_2d", dict(pixel1=1e-4, pixel2=1e-4, distance=0.05, poni1=4.5e-4, poni2=5.5e-4,
+                         wavelength=1e-10), (9, 11), 10, 8, "counting", "2th_rad", None, None),
+    ("pilatus_tilted_500x360", dict(pixel1=172e-6, pixel2=172e-6, distance=0.18,
+                                    poni1=1043 * 172e-6 / 2, poni2=981 * 172e-6 / 2, rot1=0.05,
+                                    rot2=-0.03, rot3=0.10, wavelength=0.3344e-10), (1043, 981),
+     500, 360, "pixel_split", "2th_rad", None, None),
+]
+
+
+@pytest.mark.parametrize("g", GEOMETRIES_2D, ids=[g[0] for g in GEOMETRIES_2D])
+def test_table_2d_bit_exact(g):
+    name, poni, shape, nr, na, method, unit, rr, ar = g
+    print(f"case 2d {name}", flush=True)
+    plan = GpuPlan(poni, shape, method=method, unit=unit, n_radial=nr, n_azimuthal=na,
+                   radial_range=rr, azimuthal_range=ar)
+    lut, (r0, r1, a0, a1) = O.build_lut2d(O.Poni(**poni), shape[0], shape[1], nr, na, method,
+                                          unit, rr, ar)
+    assert_same_table(plan, lut)
+    assert plan.info["hist_min"] == r0 and plan.info["hist_step"] == (r1 - r0) / nr
+    assert np.array_equal(plan.azimuthal, a0 + (np.arange(na) + 0.5) * ((a1 - a0) / na))
+
+
+def test_integrate2d_matches_oracle():
+    poni = dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                poni2=36 * 172e-6, wavelength=1e-10)
+    shape = (64, 80)
+    mask = np.zeros(shape, np.uint8)
+    mask[10:14, :] = 1
+    plan = GpuPlan(poni, shape, n_radial=32, n_azimuthal=36, polarization_factor=0.95,
+                   mask=mask, precision="f32_compensated")
+    lut, _ = O.build_lut2d(O.Poni(**poni), *shape, 32, 36)
+    fixed = O.fixed_table(lut, mask)
+    norm = O.normalization(O.Poni(**poni), *shape, 0.95)
+    den = O.denominators(fixed, norm)
+    frame = np.random.default_rng(3).poisson(500.0, size=shape).astype(np.uint32)
+    r = plan.integrate2d(frame, errors=True)
+    i, s = O.gather(fixed, den, frame.astype(np.float64))
+    mag, _ = O.gather(fixed, den, frame.astype(np.float64), False)
+    assert r.intensity.shape == (36, 32)
+    assert np.all(np.abs(r.intensity.ravel() - i) <= 2e-6 * np.abs(mag))
+    assert np.all(np.abs(r.errors.ravel() - s) <= 2e-6 * s)
+
+
+# ------------------------------------------------------------- supersampled
+
+SS_CASES = [
+    # name, poni, shape, nr, na (None = 1D), n_ss, unit, radial_range
+    ("ss1d_10x10", dict(pixel1=1e-4, pixel2=1e-4, distance=0.1, poni1=0.0, poni2=0.0,
+                        wavelength=1e-10), (10, 10), 7, None, 4, "2th_rad", None),
+    ("ss1d_range", dict(pixel1=1e-4, pixel2=1e-4, distance=0.1, poni1=0.0, poni2=0.0,
+                        wavelength=1e-10), (10, 10), 6, None, 4, "2th_rad", (0.004, 0.009)),
+    ("ss1d_tilted_deg", dict(pixel1=1e-4, pixel2=1.2e-4, distance=0.05, poni1=5e-4, poni2=7e-4,
+                             rot1=0.02, rot2=-0.01, wavelength=1e-10), (11, 13), 12, None, 3,
+     "2th_deg", None),
+    ("ss2d_tilted", dict(pixel1=1e-4, pixel2=1.2e-4, distance=0.05, poni1=5e-4, poni2=7e-4,
+                         rot1=0.02, rot2=-0.01, wavelength=1e-10), (11, 13), 12, 16, 3,
+     "2th_rad", None),
+    ("ss2d_range", dict(pixel1=1e-4, pixel2=1.2e-4, distance=0.05, poni1=5e-4, poni2=7e-4,
+                        rot1=0.02, rot2=-0.01, wavelength=1e-10), (11, 13), 12, 16, 3,
+     "2th_rad", (0.001, 0.03)),
+]
+
+
+@pytest.mark.parametrize("g", SS_CASES, ids=[g[0] for g in SS_CASES])
+def test_supersampled_table_bit_exact(g):
+    name, poni, shape, nr, na, nss, unit, rr = g
+    print(f"case {name}", flush=True)
+    plan = GpuPlan(poni, shape, method="supersampled", n_ss=nss, unit=unit, n_radial=nr,
+                   n_azimuthal=na, radial_range=rr)
+    P = O.Poni(**poni)
+    if na is None:
+        lut = O.build_lut1d_ss(P, *shape, nr, nss, unit, rr)
+    else:
+        lut, _ = O.build_lut2d_ss(P, *shape, nr, na, nss, unit, rr)
+    assert_same_table(plan, lut)
+
+
+@pytest.mark.parametrize("ndim", [1, 2])
+def test_supersampled_full_size_bit_exact(ndim):
+    w = synth.WORKLOADS["eiger4m_ss2d"]
+    poni = synth.workload_poni("eiger4m_ss2d")
+    print(f"case eiger4m_ss2d ndim {ndim}", flush=True)
+    na = w["n_azimuthal"] if ndim == 2 else None
+    plan = GpuPlan(poni, w["shape"], method="supersampled", n_ss=w["n_ss"],
+                   n_radial=w["n_radial"], n_azimuthal=na)
+    P = O.Poni(**poni)
+    if ndim == 1:
+        lut = O.build_lut1d_ss(P, *w["shape"], w["n_radial"], w["n_ss"])
+    else:
+        lut, _ = O.build_lut2d_ss(P, *w["shape"], w["n_radial"], na, w["n_ss"])
+    assert_same_table(plan, lut)
+    print(f"  nnz {lut.nnz} build {plan.info['build_ms']:.1f} ms", flush=True)

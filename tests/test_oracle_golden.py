@@ -291,3 +291,98 @@ def test_masked_table_keeps_order():
     assert list(f.row_ptr) == [0, 2, 3]
     assert list(f.col) == [4, 2, 0]
     assert list(f.w) == [0.0, 2.0, 3.0]
+
+
+# ---------------------------------------------------------------- 2D tables
+
+BEAM = O.Poni(172e-6, 172e-6, 0.05, 24 * 172e-6, 36 * 172e-6, wavelength=1e-10)
+
+
+def test_2d_full_circle_conserves_every_pixel():
+    # Explicit edges covering all pixels and the full circle: every pixel's
+    # fractions add up to one (shared_grid.rs:307-354 conservation).
+    t, _ = O.corners(BEAM, 64, 80)
+    lut, _ = O.build_lut2d(BEAM, 64, 80, 24, 36, "pixel_split",
+                           radial_range=(0.0, t.max() * 1.01),
+                           azimuthal_range=(-math.pi, math.pi))
+    tot = np.bincount(lut.col, weights=lut.w, minlength=64 * 80)
+    assert np.abs(tot - 1.0).max() < 1e-9
+
+
+def test_2d_summed_over_azimuth_equals_1d():
+    # Same radial edges (corner range + 10 eps) and the full circle: summing a
+    # pixel's 2D fractions over azimuth gives its 1D fraction per radial bin.
+    p = O.Poni(1e-4, 1.2e-4, 0.05, -0.002, -0.001, 0.01, -0.02, 0.0, 1e-10)
+    dim1, dim2, nr = 15, 17, 20
+    l1 = O.build_lut1d(p, dim1, dim2, nr)
+    l2, edges = O.build_lut2d(p, dim1, dim2, nr, 72, "pixel_split",
+                              azimuthal_range=(-math.pi, math.pi))
+    assert edges[0] == l1.hist_min
+    r1 = np.repeat(np.arange(nr), np.diff(l1.row_ptr).astype(int))
+    rows2 = np.repeat(np.arange(nr * 72), np.diff(l2.row_ptr).astype(int)) % nr
+    a = np.zeros((dim1 * dim2, nr))
+    b = np.zeros((dim1 * dim2, nr))
+    np.add.at(a, (l1.col.astype(int), r1), l1.w)
+    np.add.at(b, (l2.col.astype(int), rows2), l2.w)
+    assert np.abs(a - b).max() < 1e-9
+
+
+def test_2d_counting_matches_brute_force_bins():
+    p = O.Poni(1e-4, 1e-4, 0.05, 4.5e-4, 5.5e-4, wavelength=1e-10)
+    lut, (r0, r1, a0, a1) = O.build_lut2d(p, 9, 11, 10, 8, "counting")
+    tth, chi = O.centres(p, 9, 11)
+    assert r0 == tth.min() and r1 == tth.max() + 10 * np.finfo(float).eps
+    assert a0 == chi.min() and a1 == chi.max() + 10 * np.finfo(float).eps
+    ir = np.minimum(((tth - r0) / ((r1 - r0) / 10)).astype(int), 9)
+    ia = np.minimum(((chi - a0) / ((a1 - a0) / 8)).astype(int), 7)
+    expect = ia * 10 + ir
+    rows = np.repeat(np.arange(80), np.diff(lut.row_ptr).astype(int))
+    got = np.empty(99, int)
+    got[lut.col.astype(int)] = rows
+    assert lut.nnz == 99 and np.array_equal(got, expect)
+
+
+# ------------------------------------------------------------- supersampled
+# supersampled.rs:1545-1552 test geometry and 1737-1790 known answers.
+
+SS_GEO = O.Poni(1e-4, 1e-4, 0.1, 0.0, 0.0, wavelength=1e-10)
+
+
+def test_ss_weights_1d_are_fractions_of_the_pixel():
+    lut = O.build_lut1d_ss(SS_GEO, 10, 10, 7, 4)
+    tot = np.zeros(100)
+    for c, w in zip(lut.col, lut.w):  # sequential sums, as the reference test adds
+        tot[c] += w
+    assert np.all(tot == 1.0)
+
+
+def test_ss_explicit_range_drops_subpixels_outside_it():
+    n_ss, n_points, lo, hi = 4, 6, 0.004, 0.009
+    lut = O.build_lut1d_ss(SS_GEO, 10, 10, n_points, n_ss, radial_range=(lo, hi))
+    expected = np.zeros(100)
+    expected_bins = np.zeros(n_points, int)
+    for pixel in range(100):
+        row, col = divmod(pixel, 10)
+        d1 = np.array([row + (i + 0.5) / n_ss for i in range(n_ss) for _ in range(n_ss)])
+        d2 = np.array([((col * n_ss + j) + 0.5) / n_ss for _ in range(n_ss) for j in range(n_ss)])
+        r, _ = O.tth_chi(SS_GEO, d1, d2)
+        for v in r:
+            if lo <= v <= hi:
+                expected[pixel] += 1.0 / (n_ss * n_ss)
+                expected_bins[int(((v - lo) / (hi - lo)) * n_points)] += 1
+    tot = np.zeros(100)
+    for c, w in zip(lut.col, lut.w):
+        tot[c] += w
+    assert np.array_equal(tot, expected)
+    assert 1.0 in tot and 0.0 in tot and np.any((tot > 0) & (tot < 1))
+    for b in (0, n_points - 1):
+        assert lut.w[lut.row_ptr[b]:lut.row_ptr[b + 1]].sum() == expected_bins[b] / n_ss ** 2
+
+
+def test_ss_2d_totals_and_1d_consistency():
+    p = O.Poni(1e-4, 1.2e-4, 0.05, 5e-4, 7e-4, 0.02, -0.01, 0.0, 1e-10)
+    l2, (r0, r1, a0, a1) = O.build_lut2d_ss(p, 11, 13, 12, 16, 3)
+    tot = np.bincount(l2.col, weights=l2.w, minlength=143)
+    assert np.abs(tot - 1.0).max() < 1e-12   # clamped default grid keeps every subpixel
+    l1 = O.build_lut1d_ss(p, 11, 13, 12, 3)
+    assert l1.hist_min == r0 and abs(l1.hist_step - (r1 - r0) / 12) <= 1e-18
