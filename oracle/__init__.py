@@ -353,3 +353,44 @@ def build_lut2d_ss(p: Poni, dim1, dim2, n_radial, n_azimuthal, n_ss, unit="2th_r
     if rc != 0:
         raise ValueError("invalid 2D table request")
     return _take(c), tuple(edges)
+
+
+def build_lut1d_pyfai(p: Poni, dim1, dim2, n_points, unit="2th_rad", radial_range=None) -> Lut:
+    c = _Lut()
+    has = radial_range is not None
+    lo, hi = radial_range if has else (0.0, 0.0)
+    rc = lib().or_build_lut1d_pyfai(C.byref(p.c()), C.c_size_t(dim1), C.c_size_t(dim2),
+                                    C.c_size_t(n_points), C.c_int(UNITS[unit]), C.c_int(has),
+                                    C.c_double(lo), C.c_double(hi), C.byref(c))
+    if rc not in (0, 3):
+        raise ValueError("invalid table request")
+    lut = _take(c)
+    if rc == 3:
+        raise ValueError("pyFAI-compatible splitting exceeds the 64-million-entry bound")
+    return lut
+
+
+def build_lut2d_pyfai(p: Poni, dim1, dim2, n_radial, n_azimuthal, unit="2th_rad",
+                      radial_range=None, azimuthal_range=None):
+    c = _Lut()
+    edges = np.empty(4)
+    rr = radial_range or (0.0, 0.0)
+    ar = azimuthal_range or (0.0, 0.0)
+    rc = lib().or_build_lut2d_pyfai(C.byref(p.c()), C.c_size_t(dim1), C.c_size_t(dim2),
+                                    C.c_int(UNITS[unit]), C.c_size_t(n_radial),
+                                    C.c_size_t(n_azimuthal), C.c_int(radial_range is not None),
+                                    C.c_double(rr[0]), C.c_double(rr[1]),
+                                    C.c_int(azimuthal_range is not None), C.c_double(ar[0]),
+                                    C.c_double(ar[1]), C.byref(c), _d(edges))
+    if rc != 0:
+        raise ValueError("invalid 2D table request")
+    return _take(c), tuple(edges)
+
+
+def pyfai_split1d(poly, r0, r1, nbins):
+    p = np.ascontiguousarray(poly, np.float64).ravel()
+    b = np.empty(64, np.uint64)
+    w = np.empty(64)
+    n = lib().or_pyfai_split1d(_d(p), C.c_double(r0), C.c_double(r1), C.c_size_t(nbins),
+                               b.ctypes.data_as(C.c_void_p), _d(w), C.c_size_t(64))
+    return list(zip(b[:n].tolist(), w[:n].tolist()))

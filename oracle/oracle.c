@@ -1022,6 +1022,309 @@ int or_build_lut2d_ss(const or_poni *p, size_t dim1, size_t dim2, size_t n_radia
   return 0;
 }
 
+/* -------------------------------------------------- pyFAI-compatible split */
+
+/* pyfai_1d.rs:36-46: [max(min corner, 0), max corner * (1 + f32::EPSILON)]. */
+static void pyfai_range(const double *rc, size_t nc, double *lo, double *hi) {
+  double mn = INFINITY, mx = -INFINITY;
+  for (size_t i = 0; i < nc; i++) {
+    mn = fmin(mn, rc[i]);
+    mx = fmax(mx, rc[i]);
+  }
+  *lo = fmax(mn, 0.0);
+  *hi = mx * (1.0 + (double)FLT_EPSILON);
+}
+
+/* pyfai_1d.rs:259-354 (split1d) with sector = false and sign = +1 (no
+ * detector: corner_sign, :131-154). Appends the pixel's entries. */
+static void split1d(double poly[4][2], double r0, double r1, size_t bins, size_t pixel,
+                    bucket *bk) {
+  double a0 = -OR_PI, a1 = OR_PI;
+  double offset = (a0 >= -OR_PI && a1 <= OR_PI) ? 0.0
+                  : (a0 >= 0.0 && a1 <= OR_TAU) ? OR_PI : a0 + OR_PI;
+  for (int i = 0; i < 4; i++) poly[i][1] = wrap_tau(poly[i][1] - offset + OR_PI) - OR_PI;
+  double *a = poly[0], *b = poly[1], *c = poly[2], *d = poly[3];
+  double area = 1.0 * (0.5 * (c[0] - a[0]) * (d[1] - b[1]) - (c[1] - a[1]) * (d[0] - b[0]));
+  if (area > 0.0) {
+    for (int i = 0; i < 4; i++)
+      if (poly[i][1] < 0.0) poly[i][1] += OR_TAU;
+    double s = 0.0;
+    s = s + poly[0][1];
+    s = s + poly[1][1];
+    s = s + poly[2][1];
+    s = s + poly[3][1];
+    if (s * 0.25 > OR_PI)
+      for (int i = 0; i < 4; i++) poly[i][1] -= OR_TAU;
+  }
+  for (int i = 0; i < 4; i++) poly[i][1] += offset;
+  double dr = (r1 - r0) / (double)bins;
+  for (int i = 0; i < 4; i++) poly[i][0] = (poly[i][0] - r0) / dr;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = 0; i < 4; i++) {
+    lo = fmin(lo, poly[i][0]);
+    hi = fmax(hi, poly[i][0]);
+  }
+  if (hi < 0.0 || lo >= (double)bins) return;
+  size_t first = (size_t)sat_usize(fmax(floor(lo), 0.0));
+  long long e = sat_isize(floor(hi)) + 1;
+  if (e < 0) e = 0;
+  if (e > (long long)bins) e = (long long)bins;
+  size_t end = (size_t)e;
+  if (floor(lo) == floor(hi)) {
+    bucket_push(&bk[first], pixel, 1.0);
+    return;
+  }
+  size_t n = end > first ? end - first : 0;
+  float *buf = (float *)calloc(n ? n : 1, sizeof(float));
+  for (int i = 0; i < 4; i++) {
+    double x0 = poly[i][0], y0 = poly[i][1], x1 = poly[(i + 1) % 4][0], y1 = poly[(i + 1) % 4][1];
+    if (x0 == x1) continue;
+    double slope = (y1 - y0) / (x1 - x0);
+    double intercept = y0 - slope * x0;
+    double sign = x1 > x0 ? 1.0 : -1.0;
+    for (size_t j = 0; j < n; j++) {
+      double left = fmax(fmin(x0, x1), (double)(first + j));
+      double right = fmin(fmax(x0, x1), (double)(first + j + 1));
+      if (right > left)
+        buf[j] = buf[j] + (float)(sign * (right - left) * (0.5 * slope * (right + left) + intercept));
+    }
+  }
+  double sum = 0.0;
+  for (size_t j = 0; j < n; j++) sum += (double)buf[j];
+  if (!isfinite(sum) || sum == 0.0) {
+    free(buf);
+    return;
+  }
+  double total = 0.0;
+  for (size_t j = 0; j < n; j++) {
+    double w = (double)buf[j] / sum;
+    if (isfinite(w) && w > 0.0) total += w;
+  }
+  for (size_t j = 0; j < n; j++) {
+    double w = (double)buf[j] / sum;
+    if (isfinite(w) && w > 0.0) bucket_push(&bk[first + j], pixel, w / total);
+  }
+  free(buf);
+}
+
+/* pyfai_1d.rs:55-109 (build_lut1d_pyfai_csr) with the full azimuth. */
+int or_build_lut1d_pyfai(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, int unit,
+                         int has_range, double rmin, double rmax, or_lut *out) {
+  if (n_points == 0 || dim1 == 0 || dim2 == 0) return 1;
+  size_t nc = (dim1 + 1) * (dim2 + 1), npix = dim1 * dim2;
+  double *tc = (double *)malloc(nc * sizeof(double));
+  double *ac = (double *)malloc(nc * sizeof(double));
+  double *rc = (double *)malloc(nc * sizeof(double));
+  or_corners(p, dim1, dim2, tc, ac);
+  or_convert_unit(unit, p->wavelength, tc, nc, rc);
+  double r0 = rmin, r1 = rmax;
+  if (!has_range) pyfai_range(rc, nc, &r0, &r1);
+  if (!sample_grid_ok(n_points, r0, r1)) {
+    free(tc); free(ac); free(rc);
+    return 2;
+  }
+  trig t = make_trig(p);
+  bucket *bk = (bucket *)calloc(n_points, sizeof(bucket));
+  size_t w = dim2 + 1, entries = 0;
+  for (size_t pix = 0; pix < npix; pix++) {
+    size_t b = pix / dim2 * w + pix % dim2;
+    size_t idx[4] = {b, b + w, b + w + 1, b + 1}; /* pyFAI order a, d, c, b */
+    double poly[4][2];
+    for (int i = 0; i < 4; i++) {
+      poly[i][0] = rc[idx[i]];
+      poly[i][1] = ac[idx[i]];
+    }
+    if (singular((const double(*)[2])poly)) sample_pixel(p, &t, dim2, pix, unit, n_points, r0, r1,
+                                                          !has_range, bk);
+    else split1d(poly, r0, r1, n_points, pix, bk);
+  }
+  for (size_t k = 0; k < n_points; k++) entries += bk[k].len;
+  buckets_to_lut(bk, n_points, 1, r0, (r1 - r0) / (double)n_points, out);
+  free(bk);
+  free(tc); free(ac); free(rc);
+  return entries > 64u * 1024u * 1024u ? 3 : 0;
+}
+
+/* pyfai_split.rs:139-167 (line): signed strip areas distributed down the
+ * azimuthal rows of each radial column, in float32. */
+static void pyfai_line(float *buf, size_t stride, const double *start, const double *stop) {
+  if (start[0] == stop[0]) return;
+  float slope = (float)((stop[1] - start[1]) / (stop[0] - start[0]));
+  float intercept = (float)(stop[1] - (double)slope * stop[0]);
+  double lo = fmin(start[0], stop[0]), hi = fmax(start[0], stop[0]);
+  double sign = stop[0] > start[0] ? 1.0 : -1.0;
+  size_t r_end = (size_t)sat_usize(ceil(hi));
+  for (size_t r = (size_t)sat_usize(floor(lo)); r < r_end; r++) {
+    double left = fmax(lo, (double)r), right = fmin(hi, (double)(r + 1));
+    float area = (float)(sign * (right - left) *
+                         (0.5 * (double)slope * (right + left) + (double)intercept));
+    float width = (float)(right - left);
+    float remaining = fabsf(area);
+    for (size_t a = 0; a < stride; a++) {
+      if (remaining <= 0.0f) break;
+      float inc = fminf(width, remaining);
+      buf[r * stride + a] = buf[r * stride + a] + copysignf(inc, area);
+      remaining = remaining - inc;
+    }
+  }
+}
+
+static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+/* pyfai_split.rs:35-137 (split) with sign = +1. Returns 0 when the caller
+ * must use the fractional fallback (None), else 1 after appending entries. */
+static int pyfai_split2d(double poly[4][2], const grid2d *g, size_t pixel, bucket *bk) {
+  double a0 = g->a0, a1 = g->a1;
+  double cstart = (a0 >= -OR_PI && a1 <= OR_PI) ? -OR_PI : (a0 >= 0.0 && a1 <= OR_TAU) ? 0.0 : a0;
+  double offset = cstart + OR_PI;
+  for (int i = 0; i < 4; i++) poly[i][1] = wrap_tau(poly[i][1] - offset + OR_PI) - OR_PI;
+  double *a = poly[0], *b = poly[1], *c = poly[2], *d = poly[3];
+  double area = 1.0 * (0.5 * (c[0] - a[0]) * (d[1] - b[1]) - (c[1] - a[1]) * (d[0] - b[0]));
+  if (area > 0.0) {
+    for (int i = 0; i < 4; i++)
+      if (poly[i][1] < 0.0) poly[i][1] += OR_TAU;
+    double s = 0.0;
+    for (int i = 0; i < 4; i++) s = s + poly[i][1];
+    if (s * 0.25 > OR_PI)
+      for (int i = 0; i < 4; i++) poly[i][1] -= OR_TAU;
+  }
+  for (int i = 0; i < 4; i++) poly[i][1] += offset;
+  double r0 = g->r0, r1 = g->r1;
+  int all_lo = 1, all_hi = 1, all_alo = 1, all_ahi = 1;
+  for (int i = 0; i < 4; i++) {
+    all_lo &= poly[i][0] < r0;
+    all_hi &= poly[i][0] > r1;
+    all_alo &= poly[i][1] < a0;
+    all_ahi &= poly[i][1] > a1;
+  }
+  if (all_lo || all_hi || all_alo || all_ahi) return 1;
+  double dr = grid_rstep(g), da = grid_astep(g);
+  for (int i = 0; i < 4; i++) {
+    poly[i][0] = (clampd(poly[i][0], r0, r1) - r0) / dr;
+    poly[i][1] = (clampd(poly[i][1], a0, a1) - a0) / da;
+  }
+  double mr = INFINITY, ma = INFINITY;
+  for (int i = 0; i < 4; i++) {
+    mr = fmin(mr, poly[i][0]);
+    ma = fmin(ma, poly[i][1]);
+  }
+  size_t ir = (size_t)sat_usize(floor(mr)), ia = (size_t)sat_usize(floor(ma));
+  for (int i = 0; i < 4; i++) {
+    poly[i][0] -= (double)ir;
+    poly[i][1] -= (double)ia;
+  }
+  double xr = 0.0, xa = 0.0;
+  for (int i = 0; i < 4; i++) {
+    xr = fmax(xr, poly[i][0]);
+    xa = fmax(xa, poly[i][1]);
+  }
+  size_t nr = (size_t)sat_usize(ceil(xr)), na = (size_t)sat_usize(ceil(xa));
+  if (nr == 0 || na == 0) return 0;
+  float *buf = (float *)calloc((nr + 1) * (na + 1), sizeof(float));
+  for (int i = 0; i < 4; i++) pyfai_line(buf, na + 1, poly[i], poly[(i + 1) % 4]);
+  double total = 0.0;
+  size_t cnt = 0;
+  for (size_t r = 0; r < nr; r++)
+    for (size_t q = 0; q < na; q++) {
+      double v = 1.0 * (double)buf[r * (na + 1) + q];
+      if (!isfinite(v)) {
+        free(buf);
+        return 0;
+      }
+      if (v > 0.0 && ir + r < g->nr && ia + q < g->na) {
+        total += v;
+        cnt++;
+      }
+    }
+  if (total <= 0.0) {
+    free(buf);
+    return 0;
+  }
+  for (size_t r = 0; r < nr; r++)
+    for (size_t q = 0; q < na; q++) {
+      double v = 1.0 * (double)buf[r * (na + 1) + q];
+      if (v > 0.0 && ir + r < g->nr && ia + q < g->na)
+        bucket_push(&bk[(ia + q) * g->nr + ir + r], pixel, v / total);
+    }
+  free(buf);
+  return 1;
+}
+
+/* pyFAI-compatible 2D table (shared_grid.rs:13-150 with
+ * PixelSplitMode::PyfaiHistogram): grid chi (-pi, pi) by default
+ * (methods.rs:371-373), radial edges from pyfai_radial_range. */
+int or_build_lut2d_pyfai(const or_poni *p, size_t dim1, size_t dim2, int unit, size_t n_radial,
+                         size_t n_azimuthal, int has_rr, double rlo, double rhi, int has_ar,
+                         double alo, double ahi, or_lut *out, double *edges) {
+  if (dim1 == 0 || dim2 == 0) return 1;
+  size_t nc = (dim1 + 1) * (dim2 + 1), npix = dim1 * dim2;
+  double *tth = (double *)malloc(npix * sizeof(double));
+  double *chi = (double *)malloc(npix * sizeof(double));
+  double *rad = (double *)malloc(npix * sizeof(double));
+  double *tc = (double *)malloc(nc * sizeof(double));
+  double *ac = (double *)malloc(nc * sizeof(double));
+  double *rc = (double *)malloc(nc * sizeof(double));
+  or_centres(p, dim1, dim2, tth, chi);
+  or_convert_unit(unit, p->wavelength, tth, npix, rad);
+  or_corners(p, dim1, dim2, tc, ac);
+  or_convert_unit(unit, p->wavelength, tc, nc, rc);
+  double r_lo = rlo, r_hi = rhi;
+  if (!has_rr) pyfai_range(rc, nc, &r_lo, &r_hi);
+  double a_lo = has_ar ? alo : -OR_PI, a_hi = has_ar ? ahi : OR_PI;
+  grid2d g;
+  if (grid_new(&g, n_radial, n_azimuthal, r_lo, r_hi, a_lo, a_hi)) {
+    free(tth); free(chi); free(rad); free(tc); free(ac); free(rc);
+    return 2;
+  }
+  g.clamp_index_rounding = 1;
+  if (edges) { edges[0] = g.r0; edges[1] = g.r1; edges[2] = g.a0; edges[3] = g.a1; }
+  trig t = make_trig(p);
+  size_t nb = n_radial * n_azimuthal;
+  bucket *bk = (bucket *)calloc(nb, sizeof(bucket));
+  size_t w = dim2 + 1;
+  for (size_t pix = 0; pix < npix; pix++) {
+    size_t b0 = pix / dim2 * w + pix % dim2;
+    size_t idx[4] = {b0, b0 + 1, b0 + w + 1, b0 + w};
+    int axis_on_edge = 0, zero_r = 0;
+    double winding = 0.0;
+    for (int i = 0; i < 4; i++) {
+      double delta = ac[idx[(i + 1) % 4]] - ac[idx[i]];
+      delta = rem_euclid(delta + OR_PI, OR_TAU) - OR_PI;
+      if (fabs(delta) >= OR_PI - 64.0 * OR_EPS) axis_on_edge = 1;
+      winding += delta;
+      if (rc[idx[i]] == 0.0) zero_r = 1;
+    }
+    if (fabs(winding) > OR_PI || axis_on_edge || zero_r) {
+      sample_pixel_2d(p, &t, dim2, pix, unit, &g, bk);
+      continue;
+    }
+    size_t pidx[4] = {b0, b0 + w, b0 + w + 1, b0 + 1}; /* pyFAI order */
+    double pp[4][2];
+    for (int i = 0; i < 4; i++) {
+      pp[i][0] = rc[pidx[i]];
+      pp[i][1] = ac[pidx[i]];
+    }
+    if (pyfai_split2d(pp, &g, pix, bk)) continue;
+    double center = g.a0 + rem_euclid(chi[pix] - g.a0, OR_TAU);
+    double poly[4][2];
+    for (int i = 0; i < 4; i++) {
+      poly[i][0] = rc[idx[i]];
+      poly[i][1] = center + rem_euclid(ac[idx[i]] - center + OR_PI, OR_TAU) - OR_PI;
+    }
+    double area = fabs(polygon_area((const double(*)[2])poly, 4));
+    if (area > 0.0) {
+      split_polygon_2d((const double(*)[2])poly, area, &g, pix, bk);
+    } else {
+      long long bb = grid_bin(&g, rad[pix], chi[pix]);
+      if (bb >= 0) bucket_push(&bk[bb], pix, 1.0);
+    }
+  }
+  buckets_to_lut(bk, nb, 1, g.r0, grid_rstep(&g), out);
+  free(bk);
+  free(tth); free(chi); free(rad); free(tc); free(ac); free(rc);
+  return 0;
+}
+
 /* --------------------------------------------------- execution and gather */
 
 /* prepared.rs:163-193: entries of masked pixels dropped, order kept. An
@@ -1145,4 +1448,30 @@ void or_azimuthal_std(const or_lut *lut, const uint8_t *mask, const double *norm
     if (sigma) sigma[r] = sqrt(fmax(sum_var, 0.0)) / sum_weight;
     std_out[r] = sqrt(fmax(m2 / sum_weight, 0.0));
   }
+}
+
+/* Exposes split1d (pyfai_1d.rs:259-354) on one polygon for known-answer
+ * checks; returns the entry count written to bins/weights (capacity cap). */
+int or_pyfai_split1d(const double *poly8, double r0, double r1, size_t nbins, uint64_t *bins,
+                     double *weights, size_t cap) {
+  double poly[4][2];
+  for (int i = 0; i < 4; i++) {
+    poly[i][0] = poly8[2 * i];
+    poly[i][1] = poly8[2 * i + 1];
+  }
+  bucket *bk = (bucket *)calloc(nbins, sizeof(bucket));
+  split1d(poly, r0, r1, nbins, 0, bk);
+  size_t n = 0;
+  for (size_t b = 0; b < nbins; b++) {
+    for (size_t e = 0; e < bk[b].len; e++)
+      if (n < cap) {
+        bins[n] = b;
+        weights[n] = bk[b].w[e];
+        n++;
+      }
+    free(bk[b].col);
+    free(bk[b].w);
+  }
+  free(bk);
+  return (int)n;
 }

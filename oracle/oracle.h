@@ -18,6 +18,7 @@
  *   2D tables       src/lut/shared_grid.rs:13-232, integration/grid.rs:26-222,
  *                   579-675, 978-1028, integration/methods.rs:365-388
  *   supersampling   src/lut/supersampled.rs:21-595, 1381-1477, lut/common.rs:33-65
+ *   pyFAI split     src/lut/pyfai_1d.rs:36-354, src/lut/pyfai_split.rs:35-167
  *   masked table    src/lut/prepared.rs:163-262
  *   gather          src/lut/prepared.rs:985-1121
  *   azimuthal std   src/lut/azimuthal.rs:48-118
@@ -87,6 +88,15 @@ int or_build_lut2d_ss(const or_poni *p, size_t dim1, size_t dim2, size_t n_radia
                       size_t n_azimuthal, size_t n_ss, int unit, int has_rr, double rlo,
                       double rhi, int has_ar, double alo, double ahi, or_lut *out,
                       double *edges);
+/* pyFAI-compatible full splitting, 1D (pyfai_1d.rs:55-354) and 2D
+ * (pyfai_split.rs:35-167 in shared_grid.rs:13-150). */
+int or_build_lut1d_pyfai(const or_poni *p, size_t dim1, size_t dim2, size_t n_points, int unit,
+                         int has_range, double rmin, double rmax, or_lut *out);
+int or_build_lut2d_pyfai(const or_poni *p, size_t dim1, size_t dim2, int unit, size_t n_radial,
+                         size_t n_azimuthal, int has_rr, double rlo, double rhi, int has_ar,
+                         double alo, double ahi, or_lut *out, double *edges);
+int or_pyfai_split1d(const double *poly8, double r0, double r1, size_t nbins, uint64_t *bins,
+                     double *weights, size_t cap);
 void or_lut_free(or_lut *lut);
 
 /* Masked execution table: entries of masked pixels removed, order kept. */

@@ -386,3 +386,31 @@ def test_ss_2d_totals_and_1d_consistency():
     assert np.abs(tot - 1.0).max() < 1e-12   # clamped default grid keeps every subpixel
     l1 = O.build_lut1d_ss(p, 11, 13, 12, 3)
     assert l1.hist_min == r0 and abs(l1.hist_step - (r1 - r0) / 12) <= 1e-18
+
+
+# ------------------------------------------------------------ pyFAI split
+
+def test_pyfai_split1d_matches_pyfai_golden_weights():
+    # pyfai_1d.rs:391-413: pyFAI 2026.9.0 fullSplit1D_engine on one pixel.
+    got = O.pyfai_split1d([(0.1, 0.2), (1.2, 0.3), (1.8, 1.5), (0.4, 1.7)], 0.0, 2.0, 4)
+    expected = [0.21462747684427036, 0.4066985688044358, 0.32228298342690204,
+                0.05639097092439181]
+    assert [b for b, _ in got] == [0, 1, 2, 3]
+    for (_, w), e in zip(got, expected):
+        assert abs(w - e) < 1e-7
+
+
+def test_pyfai_1d_conserves_pixels():
+    p = O.Poni(172e-6, 172e-6, 0.18, 1043 * 172e-6 / 2 * 0.05, 981 * 172e-6 / 2 * 0.05,
+               0.05, -0.03, 0.10, 0.3344e-10)
+    lut = O.build_lut1d_pyfai(p, 40, 50, 300)
+    tot = np.bincount(lut.col, weights=lut.w, minlength=2000)
+    assert np.abs(tot - 1.0).max() < 1e-9
+
+
+def test_pyfai_2d_totals():
+    p = O.Poni(172e-6, 172e-6, 0.05, 24 * 172e-6, 36 * 172e-6, 0.02, -0.01, 0.05, 1e-10)
+    lut, (r0, r1, a0, a1) = O.build_lut2d_pyfai(p, 64, 80, 32, 36)
+    assert (a0, a1) == (-math.pi, math.pi) and r0 >= 0.0
+    tot = np.bincount(lut.col, weights=lut.w, minlength=64 * 80)
+    assert np.abs(tot - 1.0).max() < 1e-9
