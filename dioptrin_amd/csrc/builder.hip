@@ -728,6 +728,268 @@ __global__ void k_list_angles(Geo g, const double* d1, const double* d2, uint32_
   if (a.uncertain) flist[atomicAdd(fcount, 1u)] = i;
 }
 
+// ------------------------------------------------------ pyFAI-compatible
+
+struct PfParams {
+  Geo g;
+  const double* rc;
+  const double* ac;
+  const double* rad;
+  const double* chi;
+  uint32_t n;           // 1D bins
+  double r0, r1;        // 1D radial range
+  Grid2 G;              // 2D grid
+};
+
+// pyfai_1d.rs:259-354 (split1d, sector = false, sign = +1): the float32
+// line-area value of relative bin j, accumulated edge by edge in order.
+__device__ inline float pf1d_value(const double (*poly)[2], size_t first, size_t j) {
+  float v = 0.0f;
+  for (int i = 0; i < 4; i++) {
+    double x0 = poly[i][0], y0 = poly[i][1];
+    double x1 = poly[(i + 1) & 3][0], y1 = poly[(i + 1) & 3][1];
+    if (x0 == x1) continue;
+    double slope = (y1 - y0) / (x1 - x0);
+    double intercept = y0 - slope * x0;
+    double sign = x1 > x0 ? 1.0 : -1.0;
+    double left = fmax(fmin(x0, x1), (double)(first + j));
+    double right = fmin(fmax(x0, x1), (double)(first + j + 1));
+    if (right > left)
+      v = v + (float)(sign * (right - left) * (0.5 * slope * (right + left) + intercept));
+  }
+  return v;
+}
+
+template <class Emit>
+__device__ bool pf1d_pixel(const PfParams& P, uint32_t idx, Emit& emit) {
+  uint32_t dim2 = P.g.dim2;
+  uint64_t w = (uint64_t)dim2 + 1;
+  uint64_t b = (uint64_t)(idx / dim2) * w + idx % dim2;
+  uint64_t ci[4] = {b, b + w, b + w + 1, b + 1};  // pyFAI order a, d, c, b
+  double poly[4][2];
+  for (int i = 0; i < 4; i++) {
+    poly[i][0] = P.rc[ci[i]];
+    poly[i][1] = P.ac[ci[i]];
+  }
+  if (singular4(poly)) return false;
+  const double offset = 0.0;  // full azimuth (-pi, pi)
+  for (int i = 0; i < 4; i++) poly[i][1] = wrap_tau(poly[i][1] - offset + kPiD) - kPiD;
+  double* a = poly[0];
+  double* bb = poly[1];
+  double* c = poly[2];
+  double* d = poly[3];
+  double area = 1.0 * (0.5 * (c[0] - a[0]) * (d[1] - bb[1]) - (c[1] - a[1]) * (d[0] - bb[0]));
+  if (area > 0.0) {
+    for (int i = 0; i < 4; i++)
+      if (poly[i][1] < 0.0) poly[i][1] += kTauD;
+    double sum = 0.0;
+    for (int i = 0; i < 4; i++) sum = sum + poly[i][1];
+    if (sum * 0.25 > kPiD)
+      for (int i = 0; i < 4; i++) poly[i][1] -= kTauD;
+  }
+  for (int i = 0; i < 4; i++) poly[i][1] += offset;
+  double dr = (P.r1 - P.r0) / (double)P.n;
+  for (int i = 0; i < 4; i++) poly[i][0] = (poly[i][0] - P.r0) / dr;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = 0; i < 4; i++) {
+    lo = fmin(lo, poly[i][0]);
+    hi = fmax(hi, poly[i][0]);
+  }
+  if (hi < 0.0 || lo >= (double)P.n) return true;
+  size_t first = (size_t)sat_usize(fmax(floor(lo), 0.0));
+  long long e = sat_isize(floor(hi)) + 1;
+  if (e < 0) e = 0;
+  if (e > (long long)P.n) e = (long long)P.n;
+  if (floor(lo) == floor(hi)) {
+    emit((uint32_t)first, 1.0);
+    return true;
+  }
+  size_t n = (size_t)e > first ? (size_t)e - first : 0;
+  double sum = 0.0;
+  for (size_t j = 0; j < n; j++) sum += (double)pf1d_value(poly, first, j);
+  if (!isfinite(sum) || sum == 0.0) return true;
+  double total = 0.0;
+  for (size_t j = 0; j < n; j++) {
+    double wv = (double)pf1d_value(poly, first, j) / sum;
+    if (isfinite(wv) && wv > 0.0) total += wv;
+  }
+  for (size_t j = 0; j < n; j++) {
+    double wv = (double)pf1d_value(poly, first, j) / sum;
+    if (isfinite(wv) && wv > 0.0) emit((uint32_t)(first + j), wv / total);
+  }
+  return true;
+}
+
+// pyfai_split.rs:139-167 (line) for one cell: the float32 buffer value of
+// (column r, row a), lines applied in order as the reference fills it.
+__device__ inline float pf2d_cell(const double (*poly)[2], size_t r, size_t a) {
+  float v = 0.0f;
+  for (int i = 0; i < 4; i++) {
+    const double* st = poly[i];
+    const double* sp = poly[(i + 1) & 3];
+    if (st[0] == sp[0]) continue;
+    float slope = (float)((sp[1] - st[1]) / (sp[0] - st[0]));
+    float intercept = (float)(sp[1] - (double)slope * sp[0]);
+    double lo = fmin(st[0], sp[0]), hi = fmax(st[0], sp[0]);
+    double sign = sp[0] > st[0] ? 1.0 : -1.0;
+    size_t r_begin = (size_t)sat_usize(floor(lo)), r_end = (size_t)sat_usize(ceil(hi));
+    if (r < r_begin || r >= r_end) continue;
+    double left = fmax(lo, (double)r), right = fmin(hi, (double)(r + 1));
+    float area = (float)(sign * (right - left) *
+                         (0.5 * (double)slope * (right + left) + (double)intercept));
+    float width = (float)(right - left);
+    float remaining = fabsf(area);
+    for (size_t q = 0; q <= a; q++) {
+      if (remaining <= 0.0f) break;
+      float inc = fminf(width, remaining);
+      if (q == a) v = v + copysignf(inc, area);
+      remaining = remaining - inc;
+    }
+  }
+  return v;
+}
+
+__device__ inline double clampd(double x, double lo, double hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// pyfai_split.rs:35-137 (split, sign = +1). Returns 0 for "None" (the caller
+// falls back to fractional clipping), 1 otherwise.
+template <class Emit>
+__device__ int pf2d_split(const Grid2& G, double (*poly)[2], Emit& emit) {
+  double a0 = G.a0, a1 = G.a1;
+  double cstart = (a0 >= -kPiD && a1 <= kPiD) ? -kPiD : (a0 >= 0.0 && a1 <= kTauD) ? 0.0 : a0;
+  double offset = cstart + kPiD;
+  for (int i = 0; i < 4; i++) poly[i][1] = wrap_tau(poly[i][1] - offset + kPiD) - kPiD;
+  double* a = poly[0];
+  double* b = poly[1];
+  double* c = poly[2];
+  double* d = poly[3];
+  double area = 1.0 * (0.5 * (c[0] - a[0]) * (d[1] - b[1]) - (c[1] - a[1]) * (d[0] - b[0]));
+  if (area > 0.0) {
+    for (int i = 0; i < 4; i++)
+      if (poly[i][1] < 0.0) poly[i][1] += kTauD;
+    double sum = 0.0;
+    for (int i = 0; i < 4; i++) sum = sum + poly[i][1];
+    if (sum * 0.25 > kPiD)
+      for (int i = 0; i < 4; i++) poly[i][1] -= kTauD;
+  }
+  for (int i = 0; i < 4; i++) poly[i][1] += offset;
+  double r0 = G.r0, r1 = G.r1;
+  bool all_lo = true, all_hi = true, all_alo = true, all_ahi = true;
+  for (int i = 0; i < 4; i++) {
+    all_lo &= poly[i][0] < r0;
+    all_hi &= poly[i][0] > r1;
+    all_alo &= poly[i][1] < a0;
+    all_ahi &= poly[i][1] > a1;
+  }
+  if (all_lo || all_hi || all_alo || all_ahi) return 1;
+  double dr = G.rstep(), da = G.astep();
+  for (int i = 0; i < 4; i++) {
+    poly[i][0] = (clampd(poly[i][0], r0, r1) - r0) / dr;
+    poly[i][1] = (clampd(poly[i][1], a0, a1) - a0) / da;
+  }
+  double mr = INFINITY, ma = INFINITY;
+  for (int i = 0; i < 4; i++) {
+    mr = fmin(mr, poly[i][0]);
+    ma = fmin(ma, poly[i][1]);
+  }
+  size_t ir = (size_t)sat_usize(floor(mr)), ia = (size_t)sat_usize(floor(ma));
+  for (int i = 0; i < 4; i++) {
+    poly[i][0] -= (double)ir;
+    poly[i][1] -= (double)ia;
+  }
+  double xr = 0.0, xa = 0.0;
+  for (int i = 0; i < 4; i++) {
+    xr = fmax(xr, poly[i][0]);
+    xa = fmax(xa, poly[i][1]);
+  }
+  size_t nr = (size_t)sat_usize(ceil(xr)), na = (size_t)sat_usize(ceil(xa));
+  if (nr == 0 || na == 0) return 0;
+  double total = 0.0;
+  for (size_t r = 0; r < nr; r++)
+    for (size_t q = 0; q < na; q++) {
+      double v = 1.0 * (double)pf2d_cell(poly, r, q);
+      if (!isfinite(v)) return 0;
+      if (v > 0.0 && ir + r < G.nr && ia + q < G.na) total += v;
+    }
+  if (total <= 0.0) return 0;
+  for (size_t r = 0; r < nr; r++)
+    for (size_t q = 0; q < na; q++) {
+      double v = 1.0 * (double)pf2d_cell(poly, r, q);
+      if (v > 0.0 && ir + r < G.nr && ia + q < G.na)
+        emit((uint32_t)((ia + q) * G.nr + ir + r), v / total);
+    }
+  return 1;
+}
+
+template <class Emit>
+__device__ bool pf2d_pixel(const PfParams& P, uint32_t idx, Emit& emit) {
+  uint32_t dim2 = P.g.dim2;
+  uint64_t w = (uint64_t)dim2 + 1;
+  uint64_t b = (uint64_t)(idx / dim2) * w + idx % dim2;
+  uint64_t ci[4] = {b, b + 1, b + w + 1, b + w};
+  bool axis_on_edge = false, zero_r = false;
+  double winding = 0.0;
+  for (int i = 0; i < 4; i++) {
+    double delta = P.ac[ci[(i + 1) & 3]] - P.ac[ci[i]];
+    delta = rem_euclid(delta + kPiD, kTauD) - kPiD;
+    axis_on_edge |= fabs(delta) >= kPiD - 64.0 * kEps;
+    winding += delta;
+    zero_r |= P.rc[ci[i]] == 0.0;
+  }
+  if (fabs(winding) > kPiD || axis_on_edge || zero_r) return false;
+  uint64_t pi[4] = {b, b + w, b + w + 1, b + 1};
+  double pp[4][2];
+  for (int i = 0; i < 4; i++) {
+    pp[i][0] = P.rc[pi[i]];
+    pp[i][1] = P.ac[pi[i]];
+  }
+  if (pf2d_split(P.G, pp, emit)) return true;
+  // Fractional fallback (shared_grid.rs:100-115).
+  Ps2Params Q{P.g, P.G, P.rc, P.ac, P.rad, P.chi};
+  double center = P.G.a0 + rem_euclid(P.chi[idx] - P.G.a0, kTauD);
+  double poly[4][2];
+  for (int i = 0; i < 4; i++) {
+    poly[i][0] = P.rc[ci[i]];
+    poly[i][1] = center + rem_euclid(P.ac[ci[i]] - center + kPiD, kTauD) - kPiD;
+  }
+  double area = fabs(polygon_area(poly, 4));
+  if (area > 0.0) {
+    split2d(Q, poly, area, emit);
+  } else {
+    long long bin = grid_bin(P.G, P.rad[idx], P.chi[idx]);
+    if (bin >= 0) emit((uint32_t)bin, 1.0);
+  }
+  return true;
+}
+
+template <bool TWOD>
+__global__ void k_pf_count(PfParams P, uint32_t npix, uint32_t* cnt, uint32_t* sing,
+                           uint32_t* nsing) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npix) return;
+  CountEmit e;
+  bool ok = TWOD ? pf2d_pixel(P, idx, e) : pf1d_pixel(P, idx, e);
+  if (!ok) {
+    uint32_t s = atomicAdd(nsing, 1u);
+    if (s < kMaxSingular) sing[s] = idx;
+    cnt[idx] = 0;
+    return;
+  }
+  cnt[idx] = e.n;
+}
+
+template <bool TWOD>
+__global__ void k_pf_fill(PfParams P, uint32_t npix, const uint32_t* off, uint32_t* bin,
+                          uint32_t* pix, double* w) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npix) return;
+  FillEmit e{bin, pix, w, off[idx], idx};
+  if (TWOD) pf2d_pixel(P, idx, e);
+  else pf1d_pixel(P, idx, e);
+}
+
 // --------------------------------------------------------- CSR assembly
 
 __global__ void k_iota(uint32_t* v, uint64_t n) {
@@ -1524,6 +1786,123 @@ void build_ss(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, 
   DIO_HIP(hipStreamSynchronize(s));
 }
 
+// pyFAI-compatible tables: 1D (pyfai_1d.rs:55-109) and the 2D shared grid
+// with PixelSplitMode::PyfaiHistogram (methods.rs:371-373, grid.rs:579-622).
+void build_pyfai(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, Built& out) {
+  uint32_t dim1 = d.dim1, dim2 = d.dim2;
+  uint64_t npix = (uint64_t)dim1 * dim2;
+  uint64_t nc = (uint64_t)(dim1 + 1) * (dim2 + 1);
+  DevBuf<double> ttp(npix), chp(npix), radp, tc(nc), ac(nc), rcb;
+  out.uncertain += grid_angles(g, dim1, dim2, 0.5, ttp.p, chp.p, s);
+  out.uncertain += grid_angles(g, dim1 + 1, dim2 + 1, 0.0, tc.p, ac.p, s);
+  const double* rad = ttp.p;
+  const double* rc = tc.p;
+  if (g.unit != 0) {
+    radp.alloc(npix);
+    rcb.alloc(nc);
+    k_to_unit<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(g, ttp.p, radp.p, npix);
+    k_to_unit<<<blocks_for(nc, kThreads), kThreads, 0, s>>>(g, tc.p, rcb.p, nc);
+    rad = radp.p;
+    rc = rcb.p;
+  }
+  double r0, r1;
+  if (d.has_radial_range) {
+    r0 = d.radial_min;
+    r1 = d.radial_max;
+  } else {
+    // pyfai_1d.rs:36-46: [max(min corner, 0), max corner * (1 + f32::EPSILON)].
+    double lo, hi;
+    device_minmax(rc, nc, s, lo, hi);
+    r0 = std::fmax(lo, 0.0);
+    r1 = hi * (1.0 + 1.1920928955078125e-07);
+  }
+  PfParams P{};
+  P.g = g;
+  P.rc = rc;
+  P.ac = ac.p;
+  P.rad = rad;
+  P.chi = chp.p;
+  bool twod = d.ndim == 2;
+  uint32_t nbins;
+  SampleGrid SG{0, 0.0, 0.0, 0};
+  Grid2 G{};
+  if (twod) {
+    G = grid_new(d.n_radial, d.n_azimuthal, r0, r1,
+                 d.has_azimuthal_range ? d.azimuthal_min : -kPiD,
+                 d.has_azimuthal_range ? d.azimuthal_max : kPiD);
+    G.clamp_index_rounding = 1;
+    P.G = G;
+    nbins = d.n_radial * d.n_azimuthal;
+    out.n_radial = d.n_radial;
+    out.n_azimuthal = d.n_azimuthal;
+    out.hist_min = G.r0;
+    out.hist_step = G.rstep();
+    out.az_min = G.a0;
+    out.az_step = G.astep();
+  } else {
+    if (!sample_grid_ok(d.n_radial, r0, r1))
+      fail(DIO_ERR_INVALID, "radial range must have finite, increasing, resolvable bin edges");
+    P.n = d.n_radial;
+    P.r0 = r0;
+    P.r1 = r1;
+    SG = SampleGrid{d.n_radial, r0, r1, d.has_radial_range ? 0 : 1};
+    nbins = d.n_radial;
+    out.n_radial = d.n_radial;
+    out.n_azimuthal = 1;
+    out.hist_min = r0;
+    out.hist_step = (r1 - r0) / (double)d.n_radial;
+  }
+  DevBuf<uint32_t> cnt(npix + 1), off(npix + 1), sing(kMaxSingular), nsing(1), ebin, epix;
+  DevBuf<double> ew;
+  DIO_HIP(hipMemsetAsync(cnt.p, 0, (npix + 1) * sizeof(uint32_t), s));
+  DIO_HIP(hipMemsetAsync(nsing.p, 0, sizeof(uint32_t), s));
+  if (twod)
+    k_pf_count<true><<<blocks_for(npix, kThreads), kThreads, 0, s>>>(P, (uint32_t)npix, cnt.p,
+                                                                    sing.p, nsing.p);
+  else
+    k_pf_count<false><<<blocks_for(npix, kThreads), kThreads, 0, s>>>(P, (uint32_t)npix, cnt.p,
+                                                                     sing.p, nsing.p);
+  DIO_HIP(hipGetLastError());
+  uint32_t ns = read1(nsing.p, s);
+  if (ns > kMaxSingular) fail(DIO_ERR_UNSUPPORTED, "too many beam pixels for sampling");
+  Singular S;
+  S.pixels.resize(ns);
+  if (ns) {
+    DIO_HIP(hipMemcpyAsync(S.pixels.data(), sing.p, ns * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           s));
+    DIO_HIP(hipStreamSynchronize(s));
+  }
+  singular_prepare(g, SG, twod ? &G : nullptr, S, cnt.p, s);
+  out.uncertain += S.patched;
+  out.singular = ns;
+  exclusive_scan(cnt.p, off.p, npix + 1, s, sc);
+  uint64_t nnz = read1(off.p + npix, s);
+  if (!twod && nnz > 64ull * 1024 * 1024)
+    fail(DIO_ERR_UNSUPPORTED,
+         "pyFAI-compatible splitting exceeds the 64-million-entry memory bound; reduce binning.");
+  if (nnz > 0x7fffffffull) fail(DIO_ERR_UNSUPPORTED, "table exceeds 2^31 entries");
+  ebin.alloc(nnz);
+  epix.alloc(nnz);
+  ew.alloc(nnz);
+  if (twod)
+    k_pf_fill<true><<<blocks_for(npix, kThreads), kThreads, 0, s>>>(P, (uint32_t)npix, off.p,
+                                                                   ebin.p, epix.p, ew.p);
+  else
+    k_pf_fill<false><<<blocks_for(npix, kThreads), kThreads, 0, s>>>(P, (uint32_t)npix, off.p,
+                                                                    ebin.p, epix.p, ew.p);
+  if (ns)
+    k_sing_fill<<<blocks_for(ns, 64), 64, 0, s>>>(S.hits.p, nbins, ns, S.dpix.p, off.p, ebin.p,
+                                                  epix.p, ew.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+  assemble(nbins, nnz, ebin, epix, ew, s, sc, out.full);
+  out.norm.alloc(npix);
+  k_normalization<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(
+      g, ttp.p, chp.p, d.has_polarization, d.polarization_factor, out.norm.p);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace
 
 Geo make_geo(const dio_poni& p, const dio_plan_desc& d) {
@@ -1551,6 +1930,7 @@ Geo make_geo(const dio_poni& p, const dio_plan_desc& d) {
 void build_tables(const Geo& g, const dio_plan_desc& d, hipStream_t s, Built& out) {
   Scratch sc;
   if (d.method == DIO_SUPERSAMPLED) build_ss(g, d, s, sc, out);
+  else if (d.method == DIO_PYFAI_FULL) build_pyfai(g, d, s, sc, out);
   else if (d.ndim == 2) build_2d(g, d, s, sc, out);
   else build_1d(g, d, s, sc, out);
   uint64_t npix = (uint64_t)d.dim1 * d.dim2;

@@ -349,3 +349,41 @@ def test_supersampled_full_size_bit_exact(ndim):
         lut, _ = O.build_lut2d_ss(P, *w["shape"], w["n_radial"], na, w["n_ss"])
     assert_same_table(plan, lut)
     print(f"  nnz {lut.nnz} build {plan.info['build_ms']:.1f} ms", flush=True)
+
+
+# ------------------------------------------------------- pyFAI-compatible
+
+PF_CASES = [
+    # name, poni, shape, nr, na (None = 1D), unit, radial_range
+    ("pf1d_beam", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                       poni2=36 * 172e-6, rot1=0.02, rot2=-0.01, rot3=0.05, wavelength=1e-10),
+     (64, 80), 40, None, "2th_rad", None),
+    ("pf1d_range_deg", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                            poni2=36 * 172e-6, wavelength=1e-10), (64, 80), 40, None, "2th_deg",
+     (0.5, 10.0)),
+    ("pf2d_beam", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                       poni2=36 * 172e-6, rot1=0.02, rot2=-0.01, rot3=0.05, wavelength=1e-10),
+     (64, 80), 32, 36, "2th_rad", None),
+    ("pf2d_tilted_full", dict(pixel1=172e-6, pixel2=172e-6, distance=0.18,
+                              poni1=1043 * 172e-6 / 2, poni2=981 * 172e-6 / 2, rot1=0.05,
+                              rot2=-0.03, rot3=0.10, wavelength=0.3344e-10), (1043, 981), 500,
+     360, "2th_rad", None),
+    ("pf1d_tilted_full", dict(pixel1=172e-6, pixel2=172e-6, distance=0.18,
+                              poni1=1043 * 172e-6 / 2, poni2=981 * 172e-6 / 2, rot1=0.05,
+                              rot2=-0.03, rot3=0.10, wavelength=0.3344e-10), (1043, 981), 3000,
+     None, "2th_rad", None),
+]
+
+
+@pytest.mark.parametrize("g", PF_CASES, ids=[g[0] for g in PF_CASES])
+def test_pyfai_table_bit_exact(g):
+    name, poni, shape, nr, na, unit, rr = g
+    print(f"case {name}", flush=True)
+    plan = GpuPlan(poni, shape, method="pyfai_full", unit=unit, n_radial=nr, n_azimuthal=na,
+                   radial_range=rr)
+    P = O.Poni(**poni)
+    if na is None:
+        lut = O.build_lut1d_pyfai(P, *shape, nr, unit, rr)
+    else:
+        lut, _ = O.build_lut2d_pyfai(P, *shape, nr, na, unit, rr)
+    assert_same_table(plan, lut)

@@ -413,4 +413,7 @@ def test_pyfai_2d_totals():
     lut, (r0, r1, a0, a1) = O.build_lut2d_pyfai(p, 64, 80, 32, 36)
     assert (a0, a1) == (-math.pi, math.pi) and r0 >= 0.0
     tot = np.bincount(lut.col, weights=lut.w, minlength=64 * 80)
-    assert np.abs(tot - 1.0).max() < 1e-9
+    # Split pixels are renormalized to one; the sampled beam pixel keeps only
+    # its samples above the first radial edge (2D grids have no low clamp).
+    off = np.abs(tot - 1.0) > 1e-9
+    assert off.sum() <= 1 and np.all(tot[off] < 1.0)
