@@ -72,7 +72,20 @@ def main():
     ap.add_argument("--errors", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy kernel of MIB MiB 5 times "
+                         "(reference for the memory-traffic counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        from dioptrin_amd.device import DeviceArray, synchronize
+        n = args.calibrate_copy << 20
+        a, b = DeviceArray((n,), np.uint8), DeviceArray((n,), np.uint8)
+        from dioptrin_amd import _native as N
+        for _ in range(5):
+            N.check(N.lib().dio_memcpy(a._ctx, b.ptr, a.ptr, n, 2))
+        synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "launches": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
