@@ -11,6 +11,7 @@
 
 #include "builder.hpp"
 #include "gather.hpp"
+#include "tile.hpp"
 
 struct dio_ctx {
   int device = 0;
@@ -27,8 +28,9 @@ struct dio_plan {
   dio::Built b;
   dio::GatherTable gt;
   dio::StdTable st;
-  dio::GatherScratch sc;
-  uint32_t tile = 8;
+  dio::TiledTable tt;
+  dio::TileScratch sc;
+  uint32_t tile = 16;
   double build_ms = 0;
   std::mutex mu;
   dio::DevBuf<uint8_t> stage;
@@ -40,7 +42,7 @@ struct dio_plan {
 
 struct dio_graph {
   dio_plan* plan = nullptr;
-  dio::GatherScratch sc;  // owned: the plan's scratch may be reallocated later
+  dio::TileScratch sc;  // owned: the plan's scratch may be reallocated later
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~dio_graph() {
@@ -120,7 +122,7 @@ void ensure_out(dio::DevBuf<double>& b, size_t n) {
 }
 
 // Enqueue the whole integration of device frames on `s`.
-void run_device(dio_plan* p, dio::GatherScratch& sc, const void* frames, int dtype,
+void run_device(dio_plan* p, dio::TileScratch& sc, const void* frames, int dtype,
                 uint64_t n_frames, uint32_t flags, double* I, double* S, double* SD,
                 hipStream_t s) {
   bool var = flags & DIO_WITH_ERRORS;
@@ -131,8 +133,8 @@ void run_device(dio_plan* p, dio::GatherScratch& sc, const void* frames, int dty
     return;
   }
   require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
-  dio::gather_frames(p->gt, sc, frames, dtype, n_frames, var, p->desc.precision, p->tile, I, S,
-                     s);
+  uint32_t F = std::min<uint32_t>(p->tile, dio::max_frames_for(dtype));
+  dio::tiled_gather(p->tt, sc, frames, dtype, n_frames, var, p->desc.precision, F, I, S, s);
 }
 
 }  // namespace
@@ -216,7 +218,9 @@ int dio_plan_create(dio_ctx* ctx, const dio_poni* poni, const dio_plan_desc* des
     p->gt.nbins = t.nbins;
     p->gt.npix = (uint64_t)desc->dim1 * desc->dim2;
     p->gt.nnz = t.nnz;
-    dio::make_segments(p->gt, t.row_ptr.p, p->stream);
+    dio::build_tiled(p->tt, desc->dim1, desc->dim2, t.nbins, t.nnz, t.row_ptr.p, t.col.p,
+                     t.w32.p, t.w64.p, desc->precision == DIO_F64, p->stream);
+    p->tt.den = p->b.den.p;
     p->st.row_ptr = p->b.full.row_ptr.p;
     p->st.col = p->b.full.col.p;
     p->st.w32 = p->b.full.w32.p;
@@ -249,7 +253,7 @@ int dio_plan_info_get(dio_plan* p, dio_plan_info* out) {
     out->nnz = p->b.full.nnz;
     out->nnz_masked = p->b.exec_table().nnz;
     out->weighted = p->b.weighted ? 1 : 0;
-    out->n_segments = p->gt.nseg;
+    out->n_segments = p->tt.nseg;
     out->uncertain_angles = p->b.uncertain;
     out->singular_pixels = p->b.singular;
     out->build_ms = p->build_ms;
@@ -260,7 +264,7 @@ int dio_plan_info_get(dio_plan* p, dio_plan_info* out) {
     };
     out->device_bytes = tb(p->b.full) + (p->b.has_exec ? tb(p->b.exec) : 0) +
                         p->b.norm.bytes() + p->b.den.bytes() + p->b.mask.bytes() +
-                        p->gt.seg_begin.bytes() * 2 + p->gt.row_seg.bytes() + p->sc.part.bytes();
+                        p->tt.bytes() + p->sc.part.bytes();
   });
 }
 
@@ -375,7 +379,7 @@ int dio_integrate_device(dio_plan* p, const void* d_frames, int dtype, uint64_t 
 int dio_plan_set_tile(dio_plan* p, uint32_t tile) {
   return guarded([&] {
     require(p != nullptr, DIO_ERR_INVALID, "null plan");
-    require(tile >= 1 && tile <= dio::kMaxTile && (tile & (tile - 1)) == 0, DIO_ERR_INVALID,
+    require(tile >= 1 && tile <= dio::kMaxFrames && (tile & (tile - 1)) == 0, DIO_ERR_INVALID,
             "frames per tile must be a power of two in 1..32");
     std::lock_guard<std::mutex> lk(p->mu);
     p->tile = tile;
@@ -403,12 +407,13 @@ int dio_plan_autotune(dio_plan* p, int dtype, uint64_t n_frames_hint, uint32_t f
     uint32_t best_tile = p->tile;
     for (uint32_t tile : {1u, 2u, 4u, 8u, 16u, 32u}) {
       if (tile > 1 && tile > nf * 2) break;
-      dio::gather_frames(p->gt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, oi.p, os.p,
-                         p->stream);  // warm-up
+      if (tile > dio::max_frames_for(dtype)) break;
+      dio::tiled_gather(p->tt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, oi.p, os.p,
+                        p->stream);  // warm-up
       DIO_HIP(hipEventRecord(e0, p->stream));
       for (int rep = 0; rep < 3; rep++)
-        dio::gather_frames(p->gt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, oi.p,
-                           os.p, p->stream);
+        dio::tiled_gather(p->tt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, oi.p,
+                          os.p, p->stream);
       DIO_HIP(hipEventRecord(e1, p->stream));
       DIO_HIP(hipEventSynchronize(e1));
       float ms = 0;
@@ -436,7 +441,8 @@ int dio_graph_create(dio_plan* p, const void* d_frames, int dtype, uint64_t n_fr
     // Scratch must exist before capture: allocation is not capturable.
     auto g = std::make_unique<dio_graph>();
     g->plan = p;
-    g->sc.ensure(p->gt, p->tile, flags & DIO_WITH_ERRORS);
+    g->sc.ensure(p->tt, std::min<uint32_t>(p->tile, dio::max_frames_for(dtype)),
+                 flags & DIO_WITH_ERRORS);
     DIO_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     try {
       run_device(p, g->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std,
@@ -480,12 +486,10 @@ int dio_plan_time_gather(dio_plan* p, const void* d_frames, int dtype, uint64_t 
     hipEvent_t e0, e1;
     DIO_HIP(hipEventCreate(&e0));
     DIO_HIP(hipEventCreate(&e1));
-    dio::gather_segments_only(p->gt, p->sc, d_frames, dtype, nf, var, p->desc.precision,
-                              p->stream);
+    dio::tiled_pass_only(p->tt, p->sc, d_frames, dtype, nf, var, p->desc.precision, p->stream);
     DIO_HIP(hipEventRecord(e0, p->stream));
     for (int r = 0; r < reps; r++)
-      dio::gather_segments_only(p->gt, p->sc, d_frames, dtype, nf, var, p->desc.precision,
-                                p->stream);
+      dio::tiled_pass_only(p->tt, p->sc, d_frames, dtype, nf, var, p->desc.precision, p->stream);
     DIO_HIP(hipEventRecord(e1, p->stream));
     DIO_HIP(hipEventSynchronize(e1));
     float ms = 0;

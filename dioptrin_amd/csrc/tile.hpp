@@ -1,0 +1,59 @@
+// Detector-tiled execution table and its gather (gfx950).
+#pragma once
+#include "common.hpp"
+
+namespace dio {
+
+// The execution table re-grouped by detector tile: tile = kTileRows x 64
+// pixels; inside a tile, entries are grouped by bin (a segment per
+// (tile, bin)), pixels ascending as in the CSR rows. A frame tile is read from
+// HBM once, coalesced, into LDS, and every entry of the table tile is applied
+// to it; segment partials are added per bin in tile order.
+constexpr uint32_t kTileCols = 64;
+constexpr uint32_t kTileRows = 16;
+constexpr uint32_t kTilePixels = kTileCols * kTileRows;
+constexpr uint32_t kMaxFrames = 32;
+
+struct TiledTable {
+  uint32_t dim1 = 0, dim2 = 0, nbins = 0, ntx = 0, nty = 0, ntiles = 0;
+  uint64_t nnz = 0;
+  uint32_t nseg = 0;
+  DevBuf<uint16_t> lidx;      // nnz: pixel index inside its tile
+  DevBuf<float> w32;          // nnz
+  DevBuf<double> w64;         // nnz (F64 precision only)
+  DevBuf<uint32_t> seg_start; // nseg + 1: first entry of each segment
+  DevBuf<uint32_t> tile_seg;  // ntiles + 1: first segment of each tile
+  DevBuf<uint32_t> bin_seg;   // nseg: segments sorted by (bin, tile)
+  DevBuf<uint32_t> bin_ptr;   // nbins + 1
+  const double* den = nullptr;
+  size_t bytes() const {
+    return lidx.bytes() + w32.bytes() + w64.bytes() + seg_start.bytes() + tile_seg.bytes() +
+           bin_seg.bytes() + bin_ptr.bytes();
+  }
+};
+
+// Builds the tiled layout from a CSR execution table resident on the device.
+void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, uint64_t nnz,
+                 const uint32_t* row_ptr, const uint32_t* col, const float* w32,
+                 const double* w64, bool keep_w64, hipStream_t s);
+
+struct TileScratch {
+  DevBuf<double> part;
+  void ensure(const TiledTable& t, uint32_t frames, bool var) {
+    size_t need = (size_t)t.nseg * frames * (var ? 2 : 1);
+    if (part.n < need) part.alloc(need);
+  }
+};
+
+// Largest frame tile the kernel supports for this dtype (LDS capacity).
+uint32_t max_frames_for(int dtype);
+
+void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
+                  uint64_t n_frames, bool var, int precision, uint32_t frames_per_pass,
+                  double* intensity, double* sigma, hipStream_t s);
+
+// Only the tile kernel for one pass of nf frames (kernel timing).
+void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
+                     uint32_t nf, bool var, int precision, hipStream_t s);
+
+}  // namespace dio
