@@ -26,7 +26,8 @@ EXPORTS = [
     "dio_plan_normalization", "dio_integrate", "dio_integrate_device", "dio_plan_autotune",
     "dio_plan_set_tile", "dio_graph_create", "dio_graph_launch", "dio_graph_destroy",
     "dio_plan_time_gather", "dio_device_alloc", "dio_device_free", "dio_memcpy",
-    "dio_synchronize", "dio_synthetic_poisson_u32",
+    "dio_synchronize", "dio_synthetic_poisson_u32", "dio_server_create", "dio_server_submit",
+    "dio_server_poll", "dio_server_stats_get", "dio_server_destroy",
 ]
 
 
@@ -73,6 +74,21 @@ class DeviceInfo(C.Structure):
                 ("runtime_version", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("n_slots", C.c_uint32), ("plan_cache_entries", C.c_uint32),
+                ("hbm_budget_bytes", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("completed", C.c_uint64), ("failed", C.c_uint64), ("frames", C.c_uint64),
+        ("queued", C.c_uint64), ("inflight", C.c_uint32), ("peak_inflight", C.c_uint32),
+        ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64), ("plan_builds", C.c_uint64),
+        ("cache_hits", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+        ("mean_ms", C.c_double), ("max_ms", C.c_double), ("span_s", C.c_double),
+        ("frames_per_s", C.c_double)]
+
+
 _lib = None
 
 
@@ -114,6 +130,13 @@ def lib() -> C.CDLL:
             "dio_memcpy": (i32, [vp, vp, vp, u64, i32]),
             "dio_synchronize": (i32, [vp]),
             "dio_synthetic_poisson_u32": (i32, [vp, vp, u64, vp, u64, u64, vp]),
+            "dio_server_create": (i32, [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
+            "dio_server_submit": (i32, [vp, C.POINTER(Poni), C.POINTER(PlanDesc), vp, i32, u64,
+                                        u32, dp, dp, dp, C.POINTER(u64)]),
+            "dio_server_poll": (i32, [vp, u64, C.c_double, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double)]),
+            "dio_server_stats_get": (i32, [vp, C.POINTER(ServerStats)]),
+            "dio_server_destroy": (i32, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

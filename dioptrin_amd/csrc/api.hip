@@ -9,47 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "builder.hpp"
-#include "gather.hpp"
-#include "tile.hpp"
-
-struct dio_ctx {
-  int device = 0;
-  hipDeviceProp_t prop{};
-  int refs = 0;
-};
-
-struct dio_plan {
-  dio_ctx* ctx = nullptr;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  dio_plan_desc desc{};
-  dio::Geo geo{};
-  dio::Built b;
-  dio::GatherTable gt;
-  dio::StdTable st;
-  dio::TiledTable tt;
-  dio::TileScratch sc;
-  uint32_t tile = 16;
-  double build_ms = 0;
-  std::mutex mu;
-  dio::DevBuf<uint8_t> stage;
-  dio::DevBuf<double> out_i, out_s, out_d;
-  ~dio_plan() {
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
-
-struct dio_graph {
-  dio_plan* plan = nullptr;
-  dio::TileScratch sc;  // owned: the plan's scratch may be reallocated later
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  ~dio_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-  }
-};
+#include "plan.hpp"
 
 namespace {
 
@@ -121,23 +81,63 @@ void ensure_out(dio::DevBuf<double>& b, size_t n) {
   if (b.n < n) b.alloc(n);
 }
 
+}  // namespace
+
+namespace dio {
 // Enqueue the whole integration of device frames on `s`.
-void run_device(dio_plan* p, dio::TileScratch& sc, const void* frames, int dtype,
+void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
                 uint64_t n_frames, uint32_t flags, double* I, double* S, double* SD,
                 hipStream_t s) {
   bool var = flags & DIO_WITH_ERRORS;
   if (flags & DIO_WITH_AZIMUTHAL_STD) {
     require(p->desc.ndim == 1, DIO_ERR_UNSUPPORTED, "azimuthal std is defined for 1D plans");
     require(SD != nullptr, DIO_ERR_INVALID, "azimuthal std output missing");
-    dio::azimuthal_std_frames(p->st, frames, dtype, n_frames, var, I, S, SD, s);
+    azimuthal_std_frames(p->st, frames, dtype, n_frames, var, I, S, SD, s);
     return;
   }
   require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
-  uint32_t F = std::min<uint32_t>(p->tile, dio::max_frames_for(dtype));
-  dio::tiled_gather(p->tt, sc, frames, dtype, n_frames, var, p->desc.precision, F, I, S, s);
+  uint32_t F = std::min<uint32_t>(p->tile, max_frames_for(dtype));
+  tiled_gather(p->tt, sc, frames, dtype, n_frames, var, p->desc.precision, F, I, S, s);
 }
 
-}  // namespace
+}  // namespace dio
+
+namespace dio {
+dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& desc) {
+  check_desc(desc);
+    auto t0 = std::chrono::steady_clock::now();
+    DIO_HIP(hipSetDevice(ctx->device));
+    auto p = std::make_unique<dio_plan>();
+    p->ctx = ctx;
+    p->device = ctx->device;
+    p->desc = desc;
+    p->desc.mask = nullptr;
+    DIO_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->geo = dio::make_geo(poni, desc);
+    dio::build_tables(p->geo, desc, p->stream, p->b);
+    const dio::Table& t = p->b.exec_table();
+    p->gt.col = t.col.p;
+    p->gt.w32 = t.w32.p;
+    p->gt.w64 = t.w64.p;
+    p->gt.den = p->b.den.p;
+    p->gt.nbins = t.nbins;
+    p->gt.npix = (uint64_t)desc.dim1 * desc.dim2;
+    p->gt.nnz = t.nnz;
+    dio::build_tiled(p->tt, desc.dim1, desc.dim2, t.nbins, t.nnz, t.row_ptr.p, t.col.p,
+                     t.w32.p, t.w64.p, desc.precision == DIO_F64, p->stream);
+    p->tt.den = p->b.den.p;
+    p->st.row_ptr = p->b.full.row_ptr.p;
+    p->st.col = p->b.full.col.p;
+    p->st.w32 = p->b.full.w32.p;
+    p->st.mask = p->b.mask.p;
+    p->st.norm = p->b.norm.p;
+    p->st.nbins = p->b.full.nbins;
+    p->st.npix = p->gt.npix;
+    p->build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return p.release();
+}
+}  // namespace dio
 
 void dio_set_last_error(const char* msg) { g_last_error = msg; }
 
@@ -200,37 +200,7 @@ int dio_plan_create(dio_ctx* ctx, const dio_poni* poni, const dio_plan_desc* des
     *out = nullptr;
     check_desc(*desc);
     require(ctx != nullptr, DIO_ERR_INVALID, "null context");
-    auto t0 = std::chrono::steady_clock::now();
-    DIO_HIP(hipSetDevice(ctx->device));
-    auto p = std::make_unique<dio_plan>();
-    p->ctx = ctx;
-    p->device = ctx->device;
-    p->desc = *desc;
-    p->desc.mask = nullptr;
-    DIO_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-    p->geo = dio::make_geo(*poni, *desc);
-    dio::build_tables(p->geo, *desc, p->stream, p->b);
-    const dio::Table& t = p->b.exec_table();
-    p->gt.col = t.col.p;
-    p->gt.w32 = t.w32.p;
-    p->gt.w64 = t.w64.p;
-    p->gt.den = p->b.den.p;
-    p->gt.nbins = t.nbins;
-    p->gt.npix = (uint64_t)desc->dim1 * desc->dim2;
-    p->gt.nnz = t.nnz;
-    dio::build_tiled(p->tt, desc->dim1, desc->dim2, t.nbins, t.nnz, t.row_ptr.p, t.col.p,
-                     t.w32.p, t.w64.p, desc->precision == DIO_F64, p->stream);
-    p->tt.den = p->b.den.p;
-    p->st.row_ptr = p->b.full.row_ptr.p;
-    p->st.col = p->b.full.col.p;
-    p->st.w32 = p->b.full.w32.p;
-    p->st.mask = p->b.mask.p;
-    p->st.norm = p->b.norm.p;
-    p->st.nbins = p->b.full.nbins;
-    p->st.npix = p->gt.npix;
-    p->build_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    *out = p.release();
+    *out = dio::make_plan(ctx, *poni, *desc);
   });
 }
 

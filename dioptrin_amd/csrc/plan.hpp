@@ -1,0 +1,54 @@
+// Internal definitions of the C-ABI handles, shared by the API and the
+// serving loop.
+#pragma once
+#include <mutex>
+
+#include "builder.hpp"
+#include "gather.hpp"
+#include "tile.hpp"
+
+struct dio_ctx {
+  int device = 0;
+  hipDeviceProp_t prop{};
+  int refs = 0;
+};
+
+struct dio_plan {
+  dio_ctx* ctx = nullptr;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  dio_plan_desc desc{};
+  dio::Geo geo{};
+  dio::Built b;
+  dio::GatherTable gt;
+  dio::StdTable st;
+  dio::TiledTable tt;
+  dio::TileScratch sc;
+  uint32_t tile = 16;
+  double build_ms = 0;
+  std::mutex mu;
+  dio::DevBuf<uint8_t> stage;
+  dio::DevBuf<double> out_i, out_s, out_d;
+  ~dio_plan() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct dio_graph {
+  dio_plan* plan = nullptr;
+  dio::TileScratch sc;  // owned: the plan's scratch may be reallocated later
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~dio_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace dio {
+// Enqueues the integration of device frames on stream s (no synchronisation).
+void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype, uint64_t n_frames,
+                uint32_t flags, double* I, double* S, double* SD, hipStream_t s);
+// Plan construction behind dio_plan_create (validates, builds on the device).
+dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& desc);
+}  // namespace dio

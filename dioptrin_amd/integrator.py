@@ -101,6 +101,49 @@ def device_info(device: int = 0) -> dict:
             "total_memory_bytes": info.total_memory, "runtime_version": info.runtime_version}
 
 
+def plan_args(poni: dict, shape, method="pixel_split", n_ss=None, unit="2th_rad",
+              n_radial=1000, n_azimuthal=None, radial_range=None, azimuthal_range=None,
+              polarization_factor=None, mask=None, precision="f32_compensated"):
+    """The (dio_poni, dio_plan_desc, mask array) triple naming one plan. The
+    mask array must stay alive while the descriptor is in use."""
+    if method not in N.METHODS:
+        raise ValueError(f"unknown method {method!r}")
+    if unit not in N.UNITS:
+        raise ValueError(f"unknown unit {unit!r}")
+    if precision not in N.PRECISIONS:
+        raise ValueError(f"unknown precision {precision!r}; expected one of "
+                         f"{sorted(N.PRECISIONS)}")
+    dim1, dim2 = int(shape[0]), int(shape[1])
+    p = N.Poni(*(float(poni.get(k, 0.0)) for k in (
+        "pixel1", "pixel2", "distance", "poni1", "poni2", "rot1", "rot2", "rot3",
+        "wavelength")))
+    d = N.PlanDesc()
+    d.dim1, d.dim2 = dim1, dim2
+    d.method = N.METHODS[method]
+    d.n_ss = int(n_ss or (5 if method == "supersampled" else 0))
+    d.unit = N.UNITS[unit]
+    d.ndim = 1 if n_azimuthal is None else 2
+    d.n_radial = int(n_radial)
+    d.n_azimuthal = int(n_azimuthal or 1)
+    if radial_range is not None:
+        d.has_radial_range = 1
+        d.radial_min, d.radial_max = map(float, radial_range)
+    if azimuthal_range is not None:
+        d.has_azimuthal_range = 1
+        d.azimuthal_min, d.azimuthal_max = map(float, azimuthal_range)
+    if polarization_factor is not None:
+        d.has_polarization = 1
+        d.polarization_factor = float(polarization_factor)
+    m = None
+    if mask is not None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.shape != (dim1, dim2):
+            raise ValueError(f"mask shape {m.shape} does not match image shape {(dim1, dim2)}")
+        d.mask = m.ctypes.data
+    d.precision = N.PRECISIONS[precision]
+    return p, d, m
+
+
 class GpuPlan:
     """A lookup table built and kept on one GPU (Cuda1dPlan / Cuda2dPlan
     counterpart). A snapshot: later Integrator changes do not affect it."""
@@ -108,46 +151,13 @@ class GpuPlan:
     def __init__(self, poni: dict, shape, method="pixel_split", n_ss=None, unit="2th_rad",
                  n_radial=1000, n_azimuthal=None, radial_range=None, azimuthal_range=None,
                  polarization_factor=None, mask=None, precision="f32_compensated", device=0):
-        if method not in N.METHODS:
-            raise ValueError(f"unknown method {method!r}")
-        if unit not in N.UNITS:
-            raise ValueError(f"unknown unit {unit!r}")
-        if precision not in N.PRECISIONS:
-            raise ValueError(f"unknown precision {precision!r}; expected one of "
-                             f"{sorted(N.PRECISIONS)}")
-        dim1, dim2 = int(shape[0]), int(shape[1])
-        self.shape = (dim1, dim2)
+        self.shape = (int(shape[0]), int(shape[1]))
         self.ndim = 1 if n_azimuthal is None else 2
         self.precision = precision
         self.device = device
-        p = N.Poni(*(float(poni.get(k, 0.0)) for k in (
-            "pixel1", "pixel2", "distance", "poni1", "poni2", "rot1", "rot2", "rot3",
-            "wavelength")))
-        d = N.PlanDesc()
-        d.dim1, d.dim2 = dim1, dim2
-        d.method = N.METHODS[method]
-        d.n_ss = int(n_ss or (5 if method == "supersampled" else 0))
-        d.unit = N.UNITS[unit]
-        d.ndim = self.ndim
-        d.n_radial = int(n_radial)
-        d.n_azimuthal = int(n_azimuthal or 1)
-        if radial_range is not None:
-            d.has_radial_range = 1
-            d.radial_min, d.radial_max = map(float, radial_range)
-        if azimuthal_range is not None:
-            d.has_azimuthal_range = 1
-            d.azimuthal_min, d.azimuthal_max = map(float, azimuthal_range)
-        if polarization_factor is not None:
-            d.has_polarization = 1
-            d.polarization_factor = float(polarization_factor)
-        self._mask = None
-        if mask is not None:
-            m = np.ascontiguousarray(mask, dtype=np.uint8)
-            if m.shape != self.shape:
-                raise ValueError(f"mask shape {m.shape} does not match image shape {self.shape}")
-            self._mask = m
-            d.mask = m.ctypes.data
-        d.precision = N.PRECISIONS[precision]
+        p, d, self._mask = plan_args(poni, shape, method, n_ss, unit, n_radial, n_azimuthal,
+                                     radial_range, azimuthal_range, polarization_factor, mask,
+                                     precision)
         h = C.c_void_p()
         N.check(N.lib().dio_plan_create(_Context.get(device), C.byref(p), C.byref(d), C.byref(h)))
         self._h = h

@@ -43,6 +43,7 @@ enum { DIO_WITH_ERRORS = 1, DIO_WITH_AZIMUTHAL_STD = 2 };
 typedef struct dio_ctx dio_ctx;
 typedef struct dio_plan dio_plan;
 typedef struct dio_graph dio_graph;
+typedef struct dio_server dio_server;
 
 /* Poni (src/poni.rs:13-24): metres and radians. */
 typedef struct {
@@ -178,6 +179,46 @@ int dio_synchronize(dio_ctx *ctx);
 int dio_synthetic_poisson_u32(dio_ctx *ctx, const float *d_rate, uint64_t npix,
                               const double *background, uint64_t n_frames, uint64_t seed,
                               uint32_t *d_out);
+
+/* Continuous-batching server (no single reference counterpart: it extends the
+ * reference's one-entry table caches, src/integration/cache.rs:42-96, to an
+ * LRU of device-resident plans, and its one-stream-per-plan execution,
+ * src/integration/cuda/device.rs:1354-1584, to a pool of stream slots).
+ * Requests are admitted in arrival order when a slot is free and their
+ * device footprint (frames + outputs) fits hbm_budget_bytes; a request larger
+ * than the budget runs alone. Zeroed fields take the defaults shown. */
+typedef struct {
+  uint32_t n_slots;            /* concurrent requests (streams), default 4, max 64 */
+  uint32_t plan_cache_entries; /* device-resident plans kept, default 32 */
+  uint64_t hbm_budget_bytes;   /* request frames + outputs in flight, default 32 GiB */
+} dio_server_config;
+
+typedef struct {
+  uint64_t completed, failed, frames, queued;
+  uint32_t inflight, peak_inflight;
+  uint64_t bytes_in_use, peak_bytes; /* admitted request footprints */
+  uint64_t plan_builds, cache_hits;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* submit-to-result latency */
+  double span_s, frames_per_s;             /* first submit to last completion */
+} dio_server_stats;
+
+int dio_server_create(dio_ctx *ctx, const dio_server_config *cfg, dio_server **out);
+/* Queues one request; frames (host, n_frames x dim1 x dim2 of dtype) and the
+ * outputs (n_frames x nbins doubles each) are borrowed until poll reports it
+ * done. The plan is identified by (poni, desc, mask contents) like
+ * dio_plan_create's arguments (Integrator::integrate1d / integrate2d,
+ * src/integration/methods.rs:94-100, 461). */
+int dio_server_submit(dio_server *srv, const dio_poni *poni, const dio_plan_desc *desc,
+                      const void *frames, int dtype, uint64_t n_frames, uint32_t flags,
+                      double *intensity, double *sigma, double *std_out, uint64_t *ticket);
+/* Waits up to timeout_s (0: no wait) for a ticket; *done = 1 once its outputs
+ * are written, and the request's own status is returned then (the ticket is
+ * released). */
+int dio_server_poll(dio_server *srv, uint64_t ticket, double timeout_s, int *done,
+                    double *latency_ms);
+int dio_server_stats_get(dio_server *srv, dio_server_stats *out);
+/* Drains queued and running requests, then frees every plan and stream. */
+int dio_server_destroy(dio_server *srv);
 
 #ifdef __cplusplus
 }
