@@ -67,7 +67,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="f32_compensated")
-    ap.add_argument("--tile", type=int, default=0, help="frames per tile (0 = autotune)")
+    ap.add_argument("--tile", default="", help="frames per pass[,frames per lane] "
+                                               "(default: autotune)")
     ap.add_argument("--frames", type=int, default=256)
     ap.add_argument("--errors", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -112,9 +113,13 @@ def main():
     log(f"[bench] table: nnz {info['nnz']} segments {info['n_segments']} build "
         f"{info['build_ms']:.1f} ms uncertain {info['uncertain_angles']} singular "
         f"{info['singular_pixels']}")
-    tile = args.tile or plan.autotune("uint32", 64, errors=args.errors)
-    plan.set_tile(tile)
-    log(f"[bench] frames per tile {tile}")
+    if args.tile:
+        tile, lanes = (list(map(int, args.tile.split(","))) + [0])[:2]
+        plan.set_tile(tile, lanes)
+        lanes = lanes or min(tile, 4)
+    else:
+        tile, lanes = plan.autotune("uint32", 64, errors=args.errors)
+    log(f"[bench] frames per pass {tile}, frames per lane {lanes}")
 
     bins = plan.nbins
     out_i = DeviceArray((nfr, bins), np.float64)
@@ -143,7 +148,7 @@ def main():
 
     # Plain per-frame path: one call and one synchronisation per frame.
     plan1 = plan
-    plan1.set_tile(1)
+    plan1.set_tile(1, 1)
     n1 = min(nfr, 64)
     for k in range(4):
         plan1.integrate_device(frames[k:k + 1], out_i[k:k + 1], None if out_s is None else
@@ -155,7 +160,7 @@ def main():
                                out_s[k:k + 1], errors=args.errors)
         synchronize()
     per_frame_fps = n1 / (time.perf_counter() - t0)
-    plan.set_tile(tile)
+    plan.set_tile(tile, lanes)
     log(f"[bench] per-frame: {per_frame_fps:.0f} frames/s")
 
     # Dominant kernel: the segment gather of one tile, timed with HIP events
@@ -170,7 +175,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("tile") == tile and tr.get("precision") == args.precision:
+        if tr.get("tile") == [tile, lanes] and tr.get("precision") == args.precision:
             traffic = tr.get("hbm_bytes_per_launch")
 
     cpu = None
@@ -199,6 +204,7 @@ def main():
         "config": {"workload": f"{WORKLOAD}: {nfr} x {w['shape'][0]}x{w['shape'][1]} uint32, "
                                f"{bins} bins pixel_split, device-resident frames",
                    "frames_per_step": nfr, "frames_per_tile": tile,
+                   "frames_per_lane": lanes,
                    "precision": args.precision, "errors": bool(args.errors),
                    "graph": not args.no_graph, "table_entries": info["nnz_masked"]},
         "hbm_gbs_algorithmic": round(bytes_frame * fps / 1e9, 1),
@@ -208,7 +214,7 @@ def main():
         "lut_build_ms_gpu": round(info["build_ms"], 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_seg (segment gather, one tile)",
+                     "traffic": traffic, "kernel": "k_tile (detector-tiled gather, one pass)",
                      "kernel_ms": round(seg_ms, 5), "bytes_per_launch": int(per_launch)},
         "cpu_baseline": None if cpu is None else {k: (round(v, 2) if isinstance(v, float) else v)
                                                   for k, v in cpu.items()},

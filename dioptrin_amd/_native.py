@@ -118,8 +118,8 @@ def lib() -> C.CDLL:
             "dio_plan_normalization": (i32, [vp, dp]),
             "dio_integrate": (i32, [vp, vp, i32, u64, u32, dp, dp, dp]),
             "dio_integrate_device": (i32, [vp, vp, i32, u64, u32, dp, dp, dp, vp]),
-            "dio_plan_autotune": (i32, [vp, i32, u64, u32, C.POINTER(u32)]),
-            "dio_plan_set_tile": (i32, [vp, u32]),
+            "dio_plan_autotune": (i32, [vp, i32, u64, u32, C.POINTER(u32), C.POINTER(u32)]),
+            "dio_plan_set_tile": (i32, [vp, u32, u32]),
             "dio_graph_create": (i32, [vp, vp, i32, u64, u32, dp, dp, dp, C.POINTER(vp)]),
             "dio_graph_launch": (i32, [vp, vp]),
             "dio_graph_destroy": (i32, [vp]),

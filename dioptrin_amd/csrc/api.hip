@@ -96,8 +96,8 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
     return;
   }
   require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
-  uint32_t F = std::min<uint32_t>(p->tile, max_frames_for(dtype));
-  tiled_gather(p->tt, sc, frames, dtype, n_frames, var, p->desc.precision, F, I, S, s);
+  tiled_gather(p->tt, sc, frames, dtype, n_frames, var, p->desc.precision, p->tile, p->lanes, I,
+               S, s);
 }
 
 }  // namespace dio
@@ -126,6 +126,8 @@ dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& des
     dio::build_tiled(p->tt, desc.dim1, desc.dim2, t.nbins, t.nnz, t.row_ptr.p, t.col.p,
                      t.w32.p, t.w64.p, desc.precision == DIO_F64, p->stream);
     p->tt.den = p->b.den.p;
+    p->tt.cus = (uint32_t)std::max(1, ctx->prop.multiProcessorCount);
+    if (desc.precision == DIO_F64) p->tile = std::min(p->tile, dio::max_frames_for(DIO_F64));
     p->st.row_ptr = p->b.full.row_ptr.p;
     p->st.col = p->b.full.col.p;
     p->st.w32 = p->b.full.w32.p;
@@ -346,18 +348,21 @@ int dio_integrate_device(dio_plan* p, const void* d_frames, int dtype, uint64_t 
   });
 }
 
-int dio_plan_set_tile(dio_plan* p, uint32_t tile) {
+int dio_plan_set_tile(dio_plan* p, uint32_t tile, uint32_t lanes) {
   return guarded([&] {
     require(p != nullptr, DIO_ERR_INVALID, "null plan");
-    require(tile >= 1 && tile <= dio::kMaxFrames && (tile & (tile - 1)) == 0, DIO_ERR_INVALID,
-            "frames per tile must be a power of two in 1..32");
     std::lock_guard<std::mutex> lk(p->mu);
+    if (lanes == 0) lanes = std::min<uint32_t>(tile, 4);
+    require(dio::valid_tile(p->desc.precision, tile, lanes), DIO_ERR_INVALID,
+            "frames per pass must be a power of two in 1..32 (1..16 for f64) and frames per "
+            "lane a power of two in 1..min(4, frames per pass)");
     p->tile = tile;
+    p->lanes = lanes;
   });
 }
 
 int dio_plan_autotune(dio_plan* p, int dtype, uint64_t n_frames_hint, uint32_t flags,
-                      uint32_t* frames_per_tile) {
+                      uint32_t* frames_per_tile, uint32_t* frames_per_lane) {
   return guarded([&] {
     require(p != nullptr, DIO_ERR_INVALID, "null plan");
     std::lock_guard<std::mutex> lk(p->mu);
@@ -374,29 +379,34 @@ int dio_plan_autotune(dio_plan* p, int dtype, uint64_t n_frames_hint, uint32_t f
     DIO_HIP(hipEventCreate(&e0));
     DIO_HIP(hipEventCreate(&e1));
     double best = 1e300;
-    uint32_t best_tile = p->tile;
+    uint32_t best_tile = p->tile, best_lanes = p->lanes;
     for (uint32_t tile : {1u, 2u, 4u, 8u, 16u, 32u}) {
       if (tile > 1 && tile > nf * 2) break;
-      if (tile > dio::max_frames_for(dtype)) break;
-      dio::tiled_gather(p->tt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, oi.p, os.p,
-                        p->stream);  // warm-up
-      DIO_HIP(hipEventRecord(e0, p->stream));
-      for (int rep = 0; rep < 3; rep++)
-        dio::tiled_gather(p->tt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, oi.p,
-                          os.p, p->stream);
-      DIO_HIP(hipEventRecord(e1, p->stream));
-      DIO_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      DIO_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        best_tile = tile;
+      for (uint32_t lanes : {1u, 2u, 4u}) {
+        if (!dio::valid_tile(p->desc.precision, tile, lanes)) continue;
+        dio::tiled_gather(p->tt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, lanes,
+                          oi.p, os.p, p->stream);  // warm-up
+        DIO_HIP(hipEventRecord(e0, p->stream));
+        for (int rep = 0; rep < 3; rep++)
+          dio::tiled_gather(p->tt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, lanes,
+                            oi.p, os.p, p->stream);
+        DIO_HIP(hipEventRecord(e1, p->stream));
+        DIO_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        DIO_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          best_tile = tile;
+          best_lanes = lanes;
+        }
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     p->tile = best_tile;
+    p->lanes = best_lanes;
     if (frames_per_tile) *frames_per_tile = best_tile;
+    if (frames_per_lane) *frames_per_lane = best_lanes;
   });
 }
 
@@ -411,8 +421,7 @@ int dio_graph_create(dio_plan* p, const void* d_frames, int dtype, uint64_t n_fr
     // Scratch must exist before capture: allocation is not capturable.
     auto g = std::make_unique<dio_graph>();
     g->plan = p;
-    g->sc.ensure(p->tt, std::min<uint32_t>(p->tile, dio::max_frames_for(dtype)),
-                 flags & DIO_WITH_ERRORS);
+    g->sc.ensure(p->tt, p->tile, flags & DIO_WITH_ERRORS);
     DIO_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     try {
       run_device(p, g->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std,
@@ -456,10 +465,12 @@ int dio_plan_time_gather(dio_plan* p, const void* d_frames, int dtype, uint64_t 
     hipEvent_t e0, e1;
     DIO_HIP(hipEventCreate(&e0));
     DIO_HIP(hipEventCreate(&e1));
-    dio::tiled_pass_only(p->tt, p->sc, d_frames, dtype, nf, var, p->desc.precision, p->stream);
+    dio::tiled_pass_only(p->tt, p->sc, d_frames, dtype, nf, var, p->desc.precision, p->tile,
+                         p->lanes, p->stream);
     DIO_HIP(hipEventRecord(e0, p->stream));
     for (int r = 0; r < reps; r++)
-      dio::tiled_pass_only(p->tt, p->sc, d_frames, dtype, nf, var, p->desc.precision, p->stream);
+      dio::tiled_pass_only(p->tt, p->sc, d_frames, dtype, nf, var, p->desc.precision, p->tile,
+                           p->lanes, p->stream);
     DIO_HIP(hipEventRecord(e1, p->stream));
     DIO_HIP(hipEventSynchronize(e1));
     float ms = 0;

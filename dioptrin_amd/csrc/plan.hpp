@@ -24,7 +24,8 @@ struct dio_plan {
   dio::StdTable st;
   dio::TiledTable tt;
   dio::TileScratch sc;
-  uint32_t tile = 16;
+  uint32_t tile = 16;   // frames per pass
+  uint32_t lanes = 4;   // frames per lane
   double build_ms = 0;
   std::mutex mu;
   dio::DevBuf<uint8_t> stage;

@@ -1,22 +1,37 @@
 // Detector-tiled gather (gfx950): the batched integration hot path.
 //
-// HBM layout of a tiled plan: lidx u16[nnz] + w32 f32[nnz] (6 bytes per
-// entry), segment starts u32[nseg + 1], tile -> segment u32[ntiles + 1],
-// bin -> segment list u32[nseg] + u32[nbins + 1], denominators f64[nbins].
+// HBM layout of a tiled plan: lidx u16 + w32 f32 per entry (6 bytes), in
+// 4-entry groups; segment -> first group u32[nseg + 1]; tile -> first
+// segment u32[ntiles + 1]; bin -> segment list u32[nseg] + u32[nbins + 1];
+// denominators f64[nbins]. Partials of one pass: [1 + errors][nseg][F]
+// float32 (float64 for the F64 precision).
 //
-// k_tile: one workgroup per detector tile (16 rows x 64 columns). It copies
-// the tile of F frames from HBM into LDS with coalesced row loads (each frame
-// byte is read from HBM exactly once per pass), then one thread per (tile,
-// bin) segment applies the segment's entries, pixels ascending, to all F
-// frames held in registers, and writes one partial per frame. The table is
-// read once per pass of F frames. k_tfinal adds each bin's partials in tile
-// order and divides by the FP64 denominator. Neither the summation order of a
-// segment nor the order of partials depends on F, so a frame gives the same
-// bits alone or in any batch.
+// k_tile: persistent workgroups of 256 threads, each walking detector tiles
+// of 16 x 64 pixels (tile b, b + grid, ...).
+//  1. Staging: every thread owns 4 consecutive pixels of one tile row (16
+//     lanes cover a 256-byte row run) for each of the pass's F frames. For
+//     32-bit frames the next tile's words and its table chunk (4-entry
+//     groups: 8-byte indices + 16-byte weights) are loaded into registers
+//     while the current tile is computed, then converted once and stored to
+//     LDS as frames [frame][pixel] and the table chunk. Every frame byte is
+//     read from HBM exactly once per pass, the table once per pass.
+//  2. Gather: L = F / G lanes share a segment, each holding G frames in
+//     registers, and apply the segment's groups from LDS. Segments of a tile
+//     are stored longest first so the lanes of a wavefront run similar trip
+//     counts.
+//  3. One partial per (segment, frame), written frame-contiguous.
+// k_tfinal adds each bin's partials in a fixed order (coalesced over frames)
+// and divides by the FP64 denominator.
+//
+// The arithmetic of a frame (entry order inside a segment, partial order
+// inside a bin) depends only on the table, never on F, G or the batch, so a
+// frame gives the same bits alone, in any batch and in any pass size.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
+#include <type_traits>
 #include <vector>
 
 #include "tile.hpp"
@@ -26,6 +41,7 @@ namespace dio {
 namespace {
 
 constexpr unsigned kThreads = 256;
+static_assert(kTileRows * (kTileCols / 4) == kThreads, "one 4-pixel run per thread per frame");
 
 // ---------------------------------------------------------------- layout
 
@@ -75,14 +91,60 @@ __global__ void k_seg_flags(const uint32_t* tile, const uint32_t* bin, uint64_t 
 }
 
 __global__ void k_seg_fill(const uint32_t* flag, const uint32_t* segid, const uint32_t* tile,
-                           const uint32_t* bin, uint64_t nnz, uint32_t* seg_start,
+                           const uint32_t* bin, uint64_t nnz, uint32_t* seg_first,
                            uint32_t* seg_tile, uint32_t* seg_bin) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnz || !flag[i]) return;
   uint32_t s = segid[i];
-  seg_start[s] = (uint32_t)i;
+  seg_first[s] = (uint32_t)i;
   seg_tile[s] = tile[i];
   seg_bin[s] = bin[i];
+}
+
+// Sort key placing a tile's segments longest first (ties keep bin order).
+__global__ void k_seg_keys(const uint32_t* seg_first, const uint32_t* seg_tile, uint32_t nseg,
+                           uint64_t* key) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  uint32_t ng = (seg_first[s + 1] - seg_first[s] + kGroup - 1) / kGroup;
+  key[s] = ((uint64_t)seg_tile[s] << 32) | (0xFFFFFFFFu - ng);
+}
+
+__global__ void k_seg_perm(const uint32_t* perm, const uint32_t* seg_first,
+                           const uint32_t* seg_tile, const uint32_t* seg_bin, uint32_t nseg,
+                           uint32_t* inv, uint32_t* ngroups, uint32_t* tile_n, uint32_t* bin_n) {
+  uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n > nseg) return;
+  if (n == nseg) {
+    ngroups[n] = 0;
+    return;
+  }
+  uint32_t o = perm[n];
+  inv[o] = n;
+  ngroups[n] = (seg_first[o + 1] - seg_first[o] + kGroup - 1) / kGroup;
+  tile_n[n] = seg_tile[o];
+  bin_n[n] = seg_bin[o];
+}
+
+__global__ void k_pad_fill(uint64_t n, uint16_t* lidx, float* w32, double* w64) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  lidx[i] = (uint16_t)kPadSlot;
+  w32[i] = 0.0f;
+  if (w64) w64[i] = 0.0;
+}
+
+__global__ void k_place(const uint32_t* flag, const uint32_t* segid, const uint32_t* seg_first,
+                        const uint32_t* inv, const uint32_t* seg_group, uint64_t nnz,
+                        const uint16_t* lidx_in, const float* w32_in, const double* w64_in,
+                        uint16_t* lidx, float* w32, double* w64) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  uint32_t so = segid[i] + flag[i] - 1;  // segment holding entry i
+  uint64_t dst = (uint64_t)seg_group[inv[so]] * kGroup + (i - seg_first[so]);
+  lidx[dst] = lidx_in[i];
+  w32[dst] = w32_in[i];
+  if (w64) w64[dst] = w64_in[i];
 }
 
 __global__ void k_count_u32(const uint32_t* key, uint32_t n, uint32_t* hist) {
@@ -127,254 +189,588 @@ void sort_pairs(const K* kin, K* kout, const V* vin, V* vout, uint64_t n, int bi
                                              0, bits, s));
 }
 
-// ---------------------------------------------------------------- kernels
-
-template <class P> __device__ inline float f32_of(P v) { return static_cast<float>(v); }
-template <class P> __device__ inline double wide_of(P v) {
-  if constexpr (sizeof(P) == 8) return (double)(float)v;
-  else return (double)v;
-}
+// ---------------------------------------------------------------- accumulators
 
 // Float32 fused multiply-add (CudaPrecision::F32).
 struct TPlain {
   float s;
   __device__ void init() { s = 0.0f; }
-  template <class P> __device__ void add(float w, double, P x) { s = fmaf(w, f32_of(x), s); }
-  template <class P> __device__ void add_var(float w, double, P x) {
-    s = fmaf(w * w, fmaxf(f32_of(x), 0.0f), s);
-  }
+  __device__ void add(float w, double, float x) { s = fmaf(w, x, s); }
+  __device__ void add_var(float w, double, float x) { s = fmaf(w * w, fmaxf(x, 0.0f), s); }
   __device__ double value() const { return (double)s; }
 };
 
-// Double-word float32 (CudaPrecision::F32Compensated): exact products by FMA
-// residual, Knuth TwoSum, renormalisation; non-finite totals kept as is.
+// Compensated float32 (CudaPrecision::F32Compensated): the running sum hi is
+// updated with Knuth's TwoSum; its rounding errors and the FMA residuals of
+// the products (exact: w*x = p + pe) are accumulated in lo (Ogita-Rump-Oishi
+// Sum2 with error-free products, error ~ u|s| + n^2 u^2 sum|p|). A non-finite
+// hi is returned as is.
 struct TComp {
   float hi, lo;
-  __device__ void init() { hi = 0.0f; lo = 0.0f; }
-  __device__ void pair(float th, float tl) {
-    float s = hi + th;
-    if (!(fabsf(s) <= FLT_MAX)) {
-      hi = s;
-      lo = 0.0f;
-      return;
-    }
+  __device__ void init() {
+    hi = 0.0f;
+    lo = 0.0f;
+  }
+  __device__ void two_sum(float p, float pe) {
+    float s = hi + p;
     float bb = s - hi;
-    float e = (hi - (s - bb)) + (th - bb);
-    float t = (e + lo) + tl;
-    float h = s + t;
-    lo = t - (h - s);
-    hi = h;
+    float err = (hi - (s - bb)) + (p - bb);
+    lo = lo + (err + pe);
+    hi = s;
   }
-  template <class P> __device__ void add(float w, double, P raw) {
-    float x = f32_of(raw);
+  __device__ void add(float w, double, float x) {
     float p = w * x;
-    pair(p, fmaf(w, x, -p));
+    two_sum(p, fmaf(w, x, -p));
   }
-  template <class P> __device__ void add_var(float w, double, P raw) {
-    float x = fmaxf(f32_of(raw), 0.0f);
+  __device__ void add_var(float w, double, float x) {
+    x = fmaxf(x, 0.0f);
     float w2 = w * w;
     float w2l = fmaf(w, w, -w2);
     float p = w2 * x;
-    float pl = fmaf(w2, x, -p);
-    pl = fmaf(w2l, x, pl);
-    pair(p, pl);
+    float pe = fmaf(w2l, x, fmaf(w2, x, -p));
+    two_sum(p, pe);
   }
-  __device__ double value() const { return (double)hi + (double)lo; }
+  __device__ double value() const {
+    if (!(fabsf(hi) <= FLT_MAX)) return (double)hi;
+    return (double)hi + (double)lo;
+  }
 };
 
-// Float32 storage, float64 register accumulation: one full-rate DFMA.
+// Float32 weights and signal, float64 register accumulation: one full-rate
+// DFMA per term on CDNA.
 struct TWide {
   double s;
   __device__ void init() { s = 0.0; }
-  template <class P> __device__ void add(float w, double, P x) { s = fma((double)w, wide_of(x), s); }
-  template <class P> __device__ void add_var(float w, double, P x) {
+  __device__ void add(float w, double, float x) { s = fma((double)w, (double)x, s); }
+  __device__ void add_var(float w, double, float x) {
     double wd = (double)w;
-    s = fma(wd * wd, fmax(wide_of(x), 0.0), s);
+    s = fma(wd * wd, (double)fmaxf(x, 0.0f), s);
   }
   __device__ double value() const { return s; }
 };
 
-// Float64 weights and accumulation (CudaPrecision::F64).
+// Float64 weights, signal and accumulation (CudaPrecision::F64).
 struct TF64 {
   double s;
   __device__ void init() { s = 0.0; }
-  template <class P> __device__ void add(float, double w, P x) { s = fma(w, (double)x, s); }
-  template <class P> __device__ void add_var(float, double w, P x) {
-    s = fma(w * w, fmax((double)x, 0.0), s);
-  }
+  __device__ void add(float, double w, double x) { s = fma(w, x, s); }
+  __device__ void add_var(float, double w, double x) { s = fma(w * w, fmax(x, 0.0), s); }
   __device__ double value() const { return s; }
 };
 
 template <int MODE> struct TAcc;
-template <> struct TAcc<0> { using T = TPlain; };
-template <> struct TAcc<1> { using T = TComp; };
-template <> struct TAcc<2> { using T = TF64; };
-template <> struct TAcc<3> { using T = TWide; };
+// T: accumulator; L: frame value type in LDS; P: stored segment partial
+// (float32 partials add a relative error of at most 2^-24 per partial).
+template <> struct TAcc<0> { using T = TPlain; using L = float; using P = float; };
+template <> struct TAcc<1> { using T = TComp; using L = float; using P = float; };
+template <> struct TAcc<2> { using T = TF64; using L = double; using P = double; };
+template <> struct TAcc<3> { using T = TWide; using L = float; using P = float; };
 
 struct TileArgs {
+  const void* frames;
+  int dtype;
   const uint16_t* lidx;
   const float* w32;
   const double* w64;
-  const uint32_t* seg_start;
+  const uint32_t* seg_group;
   const uint32_t* tile_seg;
-  uint32_t nseg, ntx, dim1, dim2;
+  uint32_t nseg, ntx, ntiles, dim1, dim2;
   uint64_t npix;
   uint32_t nf;
-  double* part;
+  uint32_t aligned8;  // frame rows and blocks are 8-byte aligned (32-bit frames)
+  void* part;
 };
 
-template <int F, int MODE, class P, bool VAR>
-__global__ __launch_bounds__(kThreads) void k_tile(TileArgs a, const P* __restrict__ frames) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  P* lds = reinterpret_cast<P*>(lds_raw);
-  using Acc = typename TAcc<MODE>::T;
-  uint32_t t = blockIdx.x;
-  uint32_t s0 = a.tile_seg[t], s1 = a.tile_seg[t + 1];
-  if (s0 == s1) return;
-  uint32_t row0 = (t / a.ntx) * kTileRows, col0 = (t % a.ntx) * kTileCols;
-  // Stage the tile of every frame of the pass: consecutive threads read
-  // consecutive columns of one detector row (coalesced).
-  for (uint32_t e = threadIdx.x; e < (uint32_t)F * kTilePixels; e += kThreads) {
-    uint32_t f = e / kTilePixels, px = e % kTilePixels;
-    uint32_t r = row0 + px / kTileCols, c = col0 + px % kTileCols;
-    P v = P(0);
-    if (f < a.nf && r < a.dim1 && c < a.dim2) v = frames[(uint64_t)f * a.npix + (uint64_t)r * a.dim2 + c];
-    lds[e] = v;
+// ---------------------------------------------------------------- LDS plan
+
+// One workgroup's LDS: frames [F][kLdsStride] of LT, then a table chunk of at
+// most kCap groups (kCap uint2 indices, then kCap float4 weights, or 2 x
+// double2 per group for float64 weights). The budget lets kWgs workgroups
+// share a CU (160 KiB of LDS) so one stages while another computes.
+template <int F, int MODE> struct Plan {
+  using LT = typename TAcc<MODE>::L;
+  static constexpr uint32_t kWgs = F >= 16 ? 2 : F == 8 ? (MODE == 2 ? 2 : 3) : F == 4 ? (MODE == 2 ? 3 : 4) : 4;
+  static constexpr size_t kFrameBytes = (size_t)F * kLdsStride * sizeof(LT);
+  static constexpr size_t kWBytes = MODE == 2 ? 32 : 16;
+  static constexpr size_t kBudget = (160 * 1024 / kWgs) / 256 * 256;
+  static constexpr size_t kRoom = (kBudget - kFrameBytes) / (8 + kWBytes);
+  static constexpr uint32_t kCap = kRoom >= 1024 ? 1024 : (uint32_t)(kRoom / 32 * 32);
+  static constexpr size_t kBytes = kFrameBytes + (size_t)kCap * (8 + kWBytes);
+  static constexpr int kKT = (kCap + kThreads - 1) / kThreads;  // table loads per thread
+  static_assert(kBudget > kFrameBytes, "frames exceed the LDS budget");
+  // A (tile, bin) segment holds at most one entry per tile pixel.
+  static_assert(kCap >= kTilePixels / kGroup, "a whole segment must fit one table chunk");
+};
+
+// ---------------------------------------------------------------- staging
+
+template <class P, size_t A> struct alignas(A) Run2 { P v[2]; };
+template <class LT> struct alignas(2 * sizeof(LT)) Lds2 { LT v[2]; };
+
+// Staging map: thread tid owns pixel pairs (row h * 8 + tid / 32, columns
+// 2 * (tid % 32) + {0, 1}) of the tile for h = 0, 1, so 32 lanes read one
+// 256-byte row run and a wavefront two rows per load.
+struct TileGeom {
+  uint32_t r0, cp;
+  uint32_t nvalid[2];  // valid pixels of the pair in rows r0 and r0 + 8
+  uint64_t off[2];
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const TileArgs& a, uint32_t t) {
+  TileGeom g;
+  g.r0 = threadIdx.x / 32;
+  g.cp = threadIdx.x % 32;
+  const uint32_t c = (t % a.ntx) * kTileCols + g.cp * 2;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t r = (t / a.ntx) * kTileRows + g.r0 + 8 * h;
+    g.nvalid[h] = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
+    g.off[h] = (uint64_t)r * a.dim2 + c;
   }
-  __syncthreads();
-  for (uint32_t s = s0 + threadIdx.x; s < s1; s += kThreads) {
-    Acc acc[F], var[F];
+  return g;
+}
+
+__device__ __forceinline__ uint32_t lds_pixel(uint32_t r0, uint32_t cp, int h) {
+  return (r0 + 8 * h) * kTileCols + cp * 2;
+}
+
+// Raw 32-bit frame words of one tile held in registers (u32, i32, f32
+// frames), loaded one tile ahead of their use.
+template <int F> struct FrameRegs {
+  uint32_t v[F][2][2];
+};
+
+// Pair loads are 8-byte loads when every pair starts 8-byte aligned (even row
+// length and an 8-byte aligned frame block), else two 4-byte loads.
+template <int F, size_t A>
+__device__ __forceinline__ void frames_load32_as(const TileArgs& a, const TileGeom& g,
+                                                 FrameRegs<F>& fr) {
+  const uint32_t* src = static_cast<const uint32_t*>(a.frames);
 #pragma unroll
-    for (int f = 0; f < F; f++) {
-      acc[f].init();
-      if constexpr (VAR) var[f].init();
-    }
-    uint32_t k1 = a.seg_start[s + 1];
-    for (uint32_t k = a.seg_start[s]; k < k1; k++) {
-      uint32_t l = a.lidx[k];
-      float w = a.w32[k];
-      double wd = MODE == 2 ? a.w64[k] : 0.0;
+  for (int f = 0; f < F; f++) {
 #pragma unroll
-      for (int f = 0; f < F; f++) {
-        P x = lds[f * kTilePixels + l];
-        acc[f].add(w, wd, x);
-        if constexpr (VAR) var[f].add_var(w, wd, x);
-      }
-    }
-#pragma unroll
-    for (int f = 0; f < F; f++) {
-      if (f < (int)a.nf) {
-        a.part[(uint64_t)f * a.nseg + s] = acc[f].value();
-        if constexpr (VAR) a.part[(uint64_t)(F + f) * a.nseg + s] = var[f].value();
+    for (int h = 0; h < 2; h++) {
+      const uint32_t* p = src + f * a.npix + g.off[h];
+      if ((uint32_t)f < a.nf && g.nvalid[h] == 2) {
+        const Run2<uint32_t, A> r = *reinterpret_cast<const Run2<uint32_t, A>*>(p);
+        fr.v[f][h][0] = r.v[0];
+        fr.v[f][h][1] = r.v[1];
+      } else {
+        fr.v[f][h][0] = ((uint32_t)f < a.nf && g.nvalid[h] > 0) ? p[0] : 0u;
+        fr.v[f][h][1] = 0u;
       }
     }
   }
 }
 
-// prepared.rs:985-997 (normalize_bin) after adding a bin's partials in tile
-// order: I = s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0.
-template <bool PLAIN, bool VAR>
-__global__ void k_tfinal(const double* __restrict__ part, uint32_t nseg, uint32_t F,
-                         const uint32_t* __restrict__ bin_ptr, const uint32_t* __restrict__ bin_seg,
-                         const double* __restrict__ den, uint32_t nbins, uint32_t nf,
-                         double* __restrict__ I, double* __restrict__ S) {
-  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (uint64_t)nbins * nf) return;
-  uint32_t b = (uint32_t)(t % nbins), f = (uint32_t)(t / nbins);
-  uint32_t q0 = bin_ptr[b], q1 = bin_ptr[b + 1];
-  double s, v = 0.0;
-  if constexpr (PLAIN) {
-    float sf = 0.0f, vf = 0.0f;
-    for (uint32_t q = q0; q < q1; q++) {
-      uint32_t g = bin_seg[q];
-      sf = sf + (float)part[(uint64_t)f * nseg + g];
-      if constexpr (VAR) vf = vf + (float)part[(uint64_t)(F + f) * nseg + g];
-    }
-    s = sf;
-    v = vf;
-  } else {
-    s = 0.0;
-    for (uint32_t q = q0; q < q1; q++) {
-      uint32_t g = bin_seg[q];
-      s = s + part[(uint64_t)f * nseg + g];
-      if constexpr (VAR) v = v + part[(uint64_t)(F + f) * nseg + g];
+template <int F>
+__device__ __forceinline__ void frames_load32(const TileArgs& a, uint32_t t, FrameRegs<F>& fr) {
+  const TileGeom g = tile_geom(a, t);
+  if (a.aligned8) frames_load32_as<F, 8>(a, g, fr);
+  else frames_load32_as<F, 4>(a, g, fr);
+}
+
+template <class W, class LT>
+__device__ __forceinline__ LT word_to(uint32_t w) {
+  if constexpr (std::is_same<W, float>::value) return static_cast<LT>(__uint_as_float(w));
+  else return static_cast<LT>(static_cast<W>(w));
+}
+
+template <int F, class LT>
+__device__ __forceinline__ void pad_slots(LT* fl) {
+  if (threadIdx.x < (unsigned)F) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) fl[threadIdx.x * kLdsStride + kPadSlot + i] = LT(0);
+  }
+}
+
+template <int F, class W, class LT>
+__device__ __forceinline__ void frames_store_as(const FrameRegs<F>& fr, LT* fl) {
+  const uint32_t r0 = threadIdx.x / 32, cp = threadIdx.x % 32;
+#pragma unroll
+  for (int f = 0; f < F; f++) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      Lds2<LT> o;
+      o.v[0] = word_to<W, LT>(fr.v[f][h][0]);
+      o.v[1] = word_to<W, LT>(fr.v[f][h][1]);
+      *reinterpret_cast<Lds2<LT>*>(fl + f * kLdsStride + lds_pixel(r0, cp, h)) = o;
     }
   }
-  double d = den[b];
-  I[(uint64_t)f * nbins + b] = d == 0.0 ? 0.0 : s / d;
+}
+
+template <int F, class LT>
+__device__ __forceinline__ void frames_store32(const TileArgs& a, const FrameRegs<F>& fr, LT* fl) {
+  if (a.dtype == DIO_U32) frames_store_as<F, uint32_t, LT>(fr, fl);
+  else if (a.dtype == DIO_I32) frames_store_as<F, int32_t, LT>(fr, fl);
+  else frames_store_as<F, float, LT>(fr, fl);
+  pad_slots<F>(fl);
+}
+
+// Direct staging (u16 and f64 frames): load and store, eight frames of loads
+// in flight per thread.
+template <int F, class P, class LT>
+__device__ __forceinline__ void stage_direct(const TileArgs& a, uint32_t t, LT* fl) {
+  const P* fr = static_cast<const P*>(a.frames);
+  const TileGeom g = tile_geom(a, t);
+  constexpr int B = F < 8 ? F : 8;
+#pragma unroll
+  for (int k0 = 0; k0 < F; k0 += B) {
+    P v[B][2][2];
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+      const uint32_t f = k0 + k;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const P* p = fr + f * a.npix + g.off[h];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+          v[k][h][i] = (f < a.nf && (uint32_t)i < g.nvalid[h]) ? p[i] : P(0);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        Lds2<LT> o;
+        o.v[0] = static_cast<LT>(v[k][h][0]);
+        o.v[1] = static_cast<LT>(v[k][h][1]);
+        *reinterpret_cast<Lds2<LT>*>(fl + (k0 + k) * kLdsStride + lds_pixel(g.r0, g.cp, h)) = o;
+      }
+    }
+  }
+  pad_slots<F>(fl);
+}
+
+// Table groups [gb, ge) of one chunk held in registers.
+template <int KT, int MODE> struct TableRegs {
+  uint2 li[KT];
+  float4 w[KT];
+  double2 wa[MODE == 2 ? KT : 1], wb[MODE == 2 ? KT : 1];
+};
+
+template <int KT, int MODE>
+__device__ __forceinline__ void table_load(const TileArgs& a, uint32_t gb, uint32_t ge,
+                                           TableRegs<KT, MODE>& tr) {
+  const uint2* lidx4 = reinterpret_cast<const uint2*>(a.lidx);
+#pragma unroll
+  for (int k = 0; k < KT; k++) {
+    const uint32_t g = gb + threadIdx.x + k * kThreads;
+    if (g < ge) {
+      tr.li[k] = lidx4[g];
+      if constexpr (MODE == 2) {
+        const double2* wd2 = reinterpret_cast<const double2*>(a.w64);
+        tr.wa[k] = wd2[2 * g];
+        tr.wb[k] = wd2[2 * g + 1];
+      } else {
+        tr.w[k] = reinterpret_cast<const float4*>(a.w32)[g];
+      }
+    }
+  }
+}
+
+template <int KT, int MODE, uint32_t CAP>
+__device__ __forceinline__ void table_store(const TableRegs<KT, MODE>& tr, uint32_t n,
+                                            uint2* tl, unsigned char* twb) {
+#pragma unroll
+  for (int k = 0; k < KT; k++) {
+    const uint32_t i = threadIdx.x + k * kThreads;
+    if (i < n) {
+      tl[i] = tr.li[k];
+      if constexpr (MODE == 2) {
+        double2* tw = reinterpret_cast<double2*>(twb);
+        tw[2 * i] = tr.wa[k];
+        tw[2 * i + 1] = tr.wb[k];
+      } else {
+        reinterpret_cast<float4*>(twb)[i] = tr.w[k];
+      }
+    }
+  }
+}
+
+// Loads tile tt's frames and first table chunk into registers; returns the
+// number of groups loaded (0 for a tile without entries).
+template <int F, int KT, int MODE, uint32_t C>
+__device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, FrameRegs<F>& fr,
+                                             TableRegs<KT, MODE>& tr) {
+  const uint32_t s0 = a.tile_seg[tt], s1 = a.tile_seg[tt + 1];
+  if (s0 == s1) return 0;
+  const uint32_t g0 = a.seg_group[s0];
+  const uint32_t n = min(a.seg_group[s1] - g0, C);
+  frames_load32<F>(a, tt, fr);
+  table_load<KT, MODE>(a, g0, g0 + n, tr);
+  return n;
+}
+
+// ---------------------------------------------------------------- tile kernel
+
+// Persistent: workgroup b processes tiles b, b + grid, ...; for 32-bit frames
+// the next tile's frames and first table chunk are loaded into registers
+// while the current tile is computed.
+template <int F, int G, int MODE, bool VAR, bool PF>
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs))) void k_tile(TileArgs a) {
+  using Acc = typename TAcc<MODE>::T;
+  using LT = typename TAcc<MODE>::L;
+  using PT = typename TAcc<MODE>::P;
+  using PL = Plan<F, MODE>;
+  constexpr uint32_t C = PL::kCap;
+  constexpr int KT = PL::kKT;
+  constexpr uint32_t L = F / G;
+  constexpr uint32_t SLOTS = kThreads / L;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  LT* fl = reinterpret_cast<LT*>(lds_raw);
+  uint2* tl = reinterpret_cast<uint2*>(lds_raw + PL::kFrameBytes);
+  unsigned char* twb = lds_raw + PL::kFrameBytes + (size_t)C * 8;
+
+  constexpr bool pf = PF;  // 32-bit frames: register prefetch of the next tile
+  const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
+  const LT* base = fl + q * G * kLdsStride;
+
+  FrameRegs<F> fr;
+  TableRegs<KT, MODE> tr;
+  uint32_t t = blockIdx.x;
+  uint32_t pn = 0;  // groups of the prefetched chunk
+  if constexpr (pf) {
+    if (t < a.ntiles) pn = prefetch<F, KT, MODE, C>(a, t, fr, tr);
+  }
+
+  for (; t < a.ntiles; t += gridDim.x) {
+    const uint32_t s0 = a.tile_seg[t], s1 = a.tile_seg[t + 1];
+    if (s0 == s1) {  // nothing of the table in this tile
+      if constexpr (pf) {
+        if (t + gridDim.x < a.ntiles) pn = prefetch<F, KT, MODE, C>(a, t + gridDim.x, fr, tr);
+      }
+      continue;
+    }
+    const uint32_t g0 = a.seg_group[s0], g1 = a.seg_group[s1];
+    __syncthreads();  // the previous tile's compute is done with the LDS
+    if constexpr (pf) {
+      frames_store32<F, LT>(a, fr, fl);
+      table_store<KT, MODE, C>(tr, pn, tl, twb);
+    } else {
+      switch (a.dtype) {
+        case DIO_U16: stage_direct<F, uint16_t, LT>(a, t, fl); break;
+        default: stage_direct<F, double, LT>(a, t, fl); break;
+      }
+      TableRegs<KT, MODE> td;
+      const uint32_t n = min(g1 - g0, C);
+      table_load<KT, MODE>(a, g0, g0 + n, td);
+      table_store<KT, MODE, C>(td, n, tl, twb);
+    }
+    __syncthreads();
+    if constexpr (pf) {
+      if (t + gridDim.x < a.ntiles) pn = prefetch<F, KT, MODE, C>(a, t + gridDim.x, fr, tr);
+    }
+
+    uint32_t sa = s0, gc = g0;
+    while (sa < s1) {
+      uint32_t sb = s1;
+      if (g1 - gc > C) {  // largest sb with seg_group[sb] - gc <= C
+        uint32_t lo = sa + 1, hi = s1;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) / 2;
+          if (a.seg_group[mid] - gc <= C) lo = mid;
+          else hi = mid - 1;
+        }
+        sb = lo;
+      }
+      if (sa != s0) {  // a later chunk of a large tile: load it now
+        __syncthreads();
+        TableRegs<KT, MODE> td;
+        const uint32_t n = a.seg_group[sb] - gc;
+        table_load<KT, MODE>(a, gc, gc + n, td);
+        table_store<KT, MODE, C>(td, n, tl, twb);
+        __syncthreads();
+      }
+      for (uint32_t s = sa + slot; s < sb; s += SLOTS) {
+        const uint32_t ga = a.seg_group[s] - gc, ge = a.seg_group[s + 1] - gc;
+        Acc acc[G], var[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          acc[j].init();
+          if constexpr (VAR) var[j].init();
+        }
+        for (uint32_t g = ga; g < ge; g++) {
+          const uint2 lc = tl[g];
+          const uint32_t l[4] = {lc.x & 0xffffu, lc.x >> 16, lc.y & 0xffffu, lc.y >> 16};
+          float wf[4] = {0.f, 0.f, 0.f, 0.f};
+          double wd[4] = {0.0, 0.0, 0.0, 0.0};
+          if constexpr (MODE == 2) {
+            const double2* tw = reinterpret_cast<const double2*>(twb);
+            const double2 x0 = tw[2 * g], x1 = tw[2 * g + 1];
+            wd[0] = x0.x;
+            wd[1] = x0.y;
+            wd[2] = x1.x;
+            wd[3] = x1.y;
+          } else {
+            const float4 w4 = reinterpret_cast<const float4*>(twb)[g];
+            wf[0] = w4.x;
+            wf[1] = w4.y;
+            wf[2] = w4.z;
+            wf[3] = w4.w;
+          }
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+              const LT x = base[j * kLdsStride + l[e]];
+              acc[j].add(wf[e], wd[e], x);
+              if constexpr (VAR) var[j].add_var(wf[e], wd[e], x);
+            }
+          }
+        }
+        PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * F + q * G;
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          if (q * G + j < a.nf) {
+            out[j] = static_cast<PT>(acc[j].value());
+            if constexpr (VAR) out[(size_t)a.nseg * F + j] = static_cast<PT>(var[j].value());
+          }
+        }
+      }
+      sa = sb;
+      gc = a.seg_group[sb];
+    }
+  }
+}
+
+// prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =
+// s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0. A wavefront
+// serves 16 (bin, frame) pairs with 4 lanes each: lane j adds the bin's
+// partials j, j + 4, j + 8, ... in tile order, then the four lane sums are
+// combined as (s0 + s1) + (s2 + s3). The order depends only on the table.
+template <bool PLAIN, bool VAR, class PT>
+__global__ __launch_bounds__(256) void k_tfinal(const PT* __restrict__ part, uint32_t nseg,
+                                                uint32_t F, uint32_t logF,
+                                                const uint32_t* __restrict__ bin_ptr,
+                                                const uint32_t* __restrict__ bin_seg,
+                                                const double* __restrict__ den, uint32_t nbins,
+                                                uint32_t nf, double* __restrict__ I,
+                                                double* __restrict__ S) {
+  using AT = typename std::conditional<PLAIN, float, double>::type;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t j = lane >> 4, pair = lane & 15;
+  const uint64_t b = wave * (16u >> logF) + (pair >> logF);
+  const uint32_t f = pair & (F - 1);
+  const bool ok = b < nbins && f < nf;
+  AT s = 0, v = 0;
+  if (ok) {
+    const PT* pv = part + (size_t)nseg * F;
+    const uint32_t q1 = bin_ptr[b + 1];
+    uint32_t q = bin_ptr[b] + j;
+    // Eight partials of this lane in flight at a time, added in order.
+    for (; q + 28 < q1; q += 32) {
+      uint32_t g[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) g[k] = bin_seg[q + 4 * k];
+      PT x[8], y[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        x[k] = part[(size_t)g[k] * F + f];
+        if constexpr (VAR) y[k] = pv[(size_t)g[k] * F + f];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        s = s + static_cast<AT>(x[k]);
+        if constexpr (VAR) v = v + static_cast<AT>(y[k]);
+      }
+    }
+    for (; q < q1; q += 4) {
+      const size_t g = (size_t)bin_seg[q] * F + f;
+      s = s + static_cast<AT>(part[g]);
+      if constexpr (VAR) v = v + static_cast<AT>(pv[g]);
+    }
+  }
+  s = s + __shfl_xor(s, 16);
+  s = s + __shfl_xor(s, 32);
   if constexpr (VAR) {
-    double vv = d == 0.0 ? 0.0 : v / (d * d);
-    S[(uint64_t)f * nbins + b] = sqrt(fmax(vv, 0.0));
+    v = v + __shfl_xor(v, 16);
+    v = v + __shfl_xor(v, 32);
+  }
+  if (ok && j == 0) {
+    const double d = den[b];
+    I[(uint64_t)f * nbins + b] = d == 0.0 ? 0.0 : (double)s / d;
+    if constexpr (VAR) {
+      const double vv = d == 0.0 ? 0.0 : (double)v / (d * d);
+      S[(uint64_t)f * nbins + b] = sqrt(fmax(vv, 0.0));
+    }
   }
 }
 
 // ------------------------------------------------------------- dispatch
 
-constexpr size_t kMaxLds = 160 * 1024;
-
-template <int F, int MODE, class P, bool VAR>
-void launch_tile(const TiledTable& t, const void* frames, uint32_t nf, double* part,
-                 hipStream_t s) {
-  size_t lds = (size_t)F * kTilePixels * sizeof(P);
-  if (lds > kMaxLds) fail(DIO_ERR_INVALID, "frames per pass exceed the LDS of one CU");
-  static bool attr = false;
-  if (!attr) {
-    DIO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<F, MODE, P, VAR>),
+template <int F, int G, int MODE, bool VAR, bool PF>
+void launch_tile(const TiledTable& t, const TileArgs& a, hipStream_t s) {
+  using PL = Plan<F, MODE>;
+  constexpr size_t lds = PL::kBytes;
+  static_assert(lds <= 160 * 1024, "LDS plan exceeds one CU");
+  // The attribute is per device; set it once on each.
+  static std::atomic<uint64_t> done{0};
+  int dev = 0;
+  DIO_HIP(hipGetDevice(&dev));
+  const uint64_t bit = 1ull << (dev & 63);
+  if (!(done.load() & bit)) {
+    DIO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<F, G, MODE, VAR, PF>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
+    done.fetch_or(bit);
   }
-  TileArgs a{t.lidx.p, t.w32.p, t.w64.p, t.seg_start.p, t.tile_seg.p, t.nseg, t.ntx,
-             t.dim1, t.dim2, (uint64_t)t.dim1 * t.dim2, nf, part};
-  k_tile<F, MODE, P, VAR><<<t.ntiles, kThreads, lds, s>>>(a, static_cast<const P*>(frames));
+  const uint32_t grid = std::min<uint32_t>(t.ntiles, t.cus * PL::kWgs);
+  k_tile<F, G, MODE, VAR, PF><<<grid, kThreads, lds, s>>>(a);
 }
 
-template <int F, int MODE, class P>
-void launch_tile_v(const TiledTable& t, const void* fr, uint32_t nf, bool var, double* part,
-                   hipStream_t s) {
-  if (var) launch_tile<F, MODE, P, true>(t, fr, nf, part, s);
-  else launch_tile<F, MODE, P, false>(t, fr, nf, part, s);
+template <int F, int G, int MODE>
+void launch_v(const TiledTable& t, const TileArgs& a, bool var, hipStream_t s) {
+  const bool pf = a.dtype == DIO_U32 || a.dtype == DIO_I32 || a.dtype == DIO_F32T;
+  if (var && pf) launch_tile<F, G, MODE, true, true>(t, a, s);
+  else if (var) launch_tile<F, G, MODE, true, false>(t, a, s);
+  else if (pf) launch_tile<F, G, MODE, false, true>(t, a, s);
+  else launch_tile<F, G, MODE, false, false>(t, a, s);
 }
 
-template <int F, int MODE>
-void launch_tile_p(const TiledTable& t, const void* fr, int dtype, uint32_t nf, bool var,
-                   double* part, hipStream_t s) {
-  switch (dtype) {
-    case DIO_U16: launch_tile_v<F, MODE, uint16_t>(t, fr, nf, var, part, s); break;
-    case DIO_U32: launch_tile_v<F, MODE, uint32_t>(t, fr, nf, var, part, s); break;
-    case DIO_I32: launch_tile_v<F, MODE, int32_t>(t, fr, nf, var, part, s); break;
-    case DIO_F32T: launch_tile_v<F, MODE, float>(t, fr, nf, var, part, s); break;
-    case DIO_F64T:
-      if constexpr (F <= 16) launch_tile_v<F, MODE, double>(t, fr, nf, var, part, s);
-      else fail(DIO_ERR_INVALID, "float64 frames allow at most 16 frames per pass");
-      break;
-    default: fail(DIO_ERR_INVALID, "unknown frame dtype");
-  }
-}
-
-template <int F>
-void launch_tile_m(const TiledTable& t, const void* fr, int dtype, uint32_t nf, bool var,
-                   int precision, double* part, hipStream_t s) {
+template <int F, int G>
+void launch_m(const TiledTable& t, const TileArgs& a, bool var, int precision, hipStream_t s) {
   switch (precision) {
-    case DIO_F32: launch_tile_p<F, 0>(t, fr, dtype, nf, var, part, s); break;
-    case DIO_F32_COMPENSATED: launch_tile_p<F, 1>(t, fr, dtype, nf, var, part, s); break;
-    case DIO_F64: launch_tile_p<F, 2>(t, fr, dtype, nf, var, part, s); break;
-    case DIO_F32_WIDE: launch_tile_p<F, 3>(t, fr, dtype, nf, var, part, s); break;
+    case DIO_F32: launch_v<F, G, 0>(t, a, var, s); break;
+    case DIO_F32_COMPENSATED: launch_v<F, G, 1>(t, a, var, s); break;
+    case DIO_F64:
+      if constexpr (F <= 8) launch_v<F, G, 2>(t, a, var, s);
+      else fail(DIO_ERR_INVALID, "float64 precision allows at most 8 frames per pass");
+      break;
+    case DIO_F32_WIDE: launch_v<F, G, 3>(t, a, var, s); break;
     default: fail(DIO_ERR_INVALID, "unknown precision");
   }
 }
 
-void launch_pass(const TiledTable& t, uint32_t F, const void* fr, int dtype, uint32_t nf,
-                 bool var, int precision, double* part, hipStream_t s) {
-  switch (F) {
-    case 1: launch_tile_m<1>(t, fr, dtype, nf, var, precision, part, s); break;
-    case 2: launch_tile_m<2>(t, fr, dtype, nf, var, precision, part, s); break;
-    case 4: launch_tile_m<4>(t, fr, dtype, nf, var, precision, part, s); break;
-    case 8: launch_tile_m<8>(t, fr, dtype, nf, var, precision, part, s); break;
-    case 16: launch_tile_m<16>(t, fr, dtype, nf, var, precision, part, s); break;
-    case 32: launch_tile_m<32>(t, fr, dtype, nf, var, precision, part, s); break;
-    default: fail(DIO_ERR_INVALID, "frames per pass must be 1, 2, 4, 8, 16 or 32");
+void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, int dtype,
+                 uint32_t nf, bool var, int precision, void* part, hipStream_t s) {
+  const uint32_t al8 = (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % 8 == 0) ? 1u : 0u;
+  TileArgs a{fr,     dtype,   t.lidx.p, t.w32.p, t.w64.p, t.seg_group.p,
+             t.tile_seg.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
+             (uint64_t)t.dim1 * t.dim2, nf, al8, part};
+  if (precision == DIO_F64 && !t.w64.p) fail(DIO_ERR_INVALID, "plan has no float64 weights");
+  switch (F * 8 + G) {
+    case 1 * 8 + 1: launch_m<1, 1>(t, a, var, precision, s); break;
+    case 2 * 8 + 2: launch_m<2, 2>(t, a, var, precision, s); break;
+    case 4 * 8 + 4: launch_m<4, 4>(t, a, var, precision, s); break;
+    case 8 * 8 + 2: launch_m<8, 2>(t, a, var, precision, s); break;
+    case 8 * 8 + 4: launch_m<8, 4>(t, a, var, precision, s); break;
+    case 16 * 8 + 2: launch_m<16, 2>(t, a, var, precision, s); break;
+    case 16 * 8 + 4: launch_m<16, 4>(t, a, var, precision, s); break;
+    default: fail(DIO_ERR_INVALID, "unsupported (frames per pass, frames per lane) pair");
   }
   DIO_HIP(hipGetLastError());
+}
+
+template <bool PLAIN, bool VAR, class PT>
+void launch_final(const TiledTable& t, const void* part, uint32_t F, uint32_t logF, uint32_t nf,
+                  double* I, double* S, hipStream_t s) {
+  const uint64_t waves = (t.nbins + (16u >> logF) - 1) / (16u >> logF);
+  const unsigned blocks = blocks_for(waves * 64, 256);
+  k_tfinal<PLAIN, VAR, PT><<<blocks, 256, 0, s>>>(static_cast<const PT*>(part), t.nseg, F, logF,
+                                                  t.bin_ptr.p, t.bin_seg.p, t.den, t.nbins, nf,
+                                                  I, S);
 }
 
 size_t dtype_bytes(int dtype) {
@@ -386,10 +782,26 @@ size_t dtype_bytes(int dtype) {
   }
 }
 
+uint32_t log2u(uint32_t v) {
+  uint32_t l = 0;
+  while ((1u << l) < v) l++;
+  return l;
+}
+
 }  // namespace
 
-uint32_t max_frames_for(int dtype) { return dtype == DIO_F64T ? 16 : kMaxFrames; }
+uint32_t max_frames_for(int precision) { return precision == DIO_F64 ? 8 : kMaxFrames; }
 
+bool valid_tile(int precision, uint32_t F, uint32_t G) {
+  if (F > max_frames_for(precision)) return false;
+  switch (F * 8 + G) {
+    case 1 * 8 + 1: case 2 * 8 + 2: case 4 * 8 + 4: case 8 * 8 + 2: case 8 * 8 + 4:
+    case 16 * 8 + 2: case 16 * 8 + 4:
+      return true;
+    default:
+      return false;
+  }
+}
 void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, uint64_t nnz,
                  const uint32_t* row_ptr, const uint32_t* col, const float* w32,
                  const double* w64, bool keep_w64, hipStream_t s) {
@@ -401,32 +813,36 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
   t.ntx = (dim2 + kTileCols - 1) / kTileCols;
   t.nty = (dim1 + kTileRows - 1) / kTileRows;
   t.ntiles = t.ntx * t.nty;
-  t.lidx.alloc(std::max<uint64_t>(nnz, 1));
-  t.w32.alloc(std::max<uint64_t>(nnz, 1));
-  if (keep_w64) t.w64.alloc(std::max<uint64_t>(nnz, 1));
   t.tile_seg.alloc(t.ntiles + 1);
   t.bin_ptr.alloc(nbins + 1);
   if (nnz == 0) {
     t.nseg = 0;
-    t.seg_start.alloc(1);
+    t.ngroups = 0;
+    t.lidx.alloc(kGroup);
+    t.w32.alloc(kGroup);
+    if (keep_w64) t.w64.alloc(kGroup);
+    t.seg_group.alloc(1);
     t.bin_seg.alloc(1);
-    DIO_HIP(hipMemsetAsync(t.seg_start.p, 0, 4, s));
+    DIO_HIP(hipMemsetAsync(t.seg_group.p, 0, 4, s));
     DIO_HIP(hipMemsetAsync(t.tile_seg.p, 0, (t.ntiles + 1) * 4, s));
     DIO_HIP(hipMemsetAsync(t.bin_ptr.p, 0, (nbins + 1) * 4, s));
     DIO_HIP(hipStreamSynchronize(s));
     return;
   }
+  // 1. Entries keyed by (tile, bin, pixel): stable sort of the CSR order by tile.
   DevBuf<uint32_t> tile0(nnz), bin0(nnz), order0(nnz), order(nnz), tkey(nnz), tile1(nnz),
       bin1(nnz);
-  DevBuf<uint16_t> lidx0(nnz);
+  DevBuf<uint16_t> lidx0(nnz), lidx1(nnz);
+  DevBuf<float> w32_1(nnz);
+  DevBuf<double> w64_1(keep_w64 ? nnz : 0);
   k_entry_keys<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(row_ptr, nbins, col, nnz, dim2,
                                                              t.ntx, tile0.p, bin0.p, lidx0.p);
   k_iota32<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(order0.p, (uint32_t)nnz);
-  // Stable by tile: entries keep their (bin, pixel) order inside a tile.
   sort_pairs(tile0.p, tkey.p, order0.p, order.p, nnz, bits_for(t.ntiles), s, tmp);
   k_tile_gather<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(
       order.p, tile0.p, bin0.p, lidx0.p, w32, keep_w64 ? w64 : nullptr, nnz, tile1.p, bin1.p,
-      t.lidx.p, t.w32.p, keep_w64 ? t.w64.p : nullptr);
+      lidx1.p, w32_1.p, keep_w64 ? w64_1.p : nullptr);
+  // 2. Segments = runs of equal (tile, bin).
   DevBuf<uint32_t> flag(nnz + 1), segid(nnz + 1);
   k_seg_flags<<<blocks_for(nnz + 1, kThreads), kThreads, 0, s>>>(tile1.p, bin1.p, nnz, flag.p);
   scan_u32(flag.p, segid.p, nnz + 1, s, tmp);
@@ -434,66 +850,89 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
   DIO_HIP(hipMemcpyAsync(&nseg, segid.p + nnz, 4, hipMemcpyDeviceToHost, s));
   DIO_HIP(hipStreamSynchronize(s));
   t.nseg = nseg;
-  t.seg_start.alloc(nseg + 1);
-  DevBuf<uint32_t> seg_tile(nseg), seg_bin(nseg);
+  DevBuf<uint32_t> seg_first(nseg + 1), seg_tile(nseg), seg_bin(nseg);
   k_seg_fill<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(flag.p, segid.p, tile1.p, bin1.p,
-                                                           nnz, t.seg_start.p, seg_tile.p,
+                                                           nnz, seg_first.p, seg_tile.p,
                                                            seg_bin.p);
-  k_set1<<<1, 1, 0, s>>>(t.seg_start.p + nseg, (uint32_t)nnz);
-  // Tile -> first segment.
+  k_set1<<<1, 1, 0, s>>>(seg_first.p + nseg, (uint32_t)nnz);
+  // 3. Inside each tile, longest segment first.
+  DevBuf<uint64_t> skey(nseg), skey2(nseg);
+  DevBuf<uint32_t> sid(nseg), perm(nseg), inv(nseg), ngr(nseg + 1), tile_n(nseg), bin_n(nseg);
+  k_seg_keys<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(seg_first.p, seg_tile.p, nseg,
+                                                            skey.p);
+  k_iota32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(sid.p, nseg);
+  sort_pairs(skey.p, skey2.p, sid.p, perm.p, nseg, 32 + bits_for(t.ntiles), s, tmp);
+  k_seg_perm<<<blocks_for(nseg + 1, kThreads), kThreads, 0, s>>>(
+      perm.p, seg_first.p, seg_tile.p, seg_bin.p, nseg, inv.p, ngr.p, tile_n.p, bin_n.p);
+  t.seg_group.alloc(nseg + 1);
+  scan_u32(ngr.p, t.seg_group.p, nseg + 1, s, tmp);
+  uint32_t ngroups = 0;
+  DIO_HIP(hipMemcpyAsync(&ngroups, t.seg_group.p + nseg, 4, hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipStreamSynchronize(s));
+  t.ngroups = ngroups;
+  // 4. Entries into their padded groups.
+  uint64_t nent = (uint64_t)ngroups * kGroup;
+  t.lidx.alloc(nent);
+  t.w32.alloc(nent);
+  if (keep_w64) t.w64.alloc(nent);
+  k_pad_fill<<<blocks_for(nent, kThreads), kThreads, 0, s>>>(nent, t.lidx.p, t.w32.p,
+                                                            keep_w64 ? t.w64.p : nullptr);
+  k_place<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(
+      flag.p, segid.p, seg_first.p, inv.p, t.seg_group.p, nnz, lidx1.p, w32_1.p,
+      keep_w64 ? w64_1.p : nullptr, t.lidx.p, t.w32.p, keep_w64 ? t.w64.p : nullptr);
+  // 5. Tile -> first segment.
   DevBuf<uint32_t> hist(std::max(t.ntiles, nbins) + 1);
   DIO_HIP(hipMemsetAsync(hist.p, 0, (t.ntiles + 1) * 4, s));
-  k_count_u32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(seg_tile.p, nseg, hist.p);
+  k_count_u32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(tile_n.p, nseg, hist.p);
   scan_u32(hist.p, t.tile_seg.p, t.ntiles + 1, s, tmp);
-  // Bin -> segments in tile order (stable sort of segment ids by bin).
-  DevBuf<uint32_t> sid(nseg), bkey(nseg);
+  // 6. Bin -> segments in tile order (stable sort of the tile-major ids by bin).
+  DevBuf<uint32_t> bkey(nseg);
   t.bin_seg.alloc(nseg);
-  k_iota32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(sid.p, nseg);
-  sort_pairs(seg_bin.p, bkey.p, sid.p, t.bin_seg.p, nseg, bits_for(nbins), s, tmp);
+  sort_pairs(bin_n.p, bkey.p, sid.p, t.bin_seg.p, nseg, bits_for(nbins), s, tmp);
   DIO_HIP(hipMemsetAsync(hist.p, 0, (nbins + 1) * 4, s));
-  k_count_u32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(seg_bin.p, nseg, hist.p);
+  k_count_u32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(bin_n.p, nseg, hist.p);
   scan_u32(hist.p, t.bin_ptr.p, nbins + 1, s, tmp);
   DIO_HIP(hipGetLastError());
   DIO_HIP(hipStreamSynchronize(s));
 }
 
 void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
-                     uint32_t nf, bool var, int precision, hipStream_t s) {
-  uint32_t F = 1;
-  while (F < nf) F <<= 1;
+                     uint32_t nf, bool var, int precision, uint32_t F, uint32_t G,
+                     hipStream_t s) {
+  if (!valid_tile(precision, F, G) || nf == 0 || nf > F)
+    fail(DIO_ERR_INVALID, "invalid frames per pass / frames per lane");
+  (void)dtype_bytes(dtype);
   sc.ensure(t, F, var);
-  if (t.nseg) launch_pass(t, F, frames, dtype, nf, var, precision, sc.part.p, s);
+  if (t.nseg) launch_pass(t, F, G, frames, dtype, nf, var, precision, sc.part.p, s);
 }
 
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
-                  uint64_t n_frames, bool var, int precision, uint32_t F, double* intensity,
-                  double* sigma, hipStream_t s) {
-  if (F == 0 || F > max_frames_for(dtype) || (F & (F - 1)))
-    fail(DIO_ERR_INVALID, "frames per pass must be a power of two within the LDS limit");
-  size_t esz = dtype_bytes(dtype);
+                  uint64_t n_frames, bool var, int precision, uint32_t F, uint32_t G,
+                  double* intensity, double* sigma, hipStream_t s) {
+  if (!valid_tile(precision, F, G))
+    fail(DIO_ERR_INVALID, "invalid frames per pass / frames per lane");
+  const size_t esz = dtype_bytes(dtype);
   sc.ensure(t, F, var);
   const char* fb = static_cast<const char*>(frames);
-  uint64_t npix = (uint64_t)t.dim1 * t.dim2;
+  const uint64_t npix = (uint64_t)t.dim1 * t.dim2;
+  const uint32_t logF = log2u(F);
+  const bool plain = precision == DIO_F32, f64 = precision == DIO_F64;
   for (uint64_t f0 = 0; f0 < n_frames; f0 += F) {
-    uint32_t nf = (uint32_t)std::min<uint64_t>(F, n_frames - f0);
-    if (t.nseg) launch_pass(t, F, fb + f0 * npix * esz, dtype, nf, var, precision, sc.part.p, s);
-    uint64_t work = (uint64_t)t.nbins * nf;
-    unsigned blocks = blocks_for(work, 256);
+    const uint32_t nf = (uint32_t)std::min<uint64_t>(F, n_frames - f0);
+    if (t.nseg)
+      launch_pass(t, F, G, fb + f0 * npix * esz, dtype, nf, var, precision, sc.part.p, s);
     double* I = intensity + f0 * t.nbins;
     double* S = var ? sigma + f0 * t.nbins : nullptr;
-    bool plain = precision == DIO_F32;
-    if (plain && var)
-      k_tfinal<true, true><<<blocks, 256, 0, s>>>(sc.part.p, t.nseg, F, t.bin_ptr.p, t.bin_seg.p,
-                                                  t.den, t.nbins, nf, I, S);
-    else if (plain)
-      k_tfinal<true, false><<<blocks, 256, 0, s>>>(sc.part.p, t.nseg, F, t.bin_ptr.p,
-                                                   t.bin_seg.p, t.den, t.nbins, nf, I, S);
-    else if (var)
-      k_tfinal<false, true><<<blocks, 256, 0, s>>>(sc.part.p, t.nseg, F, t.bin_ptr.p,
-                                                   t.bin_seg.p, t.den, t.nbins, nf, I, S);
-    else
-      k_tfinal<false, false><<<blocks, 256, 0, s>>>(sc.part.p, t.nseg, F, t.bin_ptr.p,
-                                                    t.bin_seg.p, t.den, t.nbins, nf, I, S);
+    if (f64) {
+      if (var) launch_final<false, true, double>(t, sc.part.p, F, logF, nf, I, S, s);
+      else launch_final<false, false, double>(t, sc.part.p, F, logF, nf, I, S, s);
+    } else if (plain) {
+      if (var) launch_final<true, true, float>(t, sc.part.p, F, logF, nf, I, S, s);
+      else launch_final<true, false, float>(t, sc.part.p, F, logF, nf, I, S, s);
+    } else {
+      if (var) launch_final<false, true, float>(t, sc.part.p, F, logF, nf, I, S, s);
+      else launch_final<false, false, float>(t, sc.part.p, F, logF, nf, I, S, s);
+    }
     DIO_HIP(hipGetLastError());
   }
 }

@@ -4,30 +4,36 @@
 
 namespace dio {
 
-// The execution table re-grouped by detector tile: tile = kTileRows x 64
-// pixels; inside a tile, entries are grouped by bin (a segment per
-// (tile, bin)), pixels ascending as in the CSR rows. A frame tile is read from
-// HBM once, coalesced, into LDS, and every entry of the table tile is applied
-// to it; segment partials are added per bin in tile order.
+// The execution table re-grouped by detector tile: tile = kTileRows x
+// kTileCols pixels; inside a tile, entries are grouped by bin (a segment per
+// (tile, bin)), pixels ascending as in the CSR rows, each segment padded to a
+// whole number of 4-entry groups with zero-weight entries that point at an
+// all-zero LDS slot. A frame tile is read from HBM once, coalesced, into LDS,
+// and every entry of the table tile is applied to it; segment partials are
+// added per bin in tile order.
 constexpr uint32_t kTileCols = 64;
 constexpr uint32_t kTileRows = 16;
 constexpr uint32_t kTilePixels = kTileCols * kTileRows;
-constexpr uint32_t kMaxFrames = 32;
+constexpr uint32_t kPadSlot = kTilePixels;          // LDS index of the zero pixel
+constexpr uint32_t kLdsStride = kTilePixels + 4;    // per-frame LDS row (16-byte aligned)
+constexpr uint32_t kGroup = 4;                      // entries per table group
+constexpr uint32_t kMaxFrames = 16;
 
 struct TiledTable {
   uint32_t dim1 = 0, dim2 = 0, nbins = 0, ntx = 0, nty = 0, ntiles = 0;
+  uint32_t cus = 256;          // compute units of the device (persistent grid size)
   uint64_t nnz = 0;
-  uint32_t nseg = 0;
-  DevBuf<uint16_t> lidx;      // nnz: pixel index inside its tile
-  DevBuf<float> w32;          // nnz
-  DevBuf<double> w64;         // nnz (F64 precision only)
-  DevBuf<uint32_t> seg_start; // nseg + 1: first entry of each segment
-  DevBuf<uint32_t> tile_seg;  // ntiles + 1: first segment of each tile
-  DevBuf<uint32_t> bin_seg;   // nseg: segments sorted by (bin, tile)
-  DevBuf<uint32_t> bin_ptr;   // nbins + 1
+  uint32_t nseg = 0, ngroups = 0;
+  DevBuf<uint16_t> lidx;       // 4 * ngroups: pixel index inside its tile (kPadSlot = padding)
+  DevBuf<float> w32;           // 4 * ngroups
+  DevBuf<double> w64;          // 4 * ngroups (F64 precision only)
+  DevBuf<uint32_t> seg_group;  // nseg + 1: first group of each segment
+  DevBuf<uint32_t> tile_seg;   // ntiles + 1: first segment of each tile
+  DevBuf<uint32_t> bin_seg;    // nseg: segments sorted by (bin, tile)
+  DevBuf<uint32_t> bin_ptr;    // nbins + 1
   const double* den = nullptr;
   size_t bytes() const {
-    return lidx.bytes() + w32.bytes() + w64.bytes() + seg_start.bytes() + tile_seg.bytes() +
+    return lidx.bytes() + w32.bytes() + w64.bytes() + seg_group.bytes() + tile_seg.bytes() +
            bin_seg.bytes() + bin_ptr.bytes();
   }
 };
@@ -37,6 +43,8 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
                  const uint32_t* row_ptr, const uint32_t* col, const float* w32,
                  const double* w64, bool keep_w64, hipStream_t s);
 
+// Segment partials of one pass: [1 + errors][nseg][frames per pass], float32
+// (float64 for the F64 precision).
 struct TileScratch {
   DevBuf<double> part;
   void ensure(const TiledTable& t, uint32_t frames, bool var) {
@@ -45,15 +53,18 @@ struct TileScratch {
   }
 };
 
-// Largest frame tile the kernel supports for this dtype (LDS capacity).
-uint32_t max_frames_for(int dtype);
+// Largest frames per pass the LDS holds for this precision.
+uint32_t max_frames_for(int precision);
+// Valid (frames per pass, frames per lane) pair for this precision.
+bool valid_tile(int precision, uint32_t frames, uint32_t lanes);
 
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t frames_per_pass,
-                  double* intensity, double* sigma, hipStream_t s);
+                  uint32_t frames_per_lane, double* intensity, double* sigma, hipStream_t s);
 
-// Only the tile kernel for one pass of nf frames (kernel timing).
+// Only the tile kernel for one pass of nf <= frames_per_pass frames (timing).
 void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
-                     uint32_t nf, bool var, int precision, hipStream_t s);
+                     uint32_t nf, bool var, int precision, uint32_t frames_per_pass,
+                     uint32_t frames_per_lane, hipStream_t s);
 
 }  // namespace dio

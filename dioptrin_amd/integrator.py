@@ -278,14 +278,16 @@ class GpuPlan:
             self._h, frames.ptr, N.DTYPES[dt], frames.shape[0], flags, out_i.ptr,
             None if out_s is None else out_s.ptr, None if out_d is None else out_d.ptr, stream))
 
-    def autotune(self, dtype="uint32", n_frames=64, errors=False) -> int:
-        t = C.c_uint32()
+    def autotune(self, dtype="uint32", n_frames=64, errors=False):
+        """Picks and stores (frames per pass, frames per lane); returns them."""
+        t, g = C.c_uint32(), C.c_uint32()
         N.check(N.lib().dio_plan_autotune(self._h, N.DTYPES[dtype], n_frames,
-                                          N.WITH_ERRORS if errors else 0, C.byref(t)))
-        return t.value
+                                          N.WITH_ERRORS if errors else 0, C.byref(t),
+                                          C.byref(g)))
+        return t.value, g.value
 
-    def set_tile(self, frames_per_tile: int):
-        N.check(N.lib().dio_plan_set_tile(self._h, frames_per_tile))
+    def set_tile(self, frames_per_tile: int, frames_per_lane: int = 0):
+        N.check(N.lib().dio_plan_set_tile(self._h, frames_per_tile, frames_per_lane))
 
     def time_gather(self, frames, errors=False, reps=20) -> float:
         """Average ms of one launch of the segment kernel over one tile."""

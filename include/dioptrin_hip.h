@@ -143,11 +143,17 @@ int dio_integrate_device(dio_plan *plan, const void *d_frames, int dtype, uint64
                          uint32_t flags, double *d_intensity, double *d_sigma, double *d_std,
                          void *stream);
 
-/* Frames per tile of the fused batch kernel, from a short device sweep
- * (autotuner). The choice is stored in the plan and used by later calls. */
+/* Launch shape of the fused batch kernel (no reference counterpart; the
+ * reference launches one frame per kernel, src/integration/cuda/device.rs:
+ * 1460-1584): frames per pass (1..32, powers of two; 1..16 for DIO_F64) and
+ * frames per lane (1, 2 or 4; the lanes sharing one table segment are
+ * frames_per_tile / frames_per_lane). dio_plan_autotune picks both from a
+ * short device sweep and stores them in the plan; frames_per_lane may be
+ * null. Results do not depend on either choice. */
 int dio_plan_autotune(dio_plan *plan, int dtype, uint64_t n_frames_hint, uint32_t flags,
-                      uint32_t *frames_per_tile);
-int dio_plan_set_tile(dio_plan *plan, uint32_t frames_per_tile);
+                      uint32_t *frames_per_tile, uint32_t *frames_per_lane);
+/* frames_per_lane = 0 picks min(4, frames_per_tile). */
+int dio_plan_set_tile(dio_plan *plan, uint32_t frames_per_tile, uint32_t frames_per_lane);
 
 /* Steady-state replay: captures one dio_integrate_device call into a
  * hipGraph and replays it. */

@@ -23,6 +23,16 @@
  *   gather          src/lut/prepared.rs:985-1121
  *   azimuthal std   src/lut/azimuthal.rs:48-118
  * Each function's comment cites the lines it follows.
+ *
+ * Pinning (tests/test_oracle_golden.py): geometry by pyFAI vectors and closed
+ * forms; bin rules, pixel split, supersampling, 2D grids, gather and std by
+ * the reference's own known answers, brute force and invariants; pyFAI 1D by
+ * pyFAI golden weights. Partially pinned: the per-bin distribution of the
+ * pyFAI 2D split (totals and 1D agreement only). Parity unpinned: the
+ * ulp-level bits of the reference's builders on large geometries, which
+ * depend on Rust's platform libm and fold order; no reference test pins them
+ * below 1e-10 / 1e-6, so bit-exactness is claimed against this oracle with
+ * the host glibc.
  */
 #ifndef DIOPTRIN_ORACLE_H
 #define DIOPTRIN_ORACLE_H

@@ -143,16 +143,22 @@ def test_compensated_beats_plain_f32(pilatus):
     assert errs["f32_wide"] <= 1e-6 and errs["f32_compensated"] <= 1e-6
 
 
-@pytest.mark.parametrize("tile", [1, 4, 8, 32])
-def test_batch_equals_single_frame_bitwise(pilatus, tile):
+TILE_SHAPES = [(1, 1), (2, 2), (4, 4), (8, 2), (8, 4), (16, 2), (16, 4)]
+
+
+@pytest.mark.parametrize("precision", ["f32", "f32_compensated", "f32_wide", "f64"])
+@pytest.mark.parametrize("tile", TILE_SHAPES, ids=[f"F{f}G{g}" for f, g in TILE_SHAPES])
+def test_batch_equals_single_frame_bitwise(pilatus, tile, precision):
+    if precision == "f64" and tile[0] > 8:
+        pytest.skip("float64 passes hold at most 8 frames")
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
-                   polarization_factor=0.99, mask=d["mask"])
+                   polarization_factor=0.99, mask=d["mask"], precision=precision)
     rng = np.random.default_rng(5)
     frames = rng.poisson(200.0, size=(37,) + d["w"]["shape"]).astype(np.float32)
-    plan.set_tile(1)
+    plan.set_tile(1, 1)
     single = [plan.integrate_stack(frames[k:k + 1], errors=True) for k in range(frames.shape[0])]
-    plan.set_tile(tile)
+    plan.set_tile(*tile)
     I, S, _ = plan.integrate_stack(frames, errors=True)
     for k in range(frames.shape[0]):
         assert np.array_equal(I[k], single[k][0][0])
