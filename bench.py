@@ -61,18 +61,98 @@ def cpu_baseline(frames_dev, poni, w, seconds=12.0):
             "lut_build_ms": build_s * 1e3}
 
 
+SERVE_GEOMETRIES = ("pilatus1m_ps1d", "eiger4m_ss2d", "eiger9m_batch1d")
+
+
+def serve_bench(rates, n_requests, seed=7):
+    """Continuous-batching server over the serving mix: Poisson arrivals,
+    the three detector shapes, 1-64 device-resident uint32 frames per
+    request, 60 % 1D (1000-3000 bins) / 40 % 2D (500 x 360), pixel
+    splitting. Tables are built once (warm cache) before the timed load; the
+    same requests are also run through the plain per-frame path."""
+    from dioptrin_amd import GpuPlan, synth
+    from dioptrin_amd.device import synchronize
+    from dioptrin_amd.serve import LoadMix, Server, make_requests, run_load
+
+    geos = {}
+    for name in SERVE_GEOMETRIES:
+        w = synth.WORKLOADS[name]
+        geos[name] = (synth.workload_poni(name), w["shape"],
+                      synth.frames_device(name, 64, seed=11 + len(geos)))
+    synchronize()
+    mix = LoadMix(geometries=geos, frames_min=1, frames_max=64, p1d=0.6, bins_1d=(1000, 3000),
+                  grid_2d=(500, 360), method="pixel_split", precision="f32_wide")
+    out = {"requests_per_rate": n_requests, "geometries": list(SERVE_GEOMETRIES),
+           "frames_per_request": [1, 64], "mix": "60% 1D 1000-3000 bins / 40% 2D 500x360",
+           "frames": "device-resident uint32", "rates": []}
+    with Server(n_slots=4, plan_cache_entries=64, hbm_budget_bytes=32 << 30) as srv:
+        # Warm the plan cache: one 1-frame request per distinct plan.
+        all_reqs = [make_requests(mix, n_requests, seed + k) for k in range(len(rates))]
+        seen = {}
+        for reqs in all_reqs:
+            for r in reqs:
+                key = (id(r.poni), r.n_radial, r.n_azimuthal)
+                if key not in seen:
+                    seen[key] = r
+        t0 = time.perf_counter()
+        for r in seen.values():
+            srv.result(srv.submit(type(r)(**{**r.__dict__, "frames": r.frames[:1]})))
+        warm_s = time.perf_counter() - t0
+        out["plans"] = len(seen)
+        out["plan_build_s"] = round(warm_s, 2)
+        for rate, reqs in zip(rates, all_reqs):
+            base = srv.stats()
+            res = run_load(srv, reqs, rate=rate, seed=seed)
+            frames = res["frames_total"]
+            out["rates"].append({
+                "offered_req_s": rate,
+                "achieved_req_s": round(len(reqs) / res["wall_s"], 1),
+                "frames": frames,
+                "frames_per_s": round(res["frames_per_s_wall"], 1),
+                "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+                "max_ms": round(res["max_ms"], 3),
+                "peak_inflight": res["peak_inflight"],
+                "plan_builds_during_load": res["plan_builds"] - base["plan_builds"]})
+            log(f"[bench] serve {rate} req/s: {out['rates'][-1]}")
+    # The same requests of the first rate through the plain per-frame path:
+    # one plan per request shape (built beforehand), one call and one
+    # synchronisation per frame.
+    plans = {}
+    for r in seen.values():
+        plans[(id(r.poni), r.n_radial, r.n_azimuthal)] = GpuPlan(
+            r.poni, r.frames.shape[1:], n_radial=r.n_radial, n_azimuthal=r.n_azimuthal,
+            precision="f32_wide")
+    from dioptrin_amd.device import DeviceArray
+    outs = {}
+    for key, pl in plans.items():
+        pl.set_tile(1, 1)
+        outs[key] = DeviceArray((1, pl.nbins), np.float64)
+    synchronize()
+    nfr, t0 = 0, time.perf_counter()
+    for r in all_reqs[0]:
+        key = (id(r.poni), r.n_radial, r.n_azimuthal)
+        for k in range(r.frames.shape[0]):
+            plans[key].integrate_device(r.frames[k:k + 1], outs[key])
+            synchronize()
+            nfr += 1
+    out["per_frame_path_frames_per_s"] = round(nfr / (time.perf_counter() - t0), 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="f32_compensated")
+    ap.add_argument("--precision", default="f32_wide")
     ap.add_argument("--tile", default="", help="frames per pass[,frames per lane] "
                                                "(default: autotune)")
     ap.add_argument("--frames", type=int, default=256)
     ap.add_argument("--errors", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB 5 times "
                          "(reference for the memory-traffic counters) and exit")
@@ -184,6 +264,10 @@ def main():
         cpu = cpu_baseline(frames, poni, w)
         log(f"[bench] CPU baseline {cpu['value']:.1f} frames/s on {cpu['cores']} threads")
 
+    serve = None
+    if not args.no_serve:
+        serve = serve_bench([50.0, 200.0, 400.0], args.serve_requests)
+
     bytes_frame = algorithmic_bytes_per_frame(npix, info["nnz_masked"], bins, tile,
                                               outputs=1 + int(args.errors))
     line = {
@@ -218,7 +302,7 @@ def main():
                      "kernel_ms": round(seg_ms, 5), "bytes_per_launch": int(per_launch)},
         "cpu_baseline": None if cpu is None else {k: (round(v, 2) if isinstance(v, float) else v)
                                                   for k, v in cpu.items()},
-        "serve": None,
+        "serve": serve,
     }
     print(json.dumps(line), flush=True)
 

@@ -16,7 +16,7 @@ METHODS = {"counting": 0, "pixel_split": 1, "pyfai_full": 2, "supersampled": 3}
 UNITS = {"2th_rad": 0, "2th_deg": 1, "q_A^-1": 2, "q_nm^-1": 3}
 PRECISIONS = {"f32": 0, "f32_compensated": 1, "f64": 2, "f32_wide": 3}
 DTYPES = {"uint16": 0, "uint32": 1, "int32": 2, "float32": 3, "float64": 4}
-WITH_ERRORS, WITH_AZIMUTHAL_STD = 1, 2
+WITH_ERRORS, WITH_AZIMUTHAL_STD, DEVICE_FRAMES = 1, 2, 4
 
 # Exported symbols, in include/dioptrin_hip.h order.
 EXPORTS = [

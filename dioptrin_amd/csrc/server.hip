@@ -61,11 +61,21 @@ struct Slot {
   Entry* entry = nullptr;  // plan of the running request
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  // Gather partials in stream-ordered memory of this slot (hipMallocAsync /
+  // hipFreeAsync on the slot's stream: growing it never synchronises the
+  // device, so the other slots keep running).
   dio::TileScratch scratch;
+  double* part = nullptr;
+  size_t part_n = 0;
   double* pinned = nullptr;  // outputs staged for async device-to-host copies
   size_t pinned_n = 0;
   std::shared_ptr<Req> req;
 };
+
+// Output staging per slot, allocated with the server: 128 MiB holds 64 frames
+// of a 500 x 360 grid; larger requests grow it (page-locking is slow, so it
+// grows by doubling and is kept).
+constexpr size_t kMinPinned = (128u << 20) / sizeof(double);
 
 std::string plan_key(const Req& r) {
   std::string k(reinterpret_cast<const char*>(&r.poni), sizeof(r.poni));
@@ -75,6 +85,27 @@ std::string plan_key(const Req& r) {
   k.append(std::to_string(std::hash<std::string>()(
       std::string(reinterpret_cast<const char*>(r.mask.data()), r.mask.size()))));
   return k;
+}
+
+// Host copy of request outputs out of the pinned staging buffer: large copies
+// (2D grids of many frames reach tens of MB) are split over four threads.
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  constexpr size_t kSplit = 4u << 20;
+  constexpr int kParts = 4;
+  if (bytes < kSplit) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t part = (bytes / kParts + 63) / 64 * 64;
+  std::thread th[kParts - 1];
+  for (int i = 1; i < kParts; i++) {
+    const size_t off = std::min(bytes, part * i), n = std::min(bytes - off, part);
+    th[i - 1] = std::thread([=] {
+      std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, n);
+    });
+  }
+  std::memcpy(dst, src, std::min(bytes, part));
+  for (auto& t : th) t.join();
 }
 
 size_t dtype_size(int dtype) {
@@ -147,7 +178,7 @@ struct dio_server {
     int k = 0;
     for (int o = 0; o < 3; o++) {
       if (!r->out[o]) continue;
-      std::memcpy(r->out[o], s.pinned + k * nb, nb * sizeof(double));
+      parallel_copy(r->out[o], s.pinned + k * nb, nb * sizeof(double));
       k++;
     }
     r->t_done = Clock::now();
@@ -196,22 +227,41 @@ struct dio_server {
       uint64_t expect = (uint64_t)r->desc.n_radial * (r->desc.ndim == 2 ? r->desc.n_azimuthal : 1);
       if (p->gt.nbins != expect) dio::fail(DIO_ERR_INVALID, "plan bin count differs from the request's");
       r->nbins = p->gt.nbins;
-      if (s.pinned_n < nb * nout) {
+      if (s.pinned_n < nb * nout) {  // the slot is idle here: its stream has drained
+        const size_t n = std::max({nb * nout, 2 * s.pinned_n, kMinPinned});
         if (s.pinned) (void)hipHostFree(s.pinned);
         s.pinned = nullptr;
-        DIO_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.pinned), nb * nout * sizeof(double),
+        s.pinned_n = 0;
+        DIO_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.pinned), n * sizeof(double),
                               hipHostMallocDefault));
-        s.pinned_n = nb * nout;
+        s.pinned_n = n;
       }
-      DIO_HIP(hipMallocAsync(&r->dmem, ((fbytes + 255) / 256) * 256 + nb * nout * sizeof(double),
-                             s.stream));
+      const size_t need = dio::TileScratch::need(p->tt, p->tile, true);
+      if (s.part_n < need) {
+        const size_t n = std::max(need, 2 * s.part_n);
+        if (s.part) DIO_HIP(hipFreeAsync(s.part, s.stream));
+        s.part = nullptr;
+        s.part_n = 0;
+        DIO_HIP(hipMallocAsync(reinterpret_cast<void**>(&s.part), n * sizeof(double), s.stream));
+        s.part_n = n;
+      }
+      s.scratch.ext = s.part;
+      s.scratch.ext_n = s.part_n;
+      const bool on_device = r->flags & DIO_DEVICE_FRAMES;
+      const size_t fpad = on_device ? 0 : ((fbytes + 255) / 256) * 256;
+      DIO_HIP(hipMallocAsync(&r->dmem, fpad + nb * nout * sizeof(double), s.stream));
       char* base = static_cast<char*>(r->dmem);
-      double* dout = reinterpret_cast<double*>(base + ((fbytes + 255) / 256) * 256);
-      DIO_HIP(hipMemcpyAsync(base, r->frames, fbytes, hipMemcpyHostToDevice, s.stream));
+      double* dout = reinterpret_cast<double*>(base + fpad);
+      const void* frames = r->frames;
+      if (!on_device) {
+        DIO_HIP(hipMemcpyAsync(base, r->frames, fbytes, hipMemcpyHostToDevice, s.stream));
+        frames = base;
+      }
       double* I = dout;
       double* S = (r->flags & DIO_WITH_ERRORS) ? dout + nb : nullptr;
       double* D = (r->flags & DIO_WITH_AZIMUTHAL_STD) ? dout + nb * (S ? 2 : 1) : nullptr;
-      dio::run_device(p, s.scratch, base, r->dtype, r->n_frames, r->flags, I, S, D, s.stream);
+      dio::run_device(p, s.scratch, frames, r->dtype, r->n_frames,
+                      r->flags & (DIO_WITH_ERRORS | DIO_WITH_AZIMUTHAL_STD), I, S, D, s.stream);
       DIO_HIP(hipMemcpyAsync(s.pinned, dout, nb * nout * sizeof(double), hipMemcpyDeviceToHost,
                              s.stream));
       DIO_HIP(hipFreeAsync(r->dmem, s.stream));
@@ -277,7 +327,8 @@ struct dio_server {
         dio_plan* p = ent->plan;
         int nout = 1 + ((r->flags & DIO_WITH_ERRORS) ? 1 : 0) +
                    ((r->flags & DIO_WITH_AZIMUTHAL_STD) ? 1 : 0);
-        uint64_t foot = r->n_frames * p->gt.npix * dtype_size(r->dtype) +
+        const bool on_device = r->flags & DIO_DEVICE_FRAMES;
+        uint64_t foot = (on_device ? 0 : r->n_frames * p->gt.npix * dtype_size(r->dtype)) +
                         r->n_frames * (uint64_t)p->gt.nbins * 8 * nout;
         {
           std::lock_guard<std::mutex> lk(mu);
@@ -316,10 +367,26 @@ struct dio_server {
 
 extern "C" {
 
+namespace {
+// Frees the streams, events and staging buffers of every slot (the slots must
+// be idle).
+void free_slots(dio_server* s) {
+  for (auto& sl : s->slots) {
+    if (sl.part) (void)hipFreeAsync(sl.part, sl.stream);
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.pinned) (void)hipHostFree(sl.pinned);
+    sl = Slot{};
+  }
+}
+}  // namespace
+
 int dio_server_create(dio_ctx* ctx, const dio_server_config* cfg, dio_server** out) {
+  std::unique_ptr<dio_server> s;
   try {
     if (!ctx || !out) throw dio::Error(DIO_ERR_INVALID, "null argument");
-    auto s = std::make_unique<dio_server>();
+    s = std::make_unique<dio_server>();
     s->ctx = ctx;
     dio_server_config c{};
     c.n_slots = 4;
@@ -334,15 +401,20 @@ int dio_server_create(dio_ctx* ctx, const dio_server_config* cfg, dio_server** o
     for (auto& sl : s->slots) {
       DIO_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
       DIO_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+      DIO_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.pinned), kMinPinned * sizeof(double),
+                            hipHostMallocDefault));
+      sl.pinned_n = kMinPinned;
     }
     dio_server* raw = s.release();
     raw->worker = std::thread([raw] { raw->run(); });
     *out = raw;
     return DIO_OK;
   } catch (const dio::Error& e) {
+    if (s) free_slots(s.get());
     dio_set_last_error(e.what());
     return e.code;
   } catch (const std::exception& e) {
+    if (s) free_slots(s.get());
     dio_set_last_error(e.what());
     return DIO_ERR_DEVICE;
   }
@@ -465,12 +537,7 @@ int dio_server_destroy(dio_server* s) {
   s->cv.notify_all();
   if (s->worker.joinable()) s->worker.join();
   (void)hipSetDevice(s->ctx->device);
-  for (auto& sl : s->slots) {
-    (void)hipStreamSynchronize(sl.stream);
-    (void)hipEventDestroy(sl.done);
-    (void)hipStreamDestroy(sl.stream);
-    if (sl.pinned) (void)hipHostFree(sl.pinned);
-  }
+  free_slots(s);
   for (auto& e : s->lru) delete e.plan;
   delete s;
   return DIO_OK;

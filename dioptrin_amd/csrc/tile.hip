@@ -8,7 +8,7 @@
 //
 // k_tile: persistent workgroups of 256 threads, each walking detector tiles
 // of 16 x 64 pixels (tile b, b + grid, ...).
-//  1. Staging: every thread owns 4 consecutive pixels of one tile row (16
+//  1. Staging: every thread owns 2 consecutive pixels of one tile row (32
 //     lanes cover a 256-byte row run) for each of the pass's F frames. For
 //     32-bit frames the next tile's words and its table chunk (4-entry
 //     groups: 8-byte indices + 16-byte weights) are loaded into registers
@@ -34,14 +34,20 @@
 #include <type_traits>
 #include <vector>
 
-#include "tile.hpp"
+#include "tile_impl.hpp"
 
 namespace dio {
 
+using namespace tk;
+
 namespace {
 
-constexpr unsigned kThreads = 256;
-static_assert(kTileRows * (kTileCols / 4) == kThreads, "one 4-pixel run per thread per frame");
+constexpr unsigned kThreads = 256;      // layout and reduction kernels
+constexpr unsigned kTileThreads = 256;  // k_tile workgroup: 4 wavefronts
+constexpr unsigned kRunLanes = kTileCols / 2;             // lanes per 2-pixel row run
+constexpr unsigned kRowsPerPass = kTileThreads / kRunLanes;
+constexpr int kPasses = kTileRows / kRowsPerPass;         // staging loads per frame
+static_assert(kPasses >= 1 && kPasses * kRowsPerPass == kTileRows, "staging map covers the tile");
 
 // ---------------------------------------------------------------- layout
 
@@ -147,6 +153,17 @@ __global__ void k_place(const uint32_t* flag, const uint32_t* segid, const uint3
   if (w64) w64[dst] = w64_in[i];
 }
 
+__global__ void k_tile_meta(const uint32_t* tile_seg, const uint32_t* seg_group,
+                            uint32_t ntiles, uint32_t* meta) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t s0 = tile_seg[t], s1 = tile_seg[t + 1];
+  meta[4 * t] = s0;
+  meta[4 * t + 1] = s1;
+  meta[4 * t + 2] = seg_group[s0];
+  meta[4 * t + 3] = seg_group[s1];
+}
+
 __global__ void k_count_u32(const uint32_t* key, uint32_t n, uint32_t* hist) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) atomicAdd(&hist[key[i]], 1u);
@@ -189,98 +206,6 @@ void sort_pairs(const K* kin, K* kout, const V* vin, V* vout, uint64_t n, int bi
                                              0, bits, s));
 }
 
-// ---------------------------------------------------------------- accumulators
-
-// Float32 fused multiply-add (CudaPrecision::F32).
-struct TPlain {
-  float s;
-  __device__ void init() { s = 0.0f; }
-  __device__ void add(float w, double, float x) { s = fmaf(w, x, s); }
-  __device__ void add_var(float w, double, float x) { s = fmaf(w * w, fmaxf(x, 0.0f), s); }
-  __device__ double value() const { return (double)s; }
-};
-
-// Compensated float32 (CudaPrecision::F32Compensated): the running sum hi is
-// updated with Knuth's TwoSum; its rounding errors and the FMA residuals of
-// the products (exact: w*x = p + pe) are accumulated in lo (Ogita-Rump-Oishi
-// Sum2 with error-free products, error ~ u|s| + n^2 u^2 sum|p|). A non-finite
-// hi is returned as is.
-struct TComp {
-  float hi, lo;
-  __device__ void init() {
-    hi = 0.0f;
-    lo = 0.0f;
-  }
-  __device__ void two_sum(float p, float pe) {
-    float s = hi + p;
-    float bb = s - hi;
-    float err = (hi - (s - bb)) + (p - bb);
-    lo = lo + (err + pe);
-    hi = s;
-  }
-  __device__ void add(float w, double, float x) {
-    float p = w * x;
-    two_sum(p, fmaf(w, x, -p));
-  }
-  __device__ void add_var(float w, double, float x) {
-    x = fmaxf(x, 0.0f);
-    float w2 = w * w;
-    float w2l = fmaf(w, w, -w2);
-    float p = w2 * x;
-    float pe = fmaf(w2l, x, fmaf(w2, x, -p));
-    two_sum(p, pe);
-  }
-  __device__ double value() const {
-    if (!(fabsf(hi) <= FLT_MAX)) return (double)hi;
-    return (double)hi + (double)lo;
-  }
-};
-
-// Float32 weights and signal, float64 register accumulation: one full-rate
-// DFMA per term on CDNA.
-struct TWide {
-  double s;
-  __device__ void init() { s = 0.0; }
-  __device__ void add(float w, double, float x) { s = fma((double)w, (double)x, s); }
-  __device__ void add_var(float w, double, float x) {
-    double wd = (double)w;
-    s = fma(wd * wd, (double)fmaxf(x, 0.0f), s);
-  }
-  __device__ double value() const { return s; }
-};
-
-// Float64 weights, signal and accumulation (CudaPrecision::F64).
-struct TF64 {
-  double s;
-  __device__ void init() { s = 0.0; }
-  __device__ void add(float, double w, double x) { s = fma(w, x, s); }
-  __device__ void add_var(float, double w, double x) { s = fma(w * w, fmax(x, 0.0), s); }
-  __device__ double value() const { return s; }
-};
-
-template <int MODE> struct TAcc;
-// T: accumulator; L: frame value type in LDS; P: stored segment partial
-// (float32 partials add a relative error of at most 2^-24 per partial).
-template <> struct TAcc<0> { using T = TPlain; using L = float; using P = float; };
-template <> struct TAcc<1> { using T = TComp; using L = float; using P = float; };
-template <> struct TAcc<2> { using T = TF64; using L = double; using P = double; };
-template <> struct TAcc<3> { using T = TWide; using L = float; using P = float; };
-
-struct TileArgs {
-  const void* frames;
-  int dtype;
-  const uint16_t* lidx;
-  const float* w32;
-  const double* w64;
-  const uint32_t* seg_group;
-  const uint32_t* tile_seg;
-  uint32_t nseg, ntx, ntiles, dim1, dim2;
-  uint64_t npix;
-  uint32_t nf;
-  uint32_t aligned8;  // frame rows and blocks are 8-byte aligned (32-bit frames)
-  void* part;
-};
-
 // ---------------------------------------------------------------- LDS plan
 
 // One workgroup's LDS: frames [F][kLdsStride] of LT, then a table chunk of at
@@ -290,13 +215,13 @@ struct TileArgs {
 template <int F, int MODE> struct Plan {
   using LT = typename TAcc<MODE>::L;
   static constexpr uint32_t kWgs = F >= 16 ? 2 : F == 8 ? (MODE == 2 ? 2 : 3) : F == 4 ? (MODE == 2 ? 3 : 4) : 4;
-  static constexpr size_t kFrameBytes = (size_t)F * kLdsStride * sizeof(LT);
+  static constexpr size_t kFrameBytes = ((size_t)F * kLdsStride * sizeof(LT) + 15) / 16 * 16;
   static constexpr size_t kWBytes = MODE == 2 ? 32 : 16;
   static constexpr size_t kBudget = (160 * 1024 / kWgs) / 256 * 256;
   static constexpr size_t kRoom = (kBudget - kFrameBytes) / (8 + kWBytes);
   static constexpr uint32_t kCap = kRoom >= 1024 ? 1024 : (uint32_t)(kRoom / 32 * 32);
   static constexpr size_t kBytes = kFrameBytes + (size_t)kCap * (8 + kWBytes);
-  static constexpr int kKT = (kCap + kThreads - 1) / kThreads;  // table loads per thread
+  static constexpr int kKT = (kCap + kTileThreads - 1) / kTileThreads;  // table loads per thread
   static_assert(kBudget > kFrameBytes, "frames exceed the LDS budget");
   // A (tile, bin) segment holds at most one entry per tile pixel.
   static_assert(kCap >= kTilePixels / kGroup, "a whole segment must fit one table chunk");
@@ -307,23 +232,23 @@ template <int F, int MODE> struct Plan {
 template <class P, size_t A> struct alignas(A) Run2 { P v[2]; };
 template <class LT> struct alignas(2 * sizeof(LT)) Lds2 { LT v[2]; };
 
-// Staging map: thread tid owns pixel pairs (row h * 8 + tid / 32, columns
-// 2 * (tid % 32) + {0, 1}) of the tile for h = 0, 1, so 32 lanes read one
-// 256-byte row run and a wavefront two rows per load.
+// Staging map: thread tid owns pixel pairs (row h * kRowsPerPass + tid / 32,
+// columns 2 * (tid % 32) + {0, 1}) of the tile for h < kPasses, so 32 lanes
+// read one 256-byte row run and a wavefront two rows per load.
 struct TileGeom {
   uint32_t r0, cp;
-  uint32_t nvalid[2];  // valid pixels of the pair in rows r0 and r0 + 8
-  uint64_t off[2];
+  uint32_t nvalid[kPasses];  // valid pixels of the pair in each of its rows
+  uint64_t off[kPasses];
 };
 
 __device__ __forceinline__ TileGeom tile_geom(const TileArgs& a, uint32_t t) {
   TileGeom g;
-  g.r0 = threadIdx.x / 32;
-  g.cp = threadIdx.x % 32;
+  g.r0 = threadIdx.x / kRunLanes;
+  g.cp = threadIdx.x % kRunLanes;
   const uint32_t c = (t % a.ntx) * kTileCols + g.cp * 2;
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const uint32_t r = (t / a.ntx) * kTileRows + g.r0 + 8 * h;
+  for (int h = 0; h < kPasses; h++) {
+    const uint32_t r = (t / a.ntx) * kTileRows + g.r0 + kRowsPerPass * h;
     g.nvalid[h] = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
     g.off[h] = (uint64_t)r * a.dim2 + c;
   }
@@ -331,13 +256,13 @@ __device__ __forceinline__ TileGeom tile_geom(const TileArgs& a, uint32_t t) {
 }
 
 __device__ __forceinline__ uint32_t lds_pixel(uint32_t r0, uint32_t cp, int h) {
-  return (r0 + 8 * h) * kTileCols + cp * 2;
+  return (r0 + kRowsPerPass * h) * kTileCols + cp * 2;
 }
 
 // Raw 32-bit frame words of one tile held in registers (u32, i32, f32
 // frames), loaded one tile ahead of their use.
 template <int F> struct FrameRegs {
-  uint32_t v[F][2][2];
+  uint32_t v[F][kPasses][2];
 };
 
 // Pair loads are 8-byte loads when every pair starts 8-byte aligned (even row
@@ -349,7 +274,7 @@ __device__ __forceinline__ void frames_load32_as(const TileArgs& a, const TileGe
 #pragma unroll
   for (int f = 0; f < F; f++) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < kPasses; h++) {
       const uint32_t* p = src + f * a.npix + g.off[h];
       if ((uint32_t)f < a.nf && g.nvalid[h] == 2) {
         const Run2<uint32_t, A> r = *reinterpret_cast<const Run2<uint32_t, A>*>(p);
@@ -378,19 +303,20 @@ __device__ __forceinline__ LT word_to(uint32_t w) {
 
 template <int F, class LT>
 __device__ __forceinline__ void pad_slots(LT* fl) {
+  static_assert(kPadSlot + 2 <= kLdsStride, "zero slot and spare word inside the row");
   if (threadIdx.x < (unsigned)F) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) fl[threadIdx.x * kLdsStride + kPadSlot + i] = LT(0);
+    fl[threadIdx.x * kLdsStride + kPadSlot] = LT(0);
+    fl[threadIdx.x * kLdsStride + kPadSlot + 1] = LT(0);
   }
 }
 
 template <int F, class W, class LT>
 __device__ __forceinline__ void frames_store_as(const FrameRegs<F>& fr, LT* fl) {
-  const uint32_t r0 = threadIdx.x / 32, cp = threadIdx.x % 32;
+  const uint32_t r0 = threadIdx.x / kRunLanes, cp = threadIdx.x % kRunLanes;
 #pragma unroll
   for (int f = 0; f < F; f++) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < kPasses; h++) {
       Lds2<LT> o;
       o.v[0] = word_to<W, LT>(fr.v[f][h][0]);
       o.v[1] = word_to<W, LT>(fr.v[f][h][1]);
@@ -416,12 +342,12 @@ __device__ __forceinline__ void stage_direct(const TileArgs& a, uint32_t t, LT* 
   constexpr int B = F < 8 ? F : 8;
 #pragma unroll
   for (int k0 = 0; k0 < F; k0 += B) {
-    P v[B][2][2];
+    P v[B][kPasses][2];
 #pragma unroll
     for (int k = 0; k < B; k++) {
       const uint32_t f = k0 + k;
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
+      for (int h = 0; h < kPasses; h++) {
         const P* p = fr + f * a.npix + g.off[h];
 #pragma unroll
         for (int i = 0; i < 2; i++)
@@ -431,7 +357,7 @@ __device__ __forceinline__ void stage_direct(const TileArgs& a, uint32_t t, LT* 
 #pragma unroll
     for (int k = 0; k < B; k++) {
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
+      for (int h = 0; h < kPasses; h++) {
         Lds2<LT> o;
         o.v[0] = static_cast<LT>(v[k][h][0]);
         o.v[1] = static_cast<LT>(v[k][h][1]);
@@ -453,18 +379,18 @@ template <int KT, int MODE>
 __device__ __forceinline__ void table_load(const TileArgs& a, uint32_t gb, uint32_t ge,
                                            TableRegs<KT, MODE>& tr) {
   const uint2* lidx4 = reinterpret_cast<const uint2*>(a.lidx);
+  // Unconditional loads (indices past the chunk repeat its last group and
+  // are not stored): no divergent exec masks, no partially written registers.
 #pragma unroll
   for (int k = 0; k < KT; k++) {
-    const uint32_t g = gb + threadIdx.x + k * kThreads;
-    if (g < ge) {
-      tr.li[k] = lidx4[g];
-      if constexpr (MODE == 2) {
-        const double2* wd2 = reinterpret_cast<const double2*>(a.w64);
-        tr.wa[k] = wd2[2 * g];
-        tr.wb[k] = wd2[2 * g + 1];
-      } else {
-        tr.w[k] = reinterpret_cast<const float4*>(a.w32)[g];
-      }
+    const uint32_t g = min(gb + threadIdx.x + k * kTileThreads, ge - 1);
+    tr.li[k] = lidx4[g];
+    if constexpr (MODE == 2) {
+      const double2* wd2 = reinterpret_cast<const double2*>(a.w64);
+      tr.wa[k] = wd2[2 * g];
+      tr.wb[k] = wd2[2 * g + 1];
+    } else {
+      tr.w[k] = reinterpret_cast<const float4*>(a.w32)[g];
     }
   }
 }
@@ -474,7 +400,7 @@ __device__ __forceinline__ void table_store(const TableRegs<KT, MODE>& tr, uint3
                                             uint2* tl, unsigned char* twb) {
 #pragma unroll
   for (int k = 0; k < KT; k++) {
-    const uint32_t i = threadIdx.x + k * kThreads;
+    const uint32_t i = threadIdx.x + k * kTileThreads;
     if (i < n) {
       tl[i] = tr.li[k];
       if constexpr (MODE == 2) {
@@ -488,17 +414,22 @@ __device__ __forceinline__ void table_store(const TableRegs<KT, MODE>& tr, uint3
   }
 }
 
-// Loads tile tt's frames and first table chunk into registers; returns the
-// number of groups loaded (0 for a tile without entries).
-template <int F, int KT, int MODE, uint32_t C>
-__device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, FrameRegs<F>& fr,
-                                             TableRegs<KT, MODE>& tr) {
-  const uint32_t s0 = a.tile_seg[tt], s1 = a.tile_seg[tt + 1];
-  if (s0 == s1) return 0;
-  const uint32_t g0 = a.seg_group[s0];
-  const uint32_t n = min(a.seg_group[s1] - g0, C);
+// Loads tile tt's frames, first table chunk and the slot's first segment
+// bounds into registers (m = tile_meta[tt]); returns the groups loaded.
+template <int F, int KT, int MODE, uint32_t C, uint32_t SLOTS>
+__device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, uint4 m,
+                                             uint32_t slot, FrameRegs<F>& fr,
+                                             TableRegs<KT, MODE>& tr, SegRegs& sr) {
+  if (m.x == m.y) return 0;
+  const uint32_t n = min(m.w - m.z, C);
   frames_load32<F>(a, tt, fr);
-  table_load<KT, MODE>(a, g0, g0 + n, tr);
+  table_load<KT, MODE>(a, m.z, m.z + n, tr);
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const uint32_t s = min(m.x + seg_of_slot<SLOTS>(slot, r), m.y - 1);
+    sr.lo[r] = a.seg_group[s];
+    sr.hi[r] = a.seg_group[s + 1];
+  }
   return n;
 }
 
@@ -508,16 +439,15 @@ __device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, Fra
 // the next tile's frames and first table chunk are loaded into registers
 // while the current tile is computed.
 template <int F, int G, int MODE, bool VAR, bool PF>
-__global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs))) void k_tile(TileArgs a) {
-  using Acc = typename TAcc<MODE>::T;
+__global__ __launch_bounds__(kTileThreads)
+__attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) void k_tile(
+    TileArgs a) {
   using LT = typename TAcc<MODE>::L;
-  using PT = typename TAcc<MODE>::P;
   using PL = Plan<F, MODE>;
   constexpr uint32_t C = PL::kCap;
   constexpr int KT = PL::kKT;
   constexpr uint32_t L = F / G;
-  constexpr uint32_t SLOTS = kThreads / L;
+  constexpr uint32_t SLOTS = kTileThreads / L;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   LT* fl = reinterpret_cast<LT*>(lds_raw);
   uint2* tl = reinterpret_cast<uint2*>(lds_raw + PL::kFrameBytes);
@@ -527,27 +457,38 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs))) void k_tile(TileArgs a
   const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
   const LT* base = fl + q * G * kLdsStride;
 
+  const uint4* meta = reinterpret_cast<const uint4*>(a.tile_meta);
+  const uint32_t grid = gridDim.x;
+  uint32_t t = first_tile();
+  if (t >= a.ntiles) return;
+  // Tile metadata (s0, s1, g0, g1) is read two tiles ahead, frames and table
+  // one tile ahead.
+  uint4 m = meta[t];
+  uint4 mn = t + grid < a.ntiles ? meta[t + grid] : make_uint4(0u, 0u, 0u, 0u);
   FrameRegs<F> fr;
   TableRegs<KT, MODE> tr;
-  uint32_t t = blockIdx.x;
+  SegRegs sr;
   uint32_t pn = 0;  // groups of the prefetched chunk
-  if constexpr (pf) {
-    if (t < a.ntiles) pn = prefetch<F, KT, MODE, C>(a, t, fr, tr);
-  }
+  if constexpr (pf) pn = prefetch<F, KT, MODE, C, SLOTS>(a, t, m, slot, fr, tr, sr);
 
-  for (; t < a.ntiles; t += gridDim.x) {
-    const uint32_t s0 = a.tile_seg[t], s1 = a.tile_seg[t + 1];
+  for (; t < a.ntiles; t += grid) {
+    const uint32_t s0 = m.x, s1 = m.y, g0 = m.z, g1 = m.w;
+    const uint32_t tn = t + grid;
+    const uint4 mnn = tn + grid < a.ntiles ? meta[tn + grid] : make_uint4(0u, 0u, 0u, 0u);
     if (s0 == s1) {  // nothing of the table in this tile
       if constexpr (pf) {
-        if (t + gridDim.x < a.ntiles) pn = prefetch<F, KT, MODE, C>(a, t + gridDim.x, fr, tr);
+        if (tn < a.ntiles) pn = prefetch<F, KT, MODE, C, SLOTS>(a, tn, mn, slot, fr, tr, sr);
       }
+      m = mn;
+      mn = mnn;
       continue;
     }
-    const uint32_t g0 = a.seg_group[s0], g1 = a.seg_group[s1];
     __syncthreads();  // the previous tile's compute is done with the LDS
+    SegRegs cur;
     if constexpr (pf) {
       frames_store32<F, LT>(a, fr, fl);
       table_store<KT, MODE, C>(tr, pn, tl, twb);
+      cur = sr;
     } else {
       switch (a.dtype) {
         case DIO_U16: stage_direct<F, uint16_t, LT>(a, t, fl); break;
@@ -557,10 +498,16 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs))) void k_tile(TileArgs a
       const uint32_t n = min(g1 - g0, C);
       table_load<KT, MODE>(a, g0, g0 + n, td);
       table_store<KT, MODE, C>(td, n, tl, twb);
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        const uint32_t s = min(s0 + seg_of_slot<SLOTS>(slot, r), s1 - 1);
+        cur.lo[r] = a.seg_group[s];
+        cur.hi[r] = a.seg_group[s + 1];
+      }
     }
     __syncthreads();
     if constexpr (pf) {
-      if (t + gridDim.x < a.ntiles) pn = prefetch<F, KT, MODE, C>(a, t + gridDim.x, fr, tr);
+      if (tn < a.ntiles) pn = prefetch<F, KT, MODE, C, SLOTS>(a, tn, mn, slot, fr, tr, sr);
     }
 
     uint32_t sa = s0, gc = g0;
@@ -583,55 +530,30 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs))) void k_tile(TileArgs a
         table_store<KT, MODE, C>(td, n, tl, twb);
         __syncthreads();
       }
-      for (uint32_t s = sa + slot; s < sb; s += SLOTS) {
-        const uint32_t ga = a.seg_group[s] - gc, ge = a.seg_group[s + 1] - gc;
-        Acc acc[G], var[G];
-#pragma unroll
-        for (int j = 0; j < G; j++) {
-          acc[j].init();
-          if constexpr (VAR) var[j].init();
-        }
-        for (uint32_t g = ga; g < ge; g++) {
-          const uint2 lc = tl[g];
-          const uint32_t l[4] = {lc.x & 0xffffu, lc.x >> 16, lc.y & 0xffffu, lc.y >> 16};
-          float wf[4] = {0.f, 0.f, 0.f, 0.f};
-          double wd[4] = {0.0, 0.0, 0.0, 0.0};
-          if constexpr (MODE == 2) {
-            const double2* tw = reinterpret_cast<const double2*>(twb);
-            const double2 x0 = tw[2 * g], x1 = tw[2 * g + 1];
-            wd[0] = x0.x;
-            wd[1] = x0.y;
-            wd[2] = x1.x;
-            wd[3] = x1.y;
-          } else {
-            const float4 w4 = reinterpret_cast<const float4*>(twb)[g];
-            wf[0] = w4.x;
-            wf[1] = w4.y;
-            wf[2] = w4.z;
-            wf[3] = w4.w;
-          }
-#pragma unroll
-          for (int e = 0; e < 4; e++) {
-#pragma unroll
-            for (int j = 0; j < G; j++) {
-              const LT x = base[j * kLdsStride + l[e]];
-              acc[j].add(wf[e], wd[e], x);
-              if constexpr (VAR) var[j].add_var(wf[e], wd[e], x);
-            }
-          }
-        }
-        PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * F + q * G;
-#pragma unroll
-        for (int j = 0; j < G; j++) {
-          if (q * G + j < a.nf) {
-            out[j] = static_cast<PT>(acc[j].value());
-            if constexpr (VAR) out[(size_t)a.nseg * F + j] = static_cast<PT>(var[j].value());
-          }
-        }
+      const uint32_t rounds = (sb - sa + SLOTS - 1) / SLOTS;
+      uint32_t r0 = 0;
+      if (sa == s0) {
+        // Rounds 0 and 1 take their bounds from registers: no global load,
+        // hence no wait on the next tile's prefetch, in the common path.
+        const uint32_t s_0 = sa + seg_of_slot<SLOTS>(slot, 0);
+        if (s_0 < sb)
+          seg_apply<F, G, MODE, VAR>(a, base, tl, twb, cur.lo[0] - gc, cur.hi[0] - gc, s_0, q);
+        const uint32_t s_1 = sa + seg_of_slot<SLOTS>(slot, 1);
+        if (rounds > 1 && s_1 < sb)
+          seg_apply<F, G, MODE, VAR>(a, base, tl, twb, cur.lo[1] - gc, cur.hi[1] - gc, s_1, q);
+        r0 = 2;
+      }
+      for (uint32_t r = r0; r < rounds; r++) {
+        const uint32_t s = sa + seg_of_slot<SLOTS>(slot, r);
+        if (s < sb)
+          seg_apply<F, G, MODE, VAR>(a, base, tl, twb, a.seg_group[s] - gc,
+                                     a.seg_group[s + 1] - gc, s, q);
       }
       sa = sb;
-      gc = a.seg_group[sb];
+      if (sa < s1) gc = a.seg_group[sa];
     }
+    m = mn;
+    mn = mnn;
   }
 }
 
@@ -717,7 +639,7 @@ void launch_tile(const TiledTable& t, const TileArgs& a, hipStream_t s) {
     done.fetch_or(bit);
   }
   const uint32_t grid = std::min<uint32_t>(t.ntiles, t.cus * PL::kWgs);
-  k_tile<F, G, MODE, VAR, PF><<<grid, kThreads, lds, s>>>(a);
+  k_tile<F, G, MODE, VAR, PF><<<grid, kTileThreads, lds, s>>>(a);
 }
 
 template <int F, int G, int MODE>
@@ -747,7 +669,7 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
                  uint32_t nf, bool var, int precision, void* part, hipStream_t s) {
   const uint32_t al8 = (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % 8 == 0) ? 1u : 0u;
   TileArgs a{fr,     dtype,   t.lidx.p, t.w32.p, t.w64.p, t.seg_group.p,
-             t.tile_seg.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
+             t.tile_meta.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
              (uint64_t)t.dim1 * t.dim2, nf, al8, part};
   if (precision == DIO_F64 && !t.w64.p) fail(DIO_ERR_INVALID, "plan has no float64 weights");
   switch (F * 8 + G) {
@@ -823,6 +745,8 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
     if (keep_w64) t.w64.alloc(kGroup);
     t.seg_group.alloc(1);
     t.bin_seg.alloc(1);
+    t.tile_meta.alloc(4 * (size_t)t.ntiles);
+    DIO_HIP(hipMemsetAsync(t.tile_meta.p, 0, 16 * (size_t)t.ntiles, s));
     DIO_HIP(hipMemsetAsync(t.seg_group.p, 0, 4, s));
     DIO_HIP(hipMemsetAsync(t.tile_seg.p, 0, (t.ntiles + 1) * 4, s));
     DIO_HIP(hipMemsetAsync(t.bin_ptr.p, 0, (nbins + 1) * 4, s));
@@ -885,6 +809,9 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
   DIO_HIP(hipMemsetAsync(hist.p, 0, (t.ntiles + 1) * 4, s));
   k_count_u32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(tile_n.p, nseg, hist.p);
   scan_u32(hist.p, t.tile_seg.p, t.ntiles + 1, s, tmp);
+  t.tile_meta.alloc(4 * (size_t)t.ntiles);
+  k_tile_meta<<<blocks_for(t.ntiles, kThreads), kThreads, 0, s>>>(t.tile_seg.p, t.seg_group.p,
+                                                                 t.ntiles, t.tile_meta.p);
   // 6. Bin -> segments in tile order (stable sort of the tile-major ids by bin).
   DevBuf<uint32_t> bkey(nseg);
   t.bin_seg.alloc(nseg);
@@ -903,7 +830,7 @@ void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, i
     fail(DIO_ERR_INVALID, "invalid frames per pass / frames per lane");
   (void)dtype_bytes(dtype);
   sc.ensure(t, F, var);
-  if (t.nseg) launch_pass(t, F, G, frames, dtype, nf, var, precision, sc.part.p, s);
+  if (t.nseg) launch_pass(t, F, G, frames, dtype, nf, var, precision, sc.ptr(), s);
 }
 
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
@@ -920,18 +847,18 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   for (uint64_t f0 = 0; f0 < n_frames; f0 += F) {
     const uint32_t nf = (uint32_t)std::min<uint64_t>(F, n_frames - f0);
     if (t.nseg)
-      launch_pass(t, F, G, fb + f0 * npix * esz, dtype, nf, var, precision, sc.part.p, s);
+      launch_pass(t, F, G, fb + f0 * npix * esz, dtype, nf, var, precision, sc.ptr(), s);
     double* I = intensity + f0 * t.nbins;
     double* S = var ? sigma + f0 * t.nbins : nullptr;
     if (f64) {
-      if (var) launch_final<false, true, double>(t, sc.part.p, F, logF, nf, I, S, s);
-      else launch_final<false, false, double>(t, sc.part.p, F, logF, nf, I, S, s);
+      if (var) launch_final<false, true, double>(t, sc.ptr(), F, logF, nf, I, S, s);
+      else launch_final<false, false, double>(t, sc.ptr(), F, logF, nf, I, S, s);
     } else if (plain) {
-      if (var) launch_final<true, true, float>(t, sc.part.p, F, logF, nf, I, S, s);
-      else launch_final<true, false, float>(t, sc.part.p, F, logF, nf, I, S, s);
+      if (var) launch_final<true, true, float>(t, sc.ptr(), F, logF, nf, I, S, s);
+      else launch_final<true, false, float>(t, sc.ptr(), F, logF, nf, I, S, s);
     } else {
-      if (var) launch_final<false, true, float>(t, sc.part.p, F, logF, nf, I, S, s);
-      else launch_final<false, false, float>(t, sc.part.p, F, logF, nf, I, S, s);
+      if (var) launch_final<false, true, float>(t, sc.ptr(), F, logF, nf, I, S, s);
+      else launch_final<false, false, float>(t, sc.ptr(), F, logF, nf, I, S, s);
     }
     DIO_HIP(hipGetLastError());
   }

@@ -15,7 +15,10 @@ constexpr uint32_t kTileCols = 64;
 constexpr uint32_t kTileRows = 16;
 constexpr uint32_t kTilePixels = kTileCols * kTileRows;
 constexpr uint32_t kPadSlot = kTilePixels;          // LDS index of the zero pixel
-constexpr uint32_t kLdsStride = kTilePixels + 4;    // per-frame LDS row (16-byte aligned)
+// Per-frame LDS row: the pixels, the zero slot and one spare word. The stride
+// is 2 mod 32 words, so the frames a segment's lanes read at one pixel fall in
+// distinct banks.
+constexpr uint32_t kLdsStride = kTilePixels + 2;
 constexpr uint32_t kGroup = 4;                      // entries per table group
 constexpr uint32_t kMaxFrames = 16;
 
@@ -29,12 +32,13 @@ struct TiledTable {
   DevBuf<double> w64;          // 4 * ngroups (F64 precision only)
   DevBuf<uint32_t> seg_group;  // nseg + 1: first group of each segment
   DevBuf<uint32_t> tile_seg;   // ntiles + 1: first segment of each tile
+  DevBuf<uint32_t> tile_meta;  // 4 * ntiles: first, end segment; first, end group
   DevBuf<uint32_t> bin_seg;    // nseg: segments sorted by (bin, tile)
   DevBuf<uint32_t> bin_ptr;    // nbins + 1
   const double* den = nullptr;
   size_t bytes() const {
     return lidx.bytes() + w32.bytes() + w64.bytes() + seg_group.bytes() + tile_seg.bytes() +
-           bin_seg.bytes() + bin_ptr.bytes();
+           tile_meta.bytes() + bin_seg.bytes() + bin_ptr.bytes();
   }
 };
 
@@ -45,11 +49,23 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
 
 // Segment partials of one pass: [1 + errors][nseg][frames per pass], float32
 // (float64 for the F64 precision).
+// Either owned (grown on demand) or external memory of a stated size (the
+// serving loop's stream-ordered slot buffers).
 struct TileScratch {
   DevBuf<double> part;
+  double* ext = nullptr;
+  size_t ext_n = 0;
+  static size_t need(const TiledTable& t, uint32_t frames, bool var) {
+    return (size_t)t.nseg * frames * (var ? 2 : 1);
+  }
+  double* ptr() const { return ext ? ext : part.p; }
   void ensure(const TiledTable& t, uint32_t frames, bool var) {
-    size_t need = (size_t)t.nseg * frames * (var ? 2 : 1);
-    if (part.n < need) part.alloc(need);
+    const size_t n = need(t, frames, var);
+    if (ext) {
+      if (ext_n < n) fail(DIO_ERR_DEVICE, "external tile scratch is too small");
+      return;
+    }
+    if (part.n < n) part.alloc(n);
   }
 };
 

@@ -1,0 +1,182 @@
+// Shared device code of the tiled gather kernels (internal to the library).
+#pragma once
+#include <cfloat>
+#include <cstdint>
+
+#include "tile.hpp"
+
+namespace dio {
+namespace tk {
+
+constexpr unsigned kXcds = 8;  // accelerator complex dies of an MI355X
+
+// ---------------------------------------------------------------- accumulators
+
+// Float32 fused multiply-add (CudaPrecision::F32).
+struct TPlain {
+  float s;
+  __device__ void init() { s = 0.0f; }
+  __device__ void add(float w, double, float x) { s = fmaf(w, x, s); }
+  __device__ void add_var(float w, double, float x) { s = fmaf(w * w, fmaxf(x, 0.0f), s); }
+  __device__ double value() const { return (double)s; }
+};
+
+// Compensated float32 (CudaPrecision::F32Compensated): the running sum hi is
+// updated with Knuth's TwoSum; its rounding errors and the FMA residuals of
+// the products (exact: w*x = p + pe) are accumulated in lo (Ogita-Rump-Oishi
+// Sum2 with error-free products, error ~ u|s| + n^2 u^2 sum|p|). A non-finite
+// hi is returned as is.
+struct TComp {
+  float hi, lo;
+  __device__ void init() {
+    hi = 0.0f;
+    lo = 0.0f;
+  }
+  __device__ void two_sum(float p, float pe) {
+    float s = hi + p;
+    float bb = s - hi;
+    float err = (hi - (s - bb)) + (p - bb);
+    lo = lo + (err + pe);
+    hi = s;
+  }
+  __device__ void add(float w, double, float x) {
+    float p = w * x;
+    two_sum(p, fmaf(w, x, -p));
+  }
+  __device__ void add_var(float w, double, float x) {
+    x = fmaxf(x, 0.0f);
+    float w2 = w * w;
+    float w2l = fmaf(w, w, -w2);
+    float p = w2 * x;
+    float pe = fmaf(w2l, x, fmaf(w2, x, -p));
+    two_sum(p, pe);
+  }
+  __device__ double value() const {
+    if (!(fabsf(hi) <= FLT_MAX)) return (double)hi;
+    return (double)hi + (double)lo;
+  }
+};
+
+// Float32 weights and signal, float64 register accumulation: one full-rate
+// DFMA per term on CDNA.
+struct TWide {
+  double s;
+  __device__ void init() { s = 0.0; }
+  __device__ void add(float w, double, float x) { s = fma((double)w, (double)x, s); }
+  __device__ void add_var(float w, double, float x) {
+    double wd = (double)w;
+    s = fma(wd * wd, (double)fmaxf(x, 0.0f), s);
+  }
+  __device__ double value() const { return s; }
+};
+
+// Float64 weights, signal and accumulation (CudaPrecision::F64).
+struct TF64 {
+  double s;
+  __device__ void init() { s = 0.0; }
+  __device__ void add(float, double w, double x) { s = fma(w, x, s); }
+  __device__ void add_var(float, double w, double x) { s = fma(w * w, fmax(x, 0.0), s); }
+  __device__ double value() const { return s; }
+};
+
+template <int MODE> struct TAcc;
+// T: accumulator; L: frame value type in LDS; P: stored segment partial
+// (float32 partials add a relative error of at most 2^-24 per partial).
+template <> struct TAcc<0> { using T = TPlain; using L = float; using P = float; };
+template <> struct TAcc<1> { using T = TComp; using L = float; using P = float; };
+template <> struct TAcc<2> { using T = TF64; using L = double; using P = double; };
+template <> struct TAcc<3> { using T = TWide; using L = float; using P = float; };
+
+struct TileArgs {
+  const void* frames;
+  int dtype;
+  const uint16_t* lidx;
+  const float* w32;
+  const double* w64;
+  const uint32_t* seg_group;
+  const uint32_t* tile_meta;  // per tile: first/end segment, first/end group
+  uint32_t nseg, ntx, ntiles, dim1, dim2;
+  uint64_t npix;
+  uint32_t nf;
+  uint32_t aligned8;  // frame rows and blocks are 8-byte aligned (32-bit frames)
+  void* part;
+};
+
+// Segment (relative to the chunk's first) a slot takes in round r. Segments
+// are stored longest first; round 1 walks the slots backwards, so a slot with
+// a long first segment gets a short second one.
+template <uint32_t SLOTS>
+__device__ __forceinline__ uint32_t seg_of_slot(uint32_t slot, uint32_t r) {
+  return r == 1 ? 2 * SLOTS - 1 - slot : slot + r * SLOTS;
+}
+
+// Group bounds of the segments a slot takes in the first two rounds of a tile.
+struct SegRegs {
+  uint32_t lo[2], hi[2];
+};
+
+// Applies segment s (table groups [ga, gb) of the LDS chunk) to this lane's G
+// frames and writes its partials.
+template <int F, int G, int MODE, bool VAR, class LT>
+__device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, const uint2* tl,
+                                          const unsigned char* twb, uint32_t ga, uint32_t gb,
+                                          uint32_t s, uint32_t q) {
+  using Acc = typename TAcc<MODE>::T;
+  using PT = typename TAcc<MODE>::P;
+  Acc acc[G], var[G];
+#pragma unroll
+  for (int j = 0; j < G; j++) {
+    acc[j].init();
+    if constexpr (VAR) var[j].init();
+  }
+  for (uint32_t g = ga; g < gb; g++) {
+    const uint2 lc = tl[g];
+    const uint32_t l[4] = {lc.x & 0xffffu, lc.x >> 16, lc.y & 0xffffu, lc.y >> 16};
+    float wf[4] = {0.f, 0.f, 0.f, 0.f};
+    double wd[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (MODE == 2) {
+      const double2* tw = reinterpret_cast<const double2*>(twb);
+      const double2 x0 = tw[2 * g], x1 = tw[2 * g + 1];
+      wd[0] = x0.x;
+      wd[1] = x0.y;
+      wd[2] = x1.x;
+      wd[3] = x1.y;
+    } else {
+      const float4 w4 = reinterpret_cast<const float4*>(twb)[g];
+      wf[0] = w4.x;
+      wf[1] = w4.y;
+      wf[2] = w4.z;
+      wf[3] = w4.w;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+#pragma unroll
+      for (int j = 0; j < G; j++) {
+        const LT x = base[j * kLdsStride + l[e]];
+        acc[j].add(wf[e], wd[e], x);
+        if constexpr (VAR) var[j].add_var(wf[e], wd[e], x);
+      }
+    }
+  }
+  PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * F + q * G;
+#pragma unroll
+  for (int j = 0; j < G; j++) {
+    if (q * G + j < a.nf) {
+      out[j] = static_cast<PT>(acc[j].value());
+      if constexpr (VAR) out[(size_t)a.nseg * F + j] = static_cast<PT>(var[j].value());
+    }
+  }
+}
+
+// Persistent-grid tile order: workgroups are dealt to the XCDs round-robin
+// by id, so the workgroups of one XCD get consecutive tiles of a round; the
+// row runs neighbouring tiles share (a 256-byte run rarely starts on a
+// 128-byte line) are then fetched into the same L2 at about the same time.
+__device__ __forceinline__ uint32_t first_tile() {
+  const uint32_t grid = gridDim.x;
+  if (grid % kXcds) return blockIdx.x;
+  return (blockIdx.x % kXcds) * (grid / kXcds) + blockIdx.x / kXcds;
+}
+
+}  // namespace tk
+}  // namespace dio

@@ -21,13 +21,15 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import _native as N
+from .device import DeviceArray
 from .integrator import _NP_DTYPES, _Context, plan_args
 
 
 @dataclass
 class Request:
-    """One integration request: host frames (n, dim1, dim2) or (dim1, dim2)
-    and the plan settings (GpuPlan arguments)."""
+    """One integration request: frames (n, dim1, dim2) or (dim1, dim2) as a
+    host array, or a DeviceArray already in HBM, and the plan settings
+    (GpuPlan arguments)."""
     poni: dict
     frames: np.ndarray
     n_radial: int = 1000
@@ -70,14 +72,22 @@ class Server:
         self._pending: Dict[int, _Pending] = {}
 
     def submit(self, req: Request) -> int:
-        fr = np.asarray(req.frames)
-        if fr.ndim == 2:
-            fr = fr[None]
-        if fr.ndim != 3 or fr.shape[0] == 0:
-            raise ValueError(f"frames must be (n, dim1, dim2), got {fr.shape}")
+        on_device = isinstance(req.frames, DeviceArray)
+        if on_device:
+            fr = req.frames
+            if len(fr.shape) != 3 or fr.shape[0] == 0:
+                raise ValueError(f"device frames must be (n, dim1, dim2), got {fr.shape}")
+            fptr = fr.ptr
+        else:
+            fr = np.asarray(req.frames)
+            if fr.ndim == 2:
+                fr = fr[None]
+            if fr.ndim != 3 or fr.shape[0] == 0:
+                raise ValueError(f"frames must be (n, dim1, dim2), got {fr.shape}")
+            fr = np.ascontiguousarray(fr)
+            fptr = fr.ctypes.data
         if fr.dtype not in _NP_DTYPES:
             raise ValueError(f"unsupported frame dtype {fr.dtype}")
-        fr = np.ascontiguousarray(fr)
         poni, desc, mask = plan_args(
             req.poni, fr.shape[1:], req.method, req.n_ss, req.unit, req.n_radial,
             req.n_azimuthal, req.radial_range, req.azimuthal_range, req.polarization_factor,
@@ -88,11 +98,12 @@ class Server:
         out_s = np.empty((n, nbins)) if req.errors else None
         out_d = np.empty((n, nbins)) if req.azimuthal_std else None
         flags = (N.WITH_ERRORS if req.errors else 0) | \
-            (N.WITH_AZIMUTHAL_STD if req.azimuthal_std else 0)
+            (N.WITH_AZIMUTHAL_STD if req.azimuthal_std else 0) | \
+            (N.DEVICE_FRAMES if on_device else 0)
         t = C.c_uint64()
         dt = N.DTYPES[_NP_DTYPES[fr.dtype]]
         N.check(N.lib().dio_server_submit(
-            self._h, C.byref(poni), C.byref(desc), fr.ctypes.data, dt, n, flags, out_i.ctypes.data, None if out_s is None else out_s.ctypes.data,
+            self._h, C.byref(poni), C.byref(desc), fptr, dt, n, flags, out_i.ctypes.data, None if out_s is None else out_s.ctypes.data,
             None if out_d is None else out_d.ctypes.data, C.byref(t)))
         p = _Pending()
         p.req, p.frames, p.mask, p.out = req, fr, mask, (out_i, out_s, out_d)
@@ -215,7 +226,13 @@ def run_load(server: Server, reqs: List[Request], rate: float, seed: int = 0,
         out[k] = server.result(tk)
     wall = time.perf_counter() - t0
     st = server.stats()
-    frames = sum(1 if np.ndim(r.frames) == 2 else int(np.shape(r.frames)[0]) for r in reqs)
+    # Latency percentiles of this load (the server's own statistics cover its
+    # whole lifetime).
+    lat = np.sort([r.latency_ms for r in out])
+    pct = lambda q: float(lat[min(len(lat) - 1, int(np.ceil(q * len(lat))) - 1)])
+    st.update(p50_ms=pct(0.50), p99_ms=pct(0.99), max_ms=float(lat[-1]),
+              mean_ms=float(lat.mean()))
+    frames = sum(1 if len(r.frames.shape) == 2 else int(r.frames.shape[0]) for r in reqs)
     res = dict(st, offered_rate=rate, requests=len(reqs), wall_s=wall, frames_total=frames,
                frames_per_s_wall=frames / wall)
     if keep:

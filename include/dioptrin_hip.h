@@ -37,8 +37,9 @@ enum { DIO_UNIT_TTH_RAD = 0, DIO_UNIT_TTH_DEG = 1, DIO_UNIT_Q_A = 2, DIO_UNIT_Q_
 enum { DIO_F32 = 0, DIO_F32_COMPENSATED = 1, DIO_F64 = 2, DIO_F32_WIDE = 3 };
 /* Frame element types (CudaPixel, src/integration/cuda.rs:354-367). */
 enum { DIO_U16 = 0, DIO_U32 = 1, DIO_I32 = 2, DIO_F32T = 3, DIO_F64T = 4 };
-/* Output selection flags. */
-enum { DIO_WITH_ERRORS = 1, DIO_WITH_AZIMUTHAL_STD = 2 };
+/* Output selection flags; DIO_DEVICE_FRAMES (dio_server_submit only): the
+ * frames pointer is device memory of the server's context. */
+enum { DIO_WITH_ERRORS = 1, DIO_WITH_AZIMUTHAL_STD = 2, DIO_DEVICE_FRAMES = 4 };
 
 typedef struct dio_ctx dio_ctx;
 typedef struct dio_plan dio_plan;
@@ -209,9 +210,9 @@ typedef struct {
 } dio_server_stats;
 
 int dio_server_create(dio_ctx *ctx, const dio_server_config *cfg, dio_server **out);
-/* Queues one request; frames (host, n_frames x dim1 x dim2 of dtype) and the
- * outputs (n_frames x nbins doubles each) are borrowed until poll reports it
- * done. The plan is identified by (poni, desc, mask contents) like
+/* Queues one request; frames (n_frames x dim1 x dim2 of dtype, in host
+ * memory, or device memory with DIO_DEVICE_FRAMES) and the host outputs
+ * (n_frames x nbins doubles each) are borrowed until poll reports it done. The plan is identified by (poni, desc, mask contents) like
  * dio_plan_create's arguments (Integrator::integrate1d / integrate2d,
  * src/integration/methods.rs:94-100, 461). */
 int dio_server_submit(dio_server *srv, const dio_poni *poni, const dio_plan_desc *desc,

@@ -167,3 +167,20 @@ def test_poisson_load_reports_latency():
     assert res["frames"] == res["frames_total"]
     for r, resp in zip(reqs[:6], res["responses"][:6]):
         assert same(resp.intensity, alone(r)[0])
+
+
+def test_device_frames_equal_host_frames():
+    from dioptrin_amd.device import DeviceArray
+    reqs = mixed_requests(seed=6, n=9)
+    with Server(n_slots=3) as srv:
+        host = [srv.result(srv.submit(r)) for r in reqs]
+        dev_reqs = []
+        for r in reqs:
+            d = dict(r.__dict__)
+            d["frames"] = DeviceArray.from_numpy(np.ascontiguousarray(r.frames))
+            dev_reqs.append(Request(**d))
+        dev = [srv.result(t) for t in [srv.submit(r) for r in dev_reqs]]
+    for a, b in zip(host, dev):
+        assert same(a.intensity, b.intensity)
+        assert same(a.errors, b.errors)
+        assert same(a.azimuthal_std, b.azimuthal_std)
