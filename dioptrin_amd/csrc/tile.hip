@@ -238,7 +238,7 @@ template <class LT> struct alignas(2 * sizeof(LT)) Lds2 { LT v[2]; };
 struct TileGeom {
   uint32_t r0, cp;
   uint32_t nvalid[kPasses];  // valid pixels of the pair in each of its rows
-  uint64_t off[kPasses];
+  uint32_t off[kPasses];     // pixel offset in the frame (frames hold < 2^29 pixels)
 };
 
 __device__ __forceinline__ TileGeom tile_geom(const TileArgs& a, uint32_t t) {
@@ -250,7 +250,7 @@ __device__ __forceinline__ TileGeom tile_geom(const TileArgs& a, uint32_t t) {
   for (int h = 0; h < kPasses; h++) {
     const uint32_t r = (t / a.ntx) * kTileRows + g.r0 + kRowsPerPass * h;
     g.nvalid[h] = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
-    g.off[h] = (uint64_t)r * a.dim2 + c;
+    g.off[h] = r * a.dim2 + c;
   }
   return g;
 }
@@ -266,23 +266,30 @@ template <int F> struct FrameRegs {
 };
 
 // Pair loads are 8-byte loads when every pair starts 8-byte aligned (even row
-// length and an 8-byte aligned frame block), else two 4-byte loads.
+// length and an 8-byte aligned frame block), else two 4-byte loads. The loads
+// are unconditional, so all of a tile's loads issue back to back: lanes
+// outside the detector, or of frames past the pass, read a valid address
+// (pixel 0 of the pass's frame 0 or last frame) and their words are never used
+// (no table entry points at such a pixel; partials of frames >= nf are not
+// written).
 template <int F, size_t A>
 __device__ __forceinline__ void frames_load32_as(const TileArgs& a, const TileGeom& g,
                                                  FrameRegs<F>& fr) {
   const uint32_t* src = static_cast<const uint32_t*>(a.frames);
+  const uint32_t flast = a.nf - 1;
 #pragma unroll
   for (int f = 0; f < F; f++) {
+    const uint32_t* pf = src + min((uint32_t)f, flast) * a.npix;
 #pragma unroll
     for (int h = 0; h < kPasses; h++) {
-      const uint32_t* p = src + f * a.npix + g.off[h];
-      if ((uint32_t)f < a.nf && g.nvalid[h] == 2) {
-        const Run2<uint32_t, A> r = *reinterpret_cast<const Run2<uint32_t, A>*>(p);
+      if constexpr (A == 8) {
+        const uint32_t off = g.nvalid[h] ? g.off[h] : 0u;
+        const Run2<uint32_t, 8> r = *reinterpret_cast<const Run2<uint32_t, 8>*>(pf + off);
         fr.v[f][h][0] = r.v[0];
         fr.v[f][h][1] = r.v[1];
       } else {
-        fr.v[f][h][0] = ((uint32_t)f < a.nf && g.nvalid[h] > 0) ? p[0] : 0u;
-        fr.v[f][h][1] = 0u;
+        fr.v[f][h][0] = pf[g.nvalid[h] > 0 ? g.off[h] : 0u];
+        fr.v[f][h][1] = pf[g.nvalid[h] > 1 ? g.off[h] + 1 : 0u];
       }
     }
   }
@@ -345,13 +352,12 @@ __device__ __forceinline__ void stage_direct(const TileArgs& a, uint32_t t, LT* 
     P v[B][kPasses][2];
 #pragma unroll
     for (int k = 0; k < B; k++) {
-      const uint32_t f = k0 + k;
+      // Unconditional loads from valid addresses (see frames_load32_as).
+      const P* pf = fr + min((uint32_t)(k0 + k), a.nf - 1) * a.npix;
 #pragma unroll
       for (int h = 0; h < kPasses; h++) {
-        const P* p = fr + f * a.npix + g.off[h];
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-          v[k][h][i] = (f < a.nf && (uint32_t)i < g.nvalid[h]) ? p[i] : P(0);
+        v[k][h][0] = pf[g.nvalid[h] > 0 ? g.off[h] : 0u];
+        v[k][h][1] = pf[g.nvalid[h] > 1 ? g.off[h] + 1 : 0u];
       }
     }
 #pragma unroll
