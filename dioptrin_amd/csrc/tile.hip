@@ -376,15 +376,15 @@ __device__ __forceinline__ void stage_direct(const TileArgs& a, uint32_t t, LT* 
 
 // Table groups [gb, ge) of one chunk held in registers.
 template <int KT, int MODE> struct TableRegs {
-  uint2 li[KT];
-  float4 w[KT];
-  double2 wa[MODE == 2 ? KT : 1], wb[MODE == 2 ? KT : 1];
+  u32x2 li[KT];
+  f32x4 w[KT];
+  f64x2 wa[MODE == 2 ? KT : 1], wb[MODE == 2 ? KT : 1];
 };
 
 template <int KT, int MODE>
 __device__ __forceinline__ void table_load(const TileArgs& a, uint32_t gb, uint32_t ge,
                                            TableRegs<KT, MODE>& tr) {
-  const uint2* lidx4 = reinterpret_cast<const uint2*>(a.lidx);
+  const u32x2* lidx4 = reinterpret_cast<const u32x2*>(a.lidx);
   // Unconditional loads (indices past the chunk repeat its last group and
   // are not stored): no divergent exec masks, no partially written registers.
 #pragma unroll
@@ -392,29 +392,29 @@ __device__ __forceinline__ void table_load(const TileArgs& a, uint32_t gb, uint3
     const uint32_t g = min(gb + threadIdx.x + k * kTileThreads, ge - 1);
     tr.li[k] = lidx4[g];
     if constexpr (MODE == 2) {
-      const double2* wd2 = reinterpret_cast<const double2*>(a.w64);
+      const f64x2* wd2 = reinterpret_cast<const f64x2*>(a.w64);
       tr.wa[k] = wd2[2 * g];
       tr.wb[k] = wd2[2 * g + 1];
     } else {
-      tr.w[k] = reinterpret_cast<const float4*>(a.w32)[g];
+      tr.w[k] = reinterpret_cast<const f32x4*>(a.w32)[g];
     }
   }
 }
 
 template <int KT, int MODE, uint32_t CAP>
 __device__ __forceinline__ void table_store(const TableRegs<KT, MODE>& tr, uint32_t n,
-                                            uint2* tl, unsigned char* twb) {
+                                            u32x2* tl, unsigned char* twb) {
 #pragma unroll
   for (int k = 0; k < KT; k++) {
     const uint32_t i = threadIdx.x + k * kTileThreads;
     if (i < n) {
       tl[i] = tr.li[k];
       if constexpr (MODE == 2) {
-        double2* tw = reinterpret_cast<double2*>(twb);
+        f64x2* tw = reinterpret_cast<f64x2*>(twb);
         tw[2 * i] = tr.wa[k];
         tw[2 * i + 1] = tr.wb[k];
       } else {
-        reinterpret_cast<float4*>(twb)[i] = tr.w[k];
+        reinterpret_cast<f32x4*>(twb)[i] = tr.w[k];
       }
     }
   }
@@ -441,9 +441,79 @@ __device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, uin
 
 // ---------------------------------------------------------------- tile kernel
 
-// Persistent: workgroup b processes tiles b, b + grid, ...; for 32-bit frames
-// the next tile's frames and first table chunk are loaded into registers
-// while the current tile is computed.
+// s_waitcnt immediate for vmcnt(0) (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their no-wait maxima).
+constexpr int kWaitVm0 = 0x0f70;
+
+// Computes one staged tile (m = its tile_meta): segments [s0, s1) in chunks
+// of at most C table groups; the first chunk is already in LDS, later chunks
+// of a large tile are loaded here. cur holds the group bounds of this slot's
+// first two segments.
+template <int F, int G, int MODE, bool VAR, uint32_t C, int KT, uint32_t SLOTS, class LT>
+__device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m, const SegRegs& cur,
+                                           uint32_t slot, uint32_t q, const LT* base, u32x2* tl,
+                                           unsigned char* twb) {
+  // Chunk bounds are workgroup-uniform: read them through the scalar cache so
+  // that waiting for them never waits on the vector loads in flight.
+  typedef const __attribute__((address_space(4))) uint32_t CWord;
+  CWord* sg = (CWord*)(a.seg_group);
+  const uint32_t s0 = m.x, s1 = m.y, g0 = m.z, g1 = m.w;
+  uint32_t sa = s0, gc = g0;
+  while (sa < s1) {
+    uint32_t sb = s1;
+    if (g1 - gc > C) {  // largest sb with seg_group[sb] - gc <= C
+      uint32_t lo = sa + 1, hi = s1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (sg[mid] - gc <= C) lo = mid;
+        else hi = mid - 1;
+      }
+      sb = lo;
+    }
+    if (sa != s0) {  // a later chunk of a large tile: load it now
+      __syncthreads();
+      TableRegs<KT, MODE> td;
+      const uint32_t n = sg[sb] - gc;
+      table_load<KT, MODE>(a, gc, gc + n, td);
+      table_store<KT, MODE, C>(td, n, tl, twb);
+      // Lanes that store nothing skip the store's wait; wait here on every
+      // lane so no load of this rare path is still pending where it rejoins
+      // the common path (the compiler would otherwise wait there, on the
+      // next tile's prefetch as well).
+      __builtin_amdgcn_s_waitcnt(kWaitVm0);
+      __syncthreads();
+    }
+    const uint32_t rounds = (sb - sa + SLOTS - 1) / SLOTS;
+    uint32_t r0 = 0;
+    if (sa == s0) {
+      // Rounds 0 and 1 take their bounds from registers: no global load,
+      // hence no wait on the next tile's prefetch, in the common path.
+      const uint32_t s_0 = sa + seg_of_slot<SLOTS>(slot, 0);
+      if (s_0 < sb)
+        seg_apply<F, G, MODE, VAR>(a, base, tl, twb, cur.lo[0] - gc, cur.hi[0] - gc, s_0, q);
+      const uint32_t s_1 = sa + seg_of_slot<SLOTS>(slot, 1);
+      if (rounds > 1 && s_1 < sb)
+        seg_apply<F, G, MODE, VAR>(a, base, tl, twb, cur.lo[1] - gc, cur.hi[1] - gc, s_1, q);
+      r0 = 2;
+    }
+    for (uint32_t r = r0; r < rounds; r++) {
+      const uint32_t s = sa + seg_of_slot<SLOTS>(slot, r);
+      if (s < sb)
+        seg_apply<F, G, MODE, VAR>(a, base, tl, twb, a.seg_group[s] - gc,
+                                   a.seg_group[s + 1] - gc, s, q);
+    }
+    sa = sb;
+    if (sa < s1) gc = sg[sa];
+  }
+}
+
+// Persistent: workgroup b processes tiles b, b + grid, ... For 32-bit frames
+// the loop is software-pipelined: each iteration stages the tile whose
+// frames and first table chunk are in registers, loads the next tile into
+// the same registers, then computes the staged tile while those loads are in
+// flight. There is one prefetch site, so the prefetched registers are not
+// duplicated across branches (which would push the kernel past its register
+// budget into spills that wait on the prefetch).
 template <int F, int G, int MODE, bool VAR, bool PF>
 __global__ __launch_bounds__(kTileThreads)
 __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) void k_tile(
@@ -456,110 +526,77 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
   constexpr uint32_t SLOTS = kTileThreads / L;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   LT* fl = reinterpret_cast<LT*>(lds_raw);
-  uint2* tl = reinterpret_cast<uint2*>(lds_raw + PL::kFrameBytes);
+  u32x2* tl = reinterpret_cast<u32x2*>(lds_raw + PL::kFrameBytes);
   unsigned char* twb = lds_raw + PL::kFrameBytes + (size_t)C * 8;
 
-  constexpr bool pf = PF;  // 32-bit frames: register prefetch of the next tile
   const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
   const LT* base = fl + q * G * kLdsStride;
 
-  const uint4* meta = reinterpret_cast<const uint4*>(a.tile_meta);
+  // Tile metadata is read through the scalar cache (constant address space,
+  // workgroup-uniform index): it lands in scalar registers, so the branches on
+  // it are uniform and waiting for it never waits on the vector loads.
+  typedef const __attribute__((address_space(4))) u32x4 CMeta;
+  CMeta* meta = (CMeta*)(a.tile_meta);
   const uint32_t grid = gridDim.x;
-  uint32_t t = first_tile();
-  if (t >= a.ntiles) return;
-  // Tile metadata (s0, s1, g0, g1) is read two tiles ahead, frames and table
-  // one tile ahead.
-  uint4 m = meta[t];
-  uint4 mn = t + grid < a.ntiles ? meta[t + grid] : make_uint4(0u, 0u, 0u, 0u);
-  FrameRegs<F> fr;
-  TableRegs<KT, MODE> tr;
-  SegRegs sr;
-  uint32_t pn = 0;  // groups of the prefetched chunk
-  if constexpr (pf) pn = prefetch<F, KT, MODE, C, SLOTS>(a, t, m, slot, fr, tr, sr);
+  const uint4 none = make_uint4(0u, 0u, 0u, 0u);
+  const uint32_t t0 = first_tile();
+  if (t0 >= a.ntiles) return;
+  // Metadata of tile i, or an empty tile past the end (a clamped load and a
+  // select of values: a select of addresses would leave the scalar path).
+  auto meta_at = [&](uint32_t i) {
+    const u32x4 v = meta[min(i, a.ntiles - 1)];
+    const bool ok = i < a.ntiles;
+    return make_uint4(ok ? v.x : 0u, ok ? v.y : 0u, v.z, v.w);
+  };
 
-  for (; t < a.ntiles; t += grid) {
-    const uint32_t s0 = m.x, s1 = m.y, g0 = m.z, g1 = m.w;
-    const uint32_t tn = t + grid;
-    const uint4 mnn = tn + grid < a.ntiles ? meta[tn + grid] : make_uint4(0u, 0u, 0u, 0u);
-    if (s0 == s1) {  // nothing of the table in this tile
-      if constexpr (pf) {
-        if (tn < a.ntiles) pn = prefetch<F, KT, MODE, C, SLOTS>(a, tn, mn, slot, fr, tr, sr);
+  if constexpr (PF) {
+    FrameRegs<F> fr;
+    TableRegs<KT, MODE> tr;
+    SegRegs sr = {};
+    uint32_t pn = 0;    // groups of the chunk in tr
+    uint4 m = none;     // tile held in registers (none before the first load)
+    uint4 mn = meta_at(t0);
+    // Tile metadata (s0, s1, g0, g1) is read one tile ahead of the prefetch.
+    for (uint32_t tn = t0;; tn += grid) {
+      const bool more = tn < a.ntiles;
+      const uint4 mnn = meta_at(tn + grid);
+      const bool work = m.x != m.y;
+      const SegRegs cur = sr;
+      if (work) {
+        __syncthreads();  // the previous tile's compute is done with the LDS
+        frames_store32<F, LT>(a, fr, fl);
+        table_store<KT, MODE, C>(tr, pn, tl, twb);
+        __syncthreads();
       }
+      if (more) pn = prefetch<F, KT, MODE, C, SLOTS>(a, tn, mn, slot, fr, tr, sr);
+      if (work) tile_apply<F, G, MODE, VAR, C, KT, SLOTS>(a, m, cur, slot, q, base, tl, twb);
+      if (!more) break;
       m = mn;
       mn = mnn;
-      continue;
     }
-    __syncthreads();  // the previous tile's compute is done with the LDS
-    SegRegs cur;
-    if constexpr (pf) {
-      frames_store32<F, LT>(a, fr, fl);
-      table_store<KT, MODE, C>(tr, pn, tl, twb);
-      cur = sr;
-    } else {
+  } else {
+    for (uint32_t t = t0; t < a.ntiles; t += grid) {
+      const uint4 m = meta_at(t);
+      if (m.x == m.y) continue;  // nothing of the table in this tile
+      __syncthreads();
       switch (a.dtype) {
         case DIO_U16: stage_direct<F, uint16_t, LT>(a, t, fl); break;
         default: stage_direct<F, double, LT>(a, t, fl); break;
       }
       TableRegs<KT, MODE> td;
-      const uint32_t n = min(g1 - g0, C);
-      table_load<KT, MODE>(a, g0, g0 + n, td);
+      const uint32_t n = min(m.w - m.z, C);
+      table_load<KT, MODE>(a, m.z, m.z + n, td);
       table_store<KT, MODE, C>(td, n, tl, twb);
+      SegRegs cur;
 #pragma unroll
       for (int r = 0; r < 2; r++) {
-        const uint32_t s = min(s0 + seg_of_slot<SLOTS>(slot, r), s1 - 1);
+        const uint32_t s = min(m.x + seg_of_slot<SLOTS>(slot, r), m.y - 1);
         cur.lo[r] = a.seg_group[s];
         cur.hi[r] = a.seg_group[s + 1];
       }
+      __syncthreads();
+      tile_apply<F, G, MODE, VAR, C, KT, SLOTS>(a, m, cur, slot, q, base, tl, twb);
     }
-    __syncthreads();
-    if constexpr (pf) {
-      if (tn < a.ntiles) pn = prefetch<F, KT, MODE, C, SLOTS>(a, tn, mn, slot, fr, tr, sr);
-    }
-
-    uint32_t sa = s0, gc = g0;
-    while (sa < s1) {
-      uint32_t sb = s1;
-      if (g1 - gc > C) {  // largest sb with seg_group[sb] - gc <= C
-        uint32_t lo = sa + 1, hi = s1;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi + 1) / 2;
-          if (a.seg_group[mid] - gc <= C) lo = mid;
-          else hi = mid - 1;
-        }
-        sb = lo;
-      }
-      if (sa != s0) {  // a later chunk of a large tile: load it now
-        __syncthreads();
-        TableRegs<KT, MODE> td;
-        const uint32_t n = a.seg_group[sb] - gc;
-        table_load<KT, MODE>(a, gc, gc + n, td);
-        table_store<KT, MODE, C>(td, n, tl, twb);
-        __syncthreads();
-      }
-      const uint32_t rounds = (sb - sa + SLOTS - 1) / SLOTS;
-      uint32_t r0 = 0;
-      if (sa == s0) {
-        // Rounds 0 and 1 take their bounds from registers: no global load,
-        // hence no wait on the next tile's prefetch, in the common path.
-        const uint32_t s_0 = sa + seg_of_slot<SLOTS>(slot, 0);
-        if (s_0 < sb)
-          seg_apply<F, G, MODE, VAR>(a, base, tl, twb, cur.lo[0] - gc, cur.hi[0] - gc, s_0, q);
-        const uint32_t s_1 = sa + seg_of_slot<SLOTS>(slot, 1);
-        if (rounds > 1 && s_1 < sb)
-          seg_apply<F, G, MODE, VAR>(a, base, tl, twb, cur.lo[1] - gc, cur.hi[1] - gc, s_1, q);
-        r0 = 2;
-      }
-      for (uint32_t r = r0; r < rounds; r++) {
-        const uint32_t s = sa + seg_of_slot<SLOTS>(slot, r);
-        if (s < sb)
-          seg_apply<F, G, MODE, VAR>(a, base, tl, twb, a.seg_group[s] - gc,
-                                     a.seg_group[s + 1] - gc, s, q);
-      }
-      sa = sb;
-      if (sa < s1) gc = a.seg_group[sa];
-    }
-    m = mn;
-    mn = mnn;
   }
 }
 

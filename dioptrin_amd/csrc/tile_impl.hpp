@@ -10,6 +10,14 @@ namespace tk {
 
 constexpr unsigned kXcds = 8;  // accelerator complex dies of an MI355X
 
+// Native vector types for table words held in registers and LDS: their loads
+// and stores are single instructions, where copies of HIP's vector structs
+// can become byte copies through scratch memory.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 // ---------------------------------------------------------------- accumulators
 
 // Float32 fused multiply-add (CudaPrecision::F32).
@@ -118,7 +126,7 @@ struct SegRegs {
 // Applies segment s (table groups [ga, gb) of the LDS chunk) to this lane's G
 // frames and writes its partials.
 template <int F, int G, int MODE, bool VAR, class LT>
-__device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, const uint2* tl,
+__device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, const u32x2* tl,
                                           const unsigned char* twb, uint32_t ga, uint32_t gb,
                                           uint32_t s, uint32_t q) {
   using Acc = typename TAcc<MODE>::T;
@@ -130,19 +138,19 @@ __device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, con
     if constexpr (VAR) var[j].init();
   }
   for (uint32_t g = ga; g < gb; g++) {
-    const uint2 lc = tl[g];
+    const u32x2 lc = tl[g];
     const uint32_t l[4] = {lc.x & 0xffffu, lc.x >> 16, lc.y & 0xffffu, lc.y >> 16};
     float wf[4] = {0.f, 0.f, 0.f, 0.f};
     double wd[4] = {0.0, 0.0, 0.0, 0.0};
     if constexpr (MODE == 2) {
-      const double2* tw = reinterpret_cast<const double2*>(twb);
-      const double2 x0 = tw[2 * g], x1 = tw[2 * g + 1];
+      const f64x2* tw = reinterpret_cast<const f64x2*>(twb);
+      const f64x2 x0 = tw[2 * g], x1 = tw[2 * g + 1];
       wd[0] = x0.x;
       wd[1] = x0.y;
       wd[2] = x1.x;
       wd[3] = x1.y;
     } else {
-      const float4 w4 = reinterpret_cast<const float4*>(twb)[g];
+      const f32x4 w4 = reinterpret_cast<const f32x4*>(twb)[g];
       wf[0] = w4.x;
       wf[1] = w4.y;
       wf[2] = w4.z;
