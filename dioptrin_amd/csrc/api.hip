@@ -2,6 +2,7 @@
 // tables), integration entry points, autotuner, graph replay.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -127,6 +128,12 @@ dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& des
                      t.w32.p, t.w64.p, desc.precision == DIO_F64, p->stream);
     p->tt.den = p->b.den.p;
     p->tt.cus = (uint32_t)std::max(1, ctx->prop.multiProcessorCount);
+    // DIOPTRIN_TILE_WORKGROUPS caps the gather's persistent grid (tuning, and
+    // tests of the many-tiles-per-workgroup path on small detectors).
+    if (const char* e = std::getenv("DIOPTRIN_TILE_WORKGROUPS")) {
+      const long v = std::atol(e);
+      if (v > 0) p->tt.max_wgs = (uint32_t)std::min<long>(v, 1L << 20);
+    }
     if (desc.precision == DIO_F64) p->tile = std::min(p->tile, dio::max_frames_for(DIO_F64));
     p->st.row_ptr = p->b.full.row_ptr.p;
     p->st.col = p->b.full.col.p;

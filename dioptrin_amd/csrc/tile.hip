@@ -265,41 +265,48 @@ template <int F> struct FrameRegs {
   uint32_t v[F][kPasses][2];
 };
 
-// Pair loads are 8-byte loads when every pair starts 8-byte aligned (even row
-// length and an 8-byte aligned frame block), else two 4-byte loads. The loads
-// are unconditional, so all of a tile's loads issue back to back: lanes
-// outside the detector, or of frames past the pass, read a valid address
-// (pixel 0 of the pass's frame 0 or last frame) and their words are never used
-// (no table entry points at such a pixel; partials of frames >= nf are not
-// written).
-template <int F, size_t A>
-__device__ __forceinline__ void frames_load32_as(const TileArgs& a, const TileGeom& g,
-                                                 FrameRegs<F>& fr) {
-  const uint32_t* src = static_cast<const uint32_t*>(a.frames);
+// Load at a workgroup-uniform base plus a 32-bit per-lane byte offset. This
+// is the addressing form of global loads with a scalar base register: each
+// load needs one 32-bit offset register, not a 64-bit address pair.
+template <class T>
+__device__ __forceinline__ T load_at(const void* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(static_cast<const unsigned char*>(base) + byte_off);
+}
+
+// Pair loads are 8-byte loads (A = 8) when every pair starts 8-byte aligned
+// (even row length and an 8-byte aligned frame block), else two 4-byte loads
+// (A = 4). The loads are unconditional, so all of a tile's loads issue back to
+// back: lanes outside the detector, or of frames past the pass, read a valid
+// address (pixel 0 of the pass's frame 0 or last frame) and their words are
+// never used (no table entry points at such a pixel; partials of frames >= nf
+// are not written). Byte offsets fit 32 bits: a frame holds < 2^29 pixels.
+template <int F, int A>
+__device__ __forceinline__ void frames_load32(const TileArgs& a, uint32_t t, FrameRegs<F>& fr) {
+  static_assert(A == 4 || A == 8, "pair loads are 4 or 8 bytes");
+  const TileGeom g = tile_geom(a, t);
+  const unsigned char* src = static_cast<const unsigned char*>(a.frames);
   const uint32_t flast = a.nf - 1;
+  uint32_t o0[kPasses], o1[kPasses];
+#pragma unroll
+  for (int h = 0; h < kPasses; h++) {
+    o0[h] = g.nvalid[h] > 0 ? g.off[h] * 4u : 0u;
+    o1[h] = g.nvalid[h] > 1 ? g.off[h] * 4u + 4u : 0u;
+  }
 #pragma unroll
   for (int f = 0; f < F; f++) {
-    const uint32_t* pf = src + min((uint32_t)f, flast) * a.npix;
+    const unsigned char* pf = src + (uint64_t)min((uint32_t)f, flast) * a.npix * 4u;
 #pragma unroll
     for (int h = 0; h < kPasses; h++) {
       if constexpr (A == 8) {
-        const uint32_t off = g.nvalid[h] ? g.off[h] : 0u;
-        const Run2<uint32_t, 8> r = *reinterpret_cast<const Run2<uint32_t, 8>*>(pf + off);
-        fr.v[f][h][0] = r.v[0];
-        fr.v[f][h][1] = r.v[1];
+        const u32x2 r = load_at<u32x2>(pf, o0[h]);
+        fr.v[f][h][0] = r.x;
+        fr.v[f][h][1] = r.y;
       } else {
-        fr.v[f][h][0] = pf[g.nvalid[h] > 0 ? g.off[h] : 0u];
-        fr.v[f][h][1] = pf[g.nvalid[h] > 1 ? g.off[h] + 1 : 0u];
+        fr.v[f][h][0] = load_at<uint32_t>(pf, o0[h]);
+        fr.v[f][h][1] = load_at<uint32_t>(pf, o1[h]);
       }
     }
   }
-}
-
-template <int F>
-__device__ __forceinline__ void frames_load32(const TileArgs& a, uint32_t t, FrameRegs<F>& fr) {
-  const TileGeom g = tile_geom(a, t);
-  if (a.aligned8) frames_load32_as<F, 8>(a, g, fr);
-  else frames_load32_as<F, 4>(a, g, fr);
 }
 
 template <class W, class LT>
@@ -352,7 +359,7 @@ __device__ __forceinline__ void stage_direct(const TileArgs& a, uint32_t t, LT* 
     P v[B][kPasses][2];
 #pragma unroll
     for (int k = 0; k < B; k++) {
-      // Unconditional loads from valid addresses (see frames_load32_as).
+      // Unconditional loads from valid addresses (see frames_load32).
       const P* pf = fr + min((uint32_t)(k0 + k), a.nf - 1) * a.npix;
 #pragma unroll
       for (int h = 0; h < kPasses; h++) {
@@ -384,19 +391,22 @@ template <int KT, int MODE> struct TableRegs {
 template <int KT, int MODE>
 __device__ __forceinline__ void table_load(const TileArgs& a, uint32_t gb, uint32_t ge,
                                            TableRegs<KT, MODE>& tr) {
-  const u32x2* lidx4 = reinterpret_cast<const u32x2*>(a.lidx);
   // Unconditional loads (indices past the chunk repeat its last group and
   // are not stored): no divergent exec masks, no partially written registers.
+  // The chunk's base is uniform; lanes add 32-bit byte offsets.
+  const uint32_t last = ge - gb - 1;
+  const void* lb = reinterpret_cast<const unsigned char*>(a.lidx) + (uint64_t)gb * 8;
+  const void* wb32 = reinterpret_cast<const unsigned char*>(a.w32) + (uint64_t)gb * 16;
+  const void* wb64 = reinterpret_cast<const unsigned char*>(a.w64) + (uint64_t)gb * 32;
 #pragma unroll
   for (int k = 0; k < KT; k++) {
-    const uint32_t g = min(gb + threadIdx.x + k * kTileThreads, ge - 1);
-    tr.li[k] = lidx4[g];
+    const uint32_t i = min(threadIdx.x + k * kTileThreads, last);
+    tr.li[k] = load_at<u32x2>(lb, i * 8u);
     if constexpr (MODE == 2) {
-      const f64x2* wd2 = reinterpret_cast<const f64x2*>(a.w64);
-      tr.wa[k] = wd2[2 * g];
-      tr.wb[k] = wd2[2 * g + 1];
+      tr.wa[k] = load_at<f64x2>(wb64, i * 32u);
+      tr.wb[k] = load_at<f64x2>(wb64, i * 32u + 16u);
     } else {
-      tr.w[k] = reinterpret_cast<const f32x4*>(a.w32)[g];
+      tr.w[k] = load_at<f32x4>(wb32, i * 16u);
     }
   }
 }
@@ -420,22 +430,35 @@ __device__ __forceinline__ void table_store(const TableRegs<KT, MODE>& tr, uint3
   }
 }
 
+// Group bounds of the segments this slot takes in rounds 0 and 1 of the tile
+// with metadata m (clamped to the tile's last segment).
+template <uint32_t SLOTS>
+__device__ __forceinline__ void seg_bounds(const TileArgs& a, uint4 m, uint32_t slot,
+                                           SegRegs& sr) {
+  const void* sb = reinterpret_cast<const unsigned char*>(a.seg_group) + (uint64_t)m.x * 4;
+  const uint32_t last = m.y - m.x - 1;
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const uint32_t i = min(seg_of_slot<SLOTS>(slot, r), last);
+    sr.lo[r] = load_at<uint32_t>(sb, i * 4u);
+    sr.hi[r] = load_at<uint32_t>(sb, i * 4u + 4u);
+  }
+}
+
 // Loads tile tt's frames, first table chunk and the slot's first segment
 // bounds into registers (m = tile_meta[tt]); returns the groups loaded.
-template <int F, int KT, int MODE, uint32_t C, uint32_t SLOTS>
+template <int F, int A, int KT, int MODE, uint32_t C, uint32_t SLOTS>
 __device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, uint4 m,
                                              uint32_t slot, FrameRegs<F>& fr,
                                              TableRegs<KT, MODE>& tr, SegRegs& sr) {
   if (m.x == m.y) return 0;
   const uint32_t n = min(m.w - m.z, C);
-  frames_load32<F>(a, tt, fr);
+  // Segment bounds first: loads complete in issue order, so the copy of the
+  // bounds at the top of the next iteration (before its barrier) waits only
+  // for these, not for the frames and table behind them.
+  seg_bounds<SLOTS>(a, m, slot, sr);
+  frames_load32<F, A>(a, tt, fr);
   table_load<KT, MODE>(a, m.z, m.z + n, tr);
-#pragma unroll
-  for (int r = 0; r < 2; r++) {
-    const uint32_t s = min(m.x + seg_of_slot<SLOTS>(slot, r), m.y - 1);
-    sr.lo[r] = a.seg_group[s];
-    sr.hi[r] = a.seg_group[s + 1];
-  }
   return n;
 }
 
@@ -444,6 +467,31 @@ __device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, uin
 // s_waitcnt immediate for vmcnt(0) (gfx9 encoding: vmcnt in bits 3:0 and
 // 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their no-wait maxima).
 constexpr int kWaitVm0 = 0x0f70;
+
+// Tile metadata (s0, s1, g0, g1) of a workgroup's tiles t0 + k * grid, 64
+// iterations at a time: lane i of every wavefront holds iteration k0 + i. The
+// tile loop reads it with v_readlane into scalar registers, so it waits on no
+// memory there (a batch load waits once per 64 tiles). Iterations past the
+// last tile read as empty tiles.
+constexpr uint32_t kMetaLanes = 64;
+struct MetaLanes {
+  u32x4 v;
+  uint32_t k0;
+  __device__ __forceinline__ void load(const TileArgs& a, uint32_t t0, uint32_t grid,
+                                       uint32_t k) {
+    k0 = k;
+    const uint32_t i = t0 + (k + threadIdx.x % kMetaLanes) * grid;
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    v = i < a.ntiles ? reinterpret_cast<const u32x4*>(a.tile_meta)[i] : z;
+  }
+  __device__ __forceinline__ uint4 get(uint32_t k) const {  // k0 <= k < k0 + 64
+    const int l = (int)(k - k0);
+    return make_uint4((uint32_t)__builtin_amdgcn_readlane((int)v.x, l),
+                      (uint32_t)__builtin_amdgcn_readlane((int)v.y, l),
+                      (uint32_t)__builtin_amdgcn_readlane((int)v.z, l),
+                      (uint32_t)__builtin_amdgcn_readlane((int)v.w, l));
+  }
+};
 
 // Computes one staged tile (m = its tile_meta): segments [s0, s1) in chunks
 // of at most C table groups; the first chunk is already in LDS, later chunks
@@ -508,13 +556,14 @@ __device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m, const Seg
 }
 
 // Persistent: workgroup b processes tiles b, b + grid, ... For 32-bit frames
-// the loop is software-pipelined: each iteration stages the tile whose
+// (FA = 4 or 8: the byte width of a pair load; FA = 0 stages other frame types
+// directly) the loop is software-pipelined: each iteration stages the tile whose
 // frames and first table chunk are in registers, loads the next tile into
 // the same registers, then computes the staged tile while those loads are in
 // flight. There is one prefetch site, so the prefetched registers are not
 // duplicated across branches (which would push the kernel past its register
 // budget into spills that wait on the prefetch).
-template <int F, int G, int MODE, bool VAR, bool PF>
+template <int F, int G, int MODE, bool VAR, int FA>
 __global__ __launch_bounds__(kTileThreads)
 __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) void k_tile(
     TileArgs a) {
@@ -532,34 +581,24 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
   const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
   const LT* base = fl + q * G * kLdsStride;
 
-  // Tile metadata is read through the scalar cache (constant address space,
-  // workgroup-uniform index): it lands in scalar registers, so the branches on
-  // it are uniform and waiting for it never waits on the vector loads.
-  typedef const __attribute__((address_space(4))) u32x4 CMeta;
-  CMeta* meta = (CMeta*)(a.tile_meta);
   const uint32_t grid = gridDim.x;
   const uint4 none = make_uint4(0u, 0u, 0u, 0u);
   const uint32_t t0 = first_tile();
   if (t0 >= a.ntiles) return;
-  // Metadata of tile i, or an empty tile past the end (a clamped load and a
-  // select of values: a select of addresses would leave the scalar path).
-  auto meta_at = [&](uint32_t i) {
-    const u32x4 v = meta[min(i, a.ntiles - 1)];
-    const bool ok = i < a.ntiles;
-    return make_uint4(ok ? v.x : 0u, ok ? v.y : 0u, v.z, v.w);
-  };
+  MetaLanes ml;
+  ml.load(a, t0, grid, 0);
 
-  if constexpr (PF) {
+  if constexpr (FA != 0) {
     FrameRegs<F> fr;
     TableRegs<KT, MODE> tr;
     SegRegs sr = {};
     uint32_t pn = 0;    // groups of the chunk in tr
     uint4 m = none;     // tile held in registers (none before the first load)
-    uint4 mn = meta_at(t0);
-    // Tile metadata (s0, s1, g0, g1) is read one tile ahead of the prefetch.
-    for (uint32_t tn = t0;; tn += grid) {
+    uint4 mn = ml.get(0);
+    // Iteration k prefetches tile t0 + k * grid and computes the one before.
+    for (uint32_t k = 0;; k++) {
+      const uint32_t tn = t0 + k * grid;
       const bool more = tn < a.ntiles;
-      const uint4 mnn = meta_at(tn + grid);
       const bool work = m.x != m.y;
       const SegRegs cur = sr;
       if (work) {
@@ -568,15 +607,20 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
         table_store<KT, MODE, C>(tr, pn, tl, twb);
         __syncthreads();
       }
-      if (more) pn = prefetch<F, KT, MODE, C, SLOTS>(a, tn, mn, slot, fr, tr, sr);
+      if (k + 1 - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k + 1);
+      const uint4 mnn = ml.get(k + 1);
+      if (more) pn = prefetch<F, FA, KT, MODE, C, SLOTS>(a, tn, mn, slot, fr, tr, sr);
       if (work) tile_apply<F, G, MODE, VAR, C, KT, SLOTS>(a, m, cur, slot, q, base, tl, twb);
       if (!more) break;
       m = mn;
       mn = mnn;
     }
   } else {
-    for (uint32_t t = t0; t < a.ntiles; t += grid) {
-      const uint4 m = meta_at(t);
+    for (uint32_t k = 0;; k++) {
+      const uint32_t t = t0 + k * grid;
+      if (t >= a.ntiles) break;
+      if (k - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k);
+      const uint4 m = ml.get(k);
       if (m.x == m.y) continue;  // nothing of the table in this tile
       __syncthreads();
       switch (a.dtype) {
@@ -588,12 +632,7 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
       table_load<KT, MODE>(a, m.z, m.z + n, td);
       table_store<KT, MODE, C>(td, n, tl, twb);
       SegRegs cur;
-#pragma unroll
-      for (int r = 0; r < 2; r++) {
-        const uint32_t s = min(m.x + seg_of_slot<SLOTS>(slot, r), m.y - 1);
-        cur.lo[r] = a.seg_group[s];
-        cur.hi[r] = a.seg_group[s + 1];
-      }
+      seg_bounds<SLOTS>(a, m, slot, cur);
       __syncthreads();
       tile_apply<F, G, MODE, VAR, C, KT, SLOTS>(a, m, cur, slot, q, base, tl, twb);
     }
@@ -666,7 +705,7 @@ __global__ __launch_bounds__(256) void k_tfinal(const PT* __restrict__ part, uin
 
 // ------------------------------------------------------------- dispatch
 
-template <int F, int G, int MODE, bool VAR, bool PF>
+template <int F, int G, int MODE, bool VAR, int FA>
 void launch_tile(const TiledTable& t, const TileArgs& a, hipStream_t s) {
   using PL = Plan<F, MODE>;
   constexpr size_t lds = PL::kBytes;
@@ -677,21 +716,28 @@ void launch_tile(const TiledTable& t, const TileArgs& a, hipStream_t s) {
   DIO_HIP(hipGetDevice(&dev));
   const uint64_t bit = 1ull << (dev & 63);
   if (!(done.load() & bit)) {
-    DIO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<F, G, MODE, VAR, PF>),
+    DIO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<F, G, MODE, VAR, FA>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     done.fetch_or(bit);
   }
-  const uint32_t grid = std::min<uint32_t>(t.ntiles, t.cus * PL::kWgs);
-  k_tile<F, G, MODE, VAR, PF><<<grid, kTileThreads, lds, s>>>(a);
+  uint32_t grid = std::min<uint32_t>(t.ntiles, t.cus * PL::kWgs);
+  if (t.max_wgs) grid = std::min(grid, t.max_wgs);
+  k_tile<F, G, MODE, VAR, FA><<<grid, kTileThreads, lds, s>>>(a);
 }
 
 template <int F, int G, int MODE>
 void launch_v(const TiledTable& t, const TileArgs& a, bool var, hipStream_t s) {
-  const bool pf = a.dtype == DIO_U32 || a.dtype == DIO_I32 || a.dtype == DIO_F32T;
-  if (var && pf) launch_tile<F, G, MODE, true, true>(t, a, s);
-  else if (var) launch_tile<F, G, MODE, true, false>(t, a, s);
-  else if (pf) launch_tile<F, G, MODE, false, true>(t, a, s);
-  else launch_tile<F, G, MODE, false, false>(t, a, s);
+  const bool w32 = a.dtype == DIO_U32 || a.dtype == DIO_I32 || a.dtype == DIO_F32T;
+  const int fa = !w32 ? 0 : a.aligned8 ? 8 : 4;
+  if (var) {
+    if (fa == 8) launch_tile<F, G, MODE, true, 8>(t, a, s);
+    else if (fa == 4) launch_tile<F, G, MODE, true, 4>(t, a, s);
+    else launch_tile<F, G, MODE, true, 0>(t, a, s);
+  } else {
+    if (fa == 8) launch_tile<F, G, MODE, false, 8>(t, a, s);
+    else if (fa == 4) launch_tile<F, G, MODE, false, 4>(t, a, s);
+    else launch_tile<F, G, MODE, false, 0>(t, a, s);
+  }
 }
 
 template <int F, int G>

@@ -123,6 +123,12 @@ struct SegRegs {
   uint32_t lo[2], hi[2];
 };
 
+// Table groups a lane takes per step of a segment (more groups hide more LDS
+// latency and hold more frame values in registers).
+template <int G, int MODE> struct SegGroups {
+  static constexpr int P = (MODE == 2 || G >= 4) ? 2 : 4;
+};
+
 // Applies segment s (table groups [ga, gb) of the LDS chunk) to this lane's G
 // frames and writes its partials.
 template <int F, int G, int MODE, bool VAR, class LT>
@@ -137,34 +143,60 @@ __device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, con
     acc[j].init();
     if constexpr (VAR) var[j].init();
   }
-  for (uint32_t g = ga; g < gb; g++) {
-    const u32x2 lc = tl[g];
-    const uint32_t l[4] = {lc.x & 0xffffu, lc.x >> 16, lc.y & 0xffffu, lc.y >> 16};
-    float wf[4] = {0.f, 0.f, 0.f, 0.f};
-    double wd[4] = {0.0, 0.0, 0.0, 0.0};
-    if constexpr (MODE == 2) {
-      const f64x2* tw = reinterpret_cast<const f64x2*>(twb);
-      const f64x2 x0 = tw[2 * g], x1 = tw[2 * g + 1];
-      wd[0] = x0.x;
-      wd[1] = x0.y;
-      wd[2] = x1.x;
-      wd[3] = x1.y;
-    } else {
-      const f32x4 w4 = reinterpret_cast<const f32x4*>(twb)[g];
-      wf[0] = w4.x;
-      wf[1] = w4.y;
-      wf[2] = w4.z;
-      wf[3] = w4.w;
-    }
+  // P groups per iteration: their table words are read together, then all
+  // their frame values, so an iteration waits for two LDS round trips for P
+  // groups. Groups past the segment's end read the zero slot with weight 0:
+  // such a term adds exactly +0 to every accumulator, whose value never is -0
+  // (it starts at +0, and a sum is -0 only if both addends are), so the
+  // result is bit-identical to adding the segment's terms alone, in order.
+  constexpr int P = SegGroups<G, MODE>::P;
+  const f64x2* twd = reinterpret_cast<const f64x2*>(twb);
+  const f32x4* twf = reinterpret_cast<const f32x4*>(twb);
+  for (uint32_t g = ga; g < gb; g += P) {
+    uint32_t l[P][4];
+    float wf[P][4];
+    double wd[P][4];
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-#pragma unroll
-      for (int j = 0; j < G; j++) {
-        const LT x = base[j * kLdsStride + l[e]];
-        acc[j].add(wf[e], wd[e], x);
-        if constexpr (VAR) var[j].add_var(wf[e], wd[e], x);
+    for (int p = 0; p < P; p++) {
+      const bool in = g + p < gb;
+      const uint32_t gp = in ? g + p : g;
+      const u32x2 lc = tl[gp];
+      l[p][0] = in ? (lc.x & 0xffffu) : kPadSlot;
+      l[p][1] = in ? (lc.x >> 16) : kPadSlot;
+      l[p][2] = in ? (lc.y & 0xffffu) : kPadSlot;
+      l[p][3] = in ? (lc.y >> 16) : kPadSlot;
+      if constexpr (MODE == 2) {
+        const f64x2 x0 = twd[2 * gp], x1 = twd[2 * gp + 1];
+        wd[p][0] = in ? x0.x : 0.0;
+        wd[p][1] = in ? x0.y : 0.0;
+        wd[p][2] = in ? x1.x : 0.0;
+        wd[p][3] = in ? x1.y : 0.0;
+        wf[p][0] = wf[p][1] = wf[p][2] = wf[p][3] = 0.f;
+      } else {
+        const f32x4 w4 = twf[gp];
+        wf[p][0] = in ? w4.x : 0.f;
+        wf[p][1] = in ? w4.y : 0.f;
+        wf[p][2] = in ? w4.z : 0.f;
+        wf[p][3] = in ? w4.w : 0.f;
+        wd[p][0] = wd[p][1] = wd[p][2] = wd[p][3] = 0.0;
       }
     }
+    LT x[P][4][G];
+#pragma unroll
+    for (int p = 0; p < P; p++)
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int j = 0; j < G; j++) x[p][e][j] = base[j * kLdsStride + l[p][e]];
+#pragma unroll
+    for (int p = 0; p < P; p++)
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          acc[j].add(wf[p][e], wd[p][e], x[p][e][j]);
+          if constexpr (VAR) var[j].add_var(wf[p][e], wd[p][e], x[p][e][j]);
+        }
   }
   PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * F + q * G;
 #pragma unroll

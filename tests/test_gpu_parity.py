@@ -165,6 +165,33 @@ def test_batch_equals_single_frame_bitwise(pilatus, tile, precision):
         assert np.array_equal(S[k], single[k][1][0])
 
 
+@pytest.mark.parametrize("dtype", [np.uint32, np.float64], ids=["u32", "f64"])
+def test_capped_grid_many_tiles_per_workgroup(pilatus, dtype, monkeypatch):
+    """Eight workgroups walk the Pilatus tiles (about 130 each, so the
+    per-workgroup tile metadata is reloaded twice); results equal the
+    full-grid run bit for bit, for register-prefetched (u32) and directly
+    staged (f64) frames."""
+    d = pilatus
+    frames = np.random.default_rng(11).poisson(150.0, size=(19,) + d["w"]["shape"]).astype(dtype)
+
+    def run():
+        plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
+                       polarization_factor=0.99, mask=d["mask"], precision="f32_wide")
+        out = []
+        for tile in ((16, 2), (8, 4), (1, 1)):
+            plan.set_tile(*tile)
+            out.append(plan.integrate_stack(frames, errors=True)[:2])
+        return out
+
+    monkeypatch.delenv("DIOPTRIN_TILE_WORKGROUPS", raising=False)
+    full = run()
+    monkeypatch.setenv("DIOPTRIN_TILE_WORKGROUPS", "8")
+    capped = run()
+    for (i0, s0), (i1, s1) in zip(full, capped):
+        assert np.array_equal(i0, i1)
+        assert np.array_equal(s0, s1)
+
+
 def test_pixel_dtypes_equal_float64(pilatus):
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
