@@ -7,7 +7,7 @@
 // float32 (float64 for the F64 precision).
 //
 // k_tile: persistent workgroups of 256 threads, each walking detector tiles
-// of 16 x 64 pixels (tile b, b + grid, ...).
+// of kTileRows x kTileCols pixels (tile b, b + grid, ...).
 //  1. Staging: every thread owns 2 consecutive pixels of one tile row (32
 //     lanes cover a 256-byte row run) for each of the pass's F frames. For
 //     32-bit frames the next tile's words and its table chunk (4-entry
@@ -94,6 +94,23 @@ __global__ void k_seg_flags(const uint32_t* tile, const uint32_t* bin, uint64_t 
     return;
   }
   flag[i] = (i == 0 || tile[i] != tile[i - 1] || bin[i] != bin[i - 1]) ? 1u : 0u;
+}
+
+// Pieces: a (tile, bin) run longer than kMaxSegGroups groups is cut into
+// pieces of that many groups (the last one shorter), each a segment of its
+// own. Entry i starts a piece if it starts a run or sits a whole number of
+// pieces into its run (run_first: first entry of each run).
+__global__ void k_piece_flags(const uint32_t* flag, const uint32_t* runid,
+                              const uint32_t* run_first, uint64_t nnz, uint32_t* pflag) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nnz) return;
+  if (i == nnz) {
+    pflag[i] = 0;
+    return;
+  }
+  const uint32_t run = runid[i] + flag[i] - 1;
+  const uint64_t pos = i - run_first[run];
+  pflag[i] = (pos % ((uint64_t)kMaxSegGroups * kGroup) == 0) ? 1u : 0u;
 }
 
 __global__ void k_seg_fill(const uint32_t* flag, const uint32_t* segid, const uint32_t* tile,
@@ -210,16 +227,24 @@ void sort_pairs(const K* kin, K* kout, const V* vin, V* vout, uint64_t n, int bi
 
 // One workgroup's LDS: frames [F][kLdsStride] of LT, then a table chunk of at
 // most kCap groups (kCap uint2 indices, then kCap float4 weights, or 2 x
-// double2 per group for float64 weights). The budget lets kWgs workgroups
-// share a CU (160 KiB of LDS) so one stages while another computes.
+// double2 per group for float64 weights). kWgs workgroups share a CU (160 KiB
+// of LDS), so some stage while others compute: the most (up to 4) that still
+// leave room for a chunk of kWantGroups groups, enough for a whole tile of the
+// usual tables (entries per pixel about 2).
 template <int F, int MODE> struct Plan {
   using LT = typename TAcc<MODE>::L;
-  static constexpr uint32_t kWgs = F >= 16 ? 2 : F == 8 ? (MODE == 2 ? 2 : 3) : F == 4 ? (MODE == 2 ? 3 : 4) : 4;
   static constexpr size_t kFrameBytes = ((size_t)F * kLdsStride * sizeof(LT) + 15) / 16 * 16;
   static constexpr size_t kWBytes = MODE == 2 ? 32 : 16;
-  static constexpr size_t kBudget = (160 * 1024 / kWgs) / 256 * 256;
+  static constexpr size_t kWantGroups = kTilePixels * 5 / (2 * kGroup);
+  static constexpr size_t budget(uint32_t w) { return (160 * 1024 / w) / 256 * 256; }
+  static constexpr bool fits(uint32_t w) {
+    return budget(w) > kFrameBytes && (budget(w) - kFrameBytes) / (8 + kWBytes) >= kWantGroups;
+  }
+  // Float64 lanes need more registers: at most 3 of their workgroups per CU.
+  static constexpr uint32_t kWgs = (MODE != 2 && fits(4)) ? 4 : fits(3) ? 3 : fits(2) ? 2 : 1;
+  static constexpr size_t kBudget = budget(kWgs);
   static constexpr size_t kRoom = (kBudget - kFrameBytes) / (8 + kWBytes);
-  static constexpr uint32_t kCap = kRoom >= 1024 ? 1024 : (uint32_t)(kRoom / 32 * 32);
+  static constexpr uint32_t kCap = kRoom >= 1024 ? 1024 : (uint32_t)(kRoom / 8 * 8);
   static constexpr size_t kBytes = kFrameBytes + (size_t)kCap * (8 + kWBytes);
   static constexpr int kKT = (kCap + kTileThreads - 1) / kTileThreads;  // table loads per thread
   static_assert(kBudget > kFrameBytes, "frames exceed the LDS budget");
@@ -430,15 +455,15 @@ __device__ __forceinline__ void table_store(const TableRegs<KT, MODE>& tr, uint3
   }
 }
 
-// Group bounds of the segments this slot takes in rounds 0 and 1 of the tile
-// with metadata m (clamped to the tile's last segment).
-template <uint32_t SLOTS>
+// Group bounds of the segments this slot takes in the first R rounds of the
+// tile with metadata m (clamped to the tile's last segment).
+template <uint32_t SLOTS, int R>
 __device__ __forceinline__ void seg_bounds(const TileArgs& a, uint4 m, uint32_t slot,
-                                           SegRegs& sr) {
+                                           SegRegs<R>& sr) {
   const void* sb = reinterpret_cast<const unsigned char*>(a.seg_group) + (uint64_t)m.x * 4;
   const uint32_t last = m.y - m.x - 1;
 #pragma unroll
-  for (int r = 0; r < 2; r++) {
+  for (int r = 0; r < R; r++) {
     const uint32_t i = min(seg_of_slot<SLOTS>(slot, r), last);
     sr.lo[r] = load_at<uint32_t>(sb, i * 4u);
     sr.hi[r] = load_at<uint32_t>(sb, i * 4u + 4u);
@@ -450,7 +475,8 @@ __device__ __forceinline__ void seg_bounds(const TileArgs& a, uint4 m, uint32_t 
 template <int F, int A, int KT, int MODE, uint32_t C, uint32_t SLOTS>
 __device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, uint4 m,
                                              uint32_t slot, FrameRegs<F>& fr,
-                                             TableRegs<KT, MODE>& tr, SegRegs& sr) {
+                                             TableRegs<KT, MODE>& tr,
+                                             SegRegs<SegRounds<SLOTS>::R>& sr) {
   if (m.x == m.y) return 0;
   const uint32_t n = min(m.w - m.z, C);
   // Segment bounds first: loads complete in issue order, so the copy of the
@@ -498,7 +524,8 @@ struct MetaLanes {
 // of a large tile are loaded here. cur holds the group bounds of this slot's
 // first two segments.
 template <int F, int G, int MODE, bool VAR, uint32_t C, int KT, uint32_t SLOTS, class LT>
-__device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m, const SegRegs& cur,
+__device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m,
+                                           const SegRegs<SegRounds<SLOTS>::R>& cur,
                                            uint32_t slot, uint32_t q, const LT* base, u32x2* tl,
                                            unsigned char* twb) {
   // Chunk bounds are workgroup-uniform: read them through the scalar cache so
@@ -534,15 +561,21 @@ __device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m, const Seg
     const uint32_t rounds = (sb - sa + SLOTS - 1) / SLOTS;
     uint32_t r0 = 0;
     if (sa == s0) {
-      // Rounds 0 and 1 take their bounds from registers: no global load,
+      // The first R rounds take their bounds from registers: no global load,
       // hence no wait on the next tile's prefetch, in the common path.
-      const uint32_t s_0 = sa + seg_of_slot<SLOTS>(slot, 0);
-      if (s_0 < sb)
-        seg_apply<F, G, MODE, VAR>(a, base, tl, twb, cur.lo[0] - gc, cur.hi[0] - gc, s_0, q);
-      const uint32_t s_1 = sa + seg_of_slot<SLOTS>(slot, 1);
-      if (rounds > 1 && s_1 < sb)
-        seg_apply<F, G, MODE, VAR>(a, base, tl, twb, cur.lo[1] - gc, cur.hi[1] - gc, s_1, q);
-      r0 = 2;
+      constexpr int R = SegRounds<SLOTS>::R;
+      const uint32_t rr = min(rounds, (uint32_t)R);
+      for (uint32_t r = 0; r < rr; r++) {
+        uint32_t lo = cur.lo[0], hi = cur.hi[0];
+#pragma unroll
+        for (int k = 1; k < R; k++) {
+          lo = r == (uint32_t)k ? cur.lo[k] : lo;
+          hi = r == (uint32_t)k ? cur.hi[k] : hi;
+        }
+        const uint32_t s_r = sa + seg_of_slot<SLOTS>(slot, r);
+        if (s_r < sb) seg_apply<F, G, MODE, VAR>(a, base, tl, twb, lo - gc, hi - gc, s_r, q);
+      }
+      r0 = R;
     }
     for (uint32_t r = r0; r < rounds; r++) {
       const uint32_t s = sa + seg_of_slot<SLOTS>(slot, r);
@@ -591,7 +624,7 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
   if constexpr (FA != 0) {
     FrameRegs<F> fr;
     TableRegs<KT, MODE> tr;
-    SegRegs sr = {};
+    SegRegs<SegRounds<SLOTS>::R> sr = {};
     uint32_t pn = 0;    // groups of the chunk in tr
     uint4 m = none;     // tile held in registers (none before the first load)
     uint4 mn = ml.get(0);
@@ -600,7 +633,7 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
       const uint32_t tn = t0 + k * grid;
       const bool more = tn < a.ntiles;
       const bool work = m.x != m.y;
-      const SegRegs cur = sr;
+      const SegRegs<SegRounds<SLOTS>::R> cur = sr;
       if (work) {
         __syncthreads();  // the previous tile's compute is done with the LDS
         frames_store32<F, LT>(a, fr, fl);
@@ -631,7 +664,7 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
       const uint32_t n = min(m.w - m.z, C);
       table_load<KT, MODE>(a, m.z, m.z + n, td);
       table_store<KT, MODE, C>(td, n, tl, twb);
-      SegRegs cur;
+      SegRegs<SegRounds<SLOTS>::R> cur;
       seg_bounds<SLOTS>(a, m, slot, cur);
       __syncthreads();
       tile_apply<F, G, MODE, VAR, C, KT, SLOTS>(a, m, cur, slot, q, base, tl, twb);
@@ -855,13 +888,28 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
   k_tile_gather<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(
       order.p, tile0.p, bin0.p, lidx0.p, w32, keep_w64 ? w64 : nullptr, nnz, tile1.p, bin1.p,
       lidx1.p, w32_1.p, keep_w64 ? w64_1.p : nullptr);
-  // 2. Segments = runs of equal (tile, bin).
+  // 2. Runs of equal (tile, bin), cut into segments of at most kMaxSegGroups
+  // groups.
   DevBuf<uint32_t> flag(nnz + 1), segid(nnz + 1);
-  k_seg_flags<<<blocks_for(nnz + 1, kThreads), kThreads, 0, s>>>(tile1.p, bin1.p, nnz, flag.p);
-  scan_u32(flag.p, segid.p, nnz + 1, s, tmp);
   uint32_t nseg = 0;
-  DIO_HIP(hipMemcpyAsync(&nseg, segid.p + nnz, 4, hipMemcpyDeviceToHost, s));
-  DIO_HIP(hipStreamSynchronize(s));
+  {
+    DevBuf<uint32_t> rflag(nnz + 1), runid(nnz + 1);
+    k_seg_flags<<<blocks_for(nnz + 1, kThreads), kThreads, 0, s>>>(tile1.p, bin1.p, nnz,
+                                                                  rflag.p);
+    scan_u32(rflag.p, runid.p, nnz + 1, s, tmp);
+    uint32_t nrun = 0;
+    DIO_HIP(hipMemcpyAsync(&nrun, runid.p + nnz, 4, hipMemcpyDeviceToHost, s));
+    DIO_HIP(hipStreamSynchronize(s));
+    DevBuf<uint32_t> run_first(nrun), run_tile(nrun), run_bin(nrun);
+    k_seg_fill<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(rflag.p, runid.p, tile1.p,
+                                                             bin1.p, nnz, run_first.p,
+                                                             run_tile.p, run_bin.p);
+    k_piece_flags<<<blocks_for(nnz + 1, kThreads), kThreads, 0, s>>>(rflag.p, runid.p,
+                                                                    run_first.p, nnz, flag.p);
+    scan_u32(flag.p, segid.p, nnz + 1, s, tmp);
+    DIO_HIP(hipMemcpyAsync(&nseg, segid.p + nnz, 4, hipMemcpyDeviceToHost, s));
+    DIO_HIP(hipStreamSynchronize(s));
+  }
   t.nseg = nseg;
   DevBuf<uint32_t> seg_first(nseg + 1), seg_tile(nseg), seg_bin(nseg);
   k_seg_fill<<<blocks_for(nnz, kThreads), kThreads, 0, s>>>(flag.p, segid.p, tile1.p, bin1.p,

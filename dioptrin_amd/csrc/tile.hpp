@@ -21,6 +21,9 @@ constexpr uint32_t kPadSlot = kTilePixels;          // LDS index of the zero pix
 constexpr uint32_t kLdsStride = kTilePixels + 2;
 constexpr uint32_t kGroup = 4;                      // entries per table group
 constexpr uint32_t kMaxFrames = 16;
+// Longest segment, in groups: a (tile, bin) run beyond it is cut into several
+// segments, so no lane of a tile walks far more groups than the others.
+constexpr uint32_t kMaxSegGroups = 12;
 
 struct TiledTable {
   uint32_t dim1 = 0, dim2 = 0, nbins = 0, ntx = 0, nty = 0, ntiles = 0;

@@ -111,22 +111,28 @@ struct TileArgs {
 };
 
 // Segment (relative to the chunk's first) a slot takes in round r. Segments
-// are stored longest first; round 1 walks the slots backwards, so a slot with
-// a long first segment gets a short second one.
+// are stored longest first; odd rounds walk the slots backwards, so a slot
+// with a long segment in one round gets a short one in the next.
 template <uint32_t SLOTS>
 __device__ __forceinline__ uint32_t seg_of_slot(uint32_t slot, uint32_t r) {
-  return r == 1 ? 2 * SLOTS - 1 - slot : slot + r * SLOTS;
+  return (r & 1) ? (r + 1) * SLOTS - 1 - slot : slot + r * SLOTS;
 }
 
-// Group bounds of the segments a slot takes in the first two rounds of a tile.
-struct SegRegs {
-  uint32_t lo[2], hi[2];
+// Rounds whose segment bounds are prefetched into registers: enough for 128
+// segments per tile.
+template <uint32_t SLOTS> struct SegRounds {
+  static constexpr int R = SLOTS >= 64 ? 2 : 4;
+};
+
+// Group bounds of the segments a slot takes in the first R rounds of a tile.
+template <int R> struct SegRegs {
+  uint32_t lo[R], hi[R];
 };
 
 // Table groups a lane takes per step of a segment (more groups hide more LDS
 // latency and hold more frame values in registers).
 template <int G, int MODE> struct SegGroups {
-  static constexpr int P = (MODE == 2 || G >= 4) ? 2 : 4;
+  static constexpr int P = 1;
 };
 
 // Applies segment s (table groups [ga, gb) of the LDS chunk) to this lane's G
