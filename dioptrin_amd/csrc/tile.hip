@@ -6,7 +6,7 @@
 // denominators f64[nbins]. Partials of one pass: [1 + errors][nseg][F]
 // float32 (float64 for the F64 precision).
 //
-// k_tile: persistent workgroups of 256 threads, each walking detector tiles
+// k_tile: persistent workgroups of 512 threads, each walking detector tiles
 // of kTileRows x kTileCols pixels (tile b, b + grid, ...).
 //  1. Staging: every thread owns 2 consecutive pixels of one tile row (32
 //     lanes cover a 256-byte row run) for each of the pass's F frames. For
@@ -43,11 +43,17 @@ using namespace tk;
 namespace {
 
 constexpr unsigned kThreads = 256;      // layout and reduction kernels
-constexpr unsigned kTileThreads = 256;  // k_tile workgroup: 4 wavefronts
-constexpr unsigned kRunLanes = kTileCols / 2;             // lanes per 2-pixel row run
-constexpr unsigned kRowsPerPass = kTileThreads / kRunLanes;
-constexpr int kPasses = kTileRows / kRowsPerPass;         // staging loads per frame
-static_assert(kPasses >= 1 && kPasses * kRowsPerPass == kTileRows, "staging map covers the tile");
+// k_tile workgroup size: 512 threads (8 wavefronts) for passes of 8 or more
+// frames, 256 below, where each workgroup needs less LDS and more of them
+// share a CU.
+constexpr unsigned tile_threads(int F) { return F >= 8 ? 512u : 256u; }
+constexpr unsigned kRunLanes = kTileCols / 2;  // lanes per 2-pixel row run
+template <unsigned NT> struct Staging {
+  static constexpr unsigned kRowsPerPass = NT / kRunLanes;
+  static constexpr int kPasses = (int)(kTileRows / kRowsPerPass);  // staging loads per frame
+  static_assert(kPasses >= 1 && kPasses * kRowsPerPass == kTileRows,
+                "staging map covers the tile");
+};
 
 // ---------------------------------------------------------------- layout
 
@@ -228,11 +234,12 @@ void sort_pairs(const K* kin, K* kout, const V* vin, V* vout, uint64_t n, int bi
 // One workgroup's LDS: frames [F][kLdsStride] of LT, then a table chunk of at
 // most kCap groups (kCap uint2 indices, then kCap float4 weights, or 2 x
 // double2 per group for float64 weights). kWgs workgroups share a CU (160 KiB
-// of LDS), so some stage while others compute: the most (up to 4) that still
-// leave room for a chunk of kWantGroups groups, enough for a whole tile of the
-// usual tables (entries per pixel about 2).
+// of LDS), so one stages while the other computes: two if that still leaves
+// room for a chunk of kWantGroups groups, enough for a whole tile of the usual
+// tables (entries per pixel about 2).
 template <int F, int MODE> struct Plan {
   using LT = typename TAcc<MODE>::L;
+  static constexpr unsigned kThreads = tile_threads(F);
   static constexpr size_t kFrameBytes = ((size_t)F * kLdsStride * sizeof(LT) + 15) / 16 * 16;
   static constexpr size_t kWBytes = MODE == 2 ? 32 : 16;
   static constexpr size_t kWantGroups = kTilePixels * 5 / (2 * kGroup);
@@ -240,13 +247,16 @@ template <int F, int MODE> struct Plan {
   static constexpr bool fits(uint32_t w) {
     return budget(w) > kFrameBytes && (budget(w) - kFrameBytes) / (8 + kWBytes) >= kWantGroups;
   }
-  // Float64 lanes need more registers: at most 3 of their workgroups per CU.
-  static constexpr uint32_t kWgs = (MODE != 2 && fits(4)) ? 4 : fits(3) ? 3 : fits(2) ? 2 : 1;
+  // At most 16 wavefronts per CU (128 registers per lane); float64 lanes
+  // need more registers, so at most 3 of their 256-thread workgroups.
+  static constexpr uint32_t kWgs =
+      kThreads == 512 ? (fits(2) ? 2 : 1)
+                      : (MODE != 2 && fits(4)) ? 4 : fits(3) ? 3 : fits(2) ? 2 : 1;
   static constexpr size_t kBudget = budget(kWgs);
   static constexpr size_t kRoom = (kBudget - kFrameBytes) / (8 + kWBytes);
   static constexpr uint32_t kCap = kRoom >= 1024 ? 1024 : (uint32_t)(kRoom / 8 * 8);
   static constexpr size_t kBytes = kFrameBytes + (size_t)kCap * (8 + kWBytes);
-  static constexpr int kKT = (kCap + kTileThreads - 1) / kTileThreads;  // table loads per thread
+  static constexpr int kKT = (kCap + kThreads - 1) / kThreads;  // table loads per thread
   static_assert(kBudget > kFrameBytes, "frames exceed the LDS budget");
   // A (tile, bin) segment holds at most one entry per tile pixel.
   static_assert(kCap >= kTilePixels / kGroup, "a whole segment must fit one table chunk");
@@ -257,37 +267,41 @@ template <int F, int MODE> struct Plan {
 template <class P, size_t A> struct alignas(A) Run2 { P v[2]; };
 template <class LT> struct alignas(2 * sizeof(LT)) Lds2 { LT v[2]; };
 
-// Staging map: thread tid owns pixel pairs (row h * kRowsPerPass + tid / 32,
-// columns 2 * (tid % 32) + {0, 1}) of the tile for h < kPasses, so 32 lanes
-// read one 256-byte row run and a wavefront two rows per load.
-struct TileGeom {
+// Staging map of NT threads: thread tid owns pixel pairs (row h *
+// kRowsPerPass + tid / 32, columns 2 * (tid % 32) + {0, 1}) of the tile for
+// h < kPasses, so 32 lanes read one 256-byte row run and a wavefront two rows
+// per load.
+template <unsigned NT> struct TileGeom {
+  static constexpr int kPasses = Staging<NT>::kPasses;
   uint32_t r0, cp;
   uint32_t nvalid[kPasses];  // valid pixels of the pair in each of its rows
   uint32_t off[kPasses];     // pixel offset in the frame (frames hold < 2^29 pixels)
 };
 
-__device__ __forceinline__ TileGeom tile_geom(const TileArgs& a, uint32_t t) {
-  TileGeom g;
+template <unsigned NT>
+__device__ __forceinline__ TileGeom<NT> tile_geom(const TileArgs& a, uint32_t t) {
+  TileGeom<NT> g;
   g.r0 = threadIdx.x / kRunLanes;
   g.cp = threadIdx.x % kRunLanes;
   const uint32_t c = (t % a.ntx) * kTileCols + g.cp * 2;
 #pragma unroll
-  for (int h = 0; h < kPasses; h++) {
-    const uint32_t r = (t / a.ntx) * kTileRows + g.r0 + kRowsPerPass * h;
+  for (int h = 0; h < Staging<NT>::kPasses; h++) {
+    const uint32_t r = (t / a.ntx) * kTileRows + g.r0 + Staging<NT>::kRowsPerPass * h;
     g.nvalid[h] = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
     g.off[h] = r * a.dim2 + c;
   }
   return g;
 }
 
+template <unsigned NT>
 __device__ __forceinline__ uint32_t lds_pixel(uint32_t r0, uint32_t cp, int h) {
-  return (r0 + kRowsPerPass * h) * kTileCols + cp * 2;
+  return (r0 + Staging<NT>::kRowsPerPass * h) * kTileCols + cp * 2;
 }
 
 // Raw 32-bit frame words of one tile held in registers (u32, i32, f32
 // frames), loaded one tile ahead of their use.
-template <int F> struct FrameRegs {
-  uint32_t v[F][kPasses][2];
+template <int F, unsigned NT> struct FrameRegs {
+  uint32_t v[F][Staging<NT>::kPasses][2];
 };
 
 // Load at a workgroup-uniform base plus a 32-bit per-lane byte offset. This
@@ -305,10 +319,12 @@ __device__ __forceinline__ T load_at(const void* base, uint32_t byte_off) {
 // address (pixel 0 of the pass's frame 0 or last frame) and their words are
 // never used (no table entry points at such a pixel; partials of frames >= nf
 // are not written). Byte offsets fit 32 bits: a frame holds < 2^29 pixels.
-template <int F, int A>
-__device__ __forceinline__ void frames_load32(const TileArgs& a, uint32_t t, FrameRegs<F>& fr) {
+template <int F, int A, unsigned NT>
+__device__ __forceinline__ void frames_load32(const TileArgs& a, uint32_t t,
+                                              FrameRegs<F, NT>& fr) {
   static_assert(A == 4 || A == 8, "pair loads are 4 or 8 bytes");
-  const TileGeom g = tile_geom(a, t);
+  constexpr int kPasses = Staging<NT>::kPasses;
+  const TileGeom<NT> g = tile_geom<NT>(a, t);
   const unsigned char* src = static_cast<const unsigned char*>(a.frames);
   const uint32_t flast = a.nf - 1;
   uint32_t o0[kPasses], o1[kPasses];
@@ -349,23 +365,24 @@ __device__ __forceinline__ void pad_slots(LT* fl) {
   }
 }
 
-template <int F, class W, class LT>
-__device__ __forceinline__ void frames_store_as(const FrameRegs<F>& fr, LT* fl) {
+template <int F, class W, class LT, unsigned NT>
+__device__ __forceinline__ void frames_store_as(const FrameRegs<F, NT>& fr, LT* fl) {
   const uint32_t r0 = threadIdx.x / kRunLanes, cp = threadIdx.x % kRunLanes;
 #pragma unroll
   for (int f = 0; f < F; f++) {
 #pragma unroll
-    for (int h = 0; h < kPasses; h++) {
+    for (int h = 0; h < Staging<NT>::kPasses; h++) {
       Lds2<LT> o;
       o.v[0] = word_to<W, LT>(fr.v[f][h][0]);
       o.v[1] = word_to<W, LT>(fr.v[f][h][1]);
-      *reinterpret_cast<Lds2<LT>*>(fl + f * kLdsStride + lds_pixel(r0, cp, h)) = o;
+      *reinterpret_cast<Lds2<LT>*>(fl + f * kLdsStride + lds_pixel<NT>(r0, cp, h)) = o;
     }
   }
 }
 
-template <int F, class LT>
-__device__ __forceinline__ void frames_store32(const TileArgs& a, const FrameRegs<F>& fr, LT* fl) {
+template <int F, class LT, unsigned NT>
+__device__ __forceinline__ void frames_store32(const TileArgs& a, const FrameRegs<F, NT>& fr,
+                                               LT* fl) {
   if (a.dtype == DIO_U32) frames_store_as<F, uint32_t, LT>(fr, fl);
   else if (a.dtype == DIO_I32) frames_store_as<F, int32_t, LT>(fr, fl);
   else frames_store_as<F, float, LT>(fr, fl);
@@ -374,10 +391,11 @@ __device__ __forceinline__ void frames_store32(const TileArgs& a, const FrameReg
 
 // Direct staging (u16 and f64 frames): load and store, eight frames of loads
 // in flight per thread.
-template <int F, class P, class LT>
+template <int F, class P, class LT, unsigned NT>
 __device__ __forceinline__ void stage_direct(const TileArgs& a, uint32_t t, LT* fl) {
+  constexpr int kPasses = Staging<NT>::kPasses;
   const P* fr = static_cast<const P*>(a.frames);
-  const TileGeom g = tile_geom(a, t);
+  const TileGeom<NT> g = tile_geom<NT>(a, t);
   constexpr int B = F < 8 ? F : 8;
 #pragma unroll
   for (int k0 = 0; k0 < F; k0 += B) {
@@ -399,7 +417,8 @@ __device__ __forceinline__ void stage_direct(const TileArgs& a, uint32_t t, LT* 
         Lds2<LT> o;
         o.v[0] = static_cast<LT>(v[k][h][0]);
         o.v[1] = static_cast<LT>(v[k][h][1]);
-        *reinterpret_cast<Lds2<LT>*>(fl + (k0 + k) * kLdsStride + lds_pixel(g.r0, g.cp, h)) = o;
+        *reinterpret_cast<Lds2<LT>*>(fl + (k0 + k) * kLdsStride + lds_pixel<NT>(g.r0, g.cp, h)) =
+            o;
       }
     }
   }
@@ -413,7 +432,7 @@ template <int KT, int MODE> struct TableRegs {
   f64x2 wa[MODE == 2 ? KT : 1], wb[MODE == 2 ? KT : 1];
 };
 
-template <int KT, int MODE>
+template <int KT, int MODE, unsigned NT>
 __device__ __forceinline__ void table_load(const TileArgs& a, uint32_t gb, uint32_t ge,
                                            TableRegs<KT, MODE>& tr) {
   // Unconditional loads (indices past the chunk repeat its last group and
@@ -425,7 +444,7 @@ __device__ __forceinline__ void table_load(const TileArgs& a, uint32_t gb, uint3
   const void* wb64 = reinterpret_cast<const unsigned char*>(a.w64) + (uint64_t)gb * 32;
 #pragma unroll
   for (int k = 0; k < KT; k++) {
-    const uint32_t i = min(threadIdx.x + k * kTileThreads, last);
+    const uint32_t i = min(threadIdx.x + k * NT, last);
     tr.li[k] = load_at<u32x2>(lb, i * 8u);
     if constexpr (MODE == 2) {
       tr.wa[k] = load_at<f64x2>(wb64, i * 32u);
@@ -436,12 +455,12 @@ __device__ __forceinline__ void table_load(const TileArgs& a, uint32_t gb, uint3
   }
 }
 
-template <int KT, int MODE, uint32_t CAP>
+template <int KT, int MODE, unsigned NT>
 __device__ __forceinline__ void table_store(const TableRegs<KT, MODE>& tr, uint32_t n,
                                             u32x2* tl, unsigned char* twb) {
 #pragma unroll
   for (int k = 0; k < KT; k++) {
-    const uint32_t i = threadIdx.x + k * kTileThreads;
+    const uint32_t i = threadIdx.x + k * NT;
     if (i < n) {
       tl[i] = tr.li[k];
       if constexpr (MODE == 2) {
@@ -472,9 +491,9 @@ __device__ __forceinline__ void seg_bounds(const TileArgs& a, uint4 m, uint32_t 
 
 // Loads tile tt's frames, first table chunk and the slot's first segment
 // bounds into registers (m = tile_meta[tt]); returns the groups loaded.
-template <int F, int A, int KT, int MODE, uint32_t C, uint32_t SLOTS>
+template <int F, int A, int KT, int MODE, uint32_t C, uint32_t SLOTS, unsigned NT>
 __device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, uint4 m,
-                                             uint32_t slot, FrameRegs<F>& fr,
+                                             uint32_t slot, FrameRegs<F, NT>& fr,
                                              TableRegs<KT, MODE>& tr,
                                              SegRegs<SegRounds<SLOTS>::R>& sr) {
   if (m.x == m.y) return 0;
@@ -484,7 +503,7 @@ __device__ __forceinline__ uint32_t prefetch(const TileArgs& a, uint32_t tt, uin
   // for these, not for the frames and table behind them.
   seg_bounds<SLOTS>(a, m, slot, sr);
   frames_load32<F, A>(a, tt, fr);
-  table_load<KT, MODE>(a, m.z, m.z + n, tr);
+  table_load<KT, MODE, NT>(a, m.z, m.z + n, tr);
   return n;
 }
 
@@ -523,7 +542,8 @@ struct MetaLanes {
 // of at most C table groups; the first chunk is already in LDS, later chunks
 // of a large tile are loaded here. cur holds the group bounds of this slot's
 // first two segments.
-template <int F, int G, int MODE, bool VAR, uint32_t C, int KT, uint32_t SLOTS, class LT>
+template <int F, int G, int MODE, bool VAR, uint32_t C, int KT, uint32_t SLOTS, unsigned NT,
+          class LT>
 __device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m,
                                            const SegRegs<SegRounds<SLOTS>::R>& cur,
                                            uint32_t slot, uint32_t q, const LT* base, u32x2* tl,
@@ -549,8 +569,8 @@ __device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m,
       __syncthreads();
       TableRegs<KT, MODE> td;
       const uint32_t n = sg[sb] - gc;
-      table_load<KT, MODE>(a, gc, gc + n, td);
-      table_store<KT, MODE, C>(td, n, tl, twb);
+      table_load<KT, MODE, NT>(a, gc, gc + n, td);
+      table_store<KT, MODE, NT>(td, n, tl, twb);
       // Lanes that store nothing skip the store's wait; wait here on every
       // lane so no load of this rare path is still pending where it rejoins
       // the common path (the compiler would otherwise wait there, on the
@@ -597,15 +617,16 @@ __device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m,
 // duplicated across branches (which would push the kernel past its register
 // budget into spills that wait on the prefetch).
 template <int F, int G, int MODE, bool VAR, int FA>
-__global__ __launch_bounds__(kTileThreads)
-__attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) void k_tile(
+__global__ __launch_bounds__(tile_threads(F))
+__attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * tile_threads(F) / 256))) void k_tile(
     TileArgs a) {
   using LT = typename TAcc<MODE>::L;
   using PL = Plan<F, MODE>;
   constexpr uint32_t C = PL::kCap;
   constexpr int KT = PL::kKT;
   constexpr uint32_t L = F / G;
-  constexpr uint32_t SLOTS = kTileThreads / L;
+  constexpr unsigned NT = PL::kThreads;
+  constexpr uint32_t SLOTS = NT / L;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   LT* fl = reinterpret_cast<LT*>(lds_raw);
   u32x2* tl = reinterpret_cast<u32x2*>(lds_raw + PL::kFrameBytes);
@@ -622,7 +643,7 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
   ml.load(a, t0, grid, 0);
 
   if constexpr (FA != 0) {
-    FrameRegs<F> fr;
+    FrameRegs<F, NT> fr;
     TableRegs<KT, MODE> tr;
     SegRegs<SegRounds<SLOTS>::R> sr = {};
     uint32_t pn = 0;    // groups of the chunk in tr
@@ -637,13 +658,13 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
       if (work) {
         __syncthreads();  // the previous tile's compute is done with the LDS
         frames_store32<F, LT>(a, fr, fl);
-        table_store<KT, MODE, C>(tr, pn, tl, twb);
+        table_store<KT, MODE, NT>(tr, pn, tl, twb);
         __syncthreads();
       }
       if (k + 1 - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k + 1);
       const uint4 mnn = ml.get(k + 1);
-      if (more) pn = prefetch<F, FA, KT, MODE, C, SLOTS>(a, tn, mn, slot, fr, tr, sr);
-      if (work) tile_apply<F, G, MODE, VAR, C, KT, SLOTS>(a, m, cur, slot, q, base, tl, twb);
+      if (more) pn = prefetch<F, FA, KT, MODE, C, SLOTS, NT>(a, tn, mn, slot, fr, tr, sr);
+      if (work) tile_apply<F, G, MODE, VAR, C, KT, SLOTS, NT>(a, m, cur, slot, q, base, tl, twb);
       if (!more) break;
       m = mn;
       mn = mnn;
@@ -657,17 +678,17 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * kTileThreads / 256))) v
       if (m.x == m.y) continue;  // nothing of the table in this tile
       __syncthreads();
       switch (a.dtype) {
-        case DIO_U16: stage_direct<F, uint16_t, LT>(a, t, fl); break;
-        default: stage_direct<F, double, LT>(a, t, fl); break;
+        case DIO_U16: stage_direct<F, uint16_t, LT, NT>(a, t, fl); break;
+        default: stage_direct<F, double, LT, NT>(a, t, fl); break;
       }
       TableRegs<KT, MODE> td;
       const uint32_t n = min(m.w - m.z, C);
-      table_load<KT, MODE>(a, m.z, m.z + n, td);
-      table_store<KT, MODE, C>(td, n, tl, twb);
+      table_load<KT, MODE, NT>(a, m.z, m.z + n, td);
+      table_store<KT, MODE, NT>(td, n, tl, twb);
       SegRegs<SegRounds<SLOTS>::R> cur;
       seg_bounds<SLOTS>(a, m, slot, cur);
       __syncthreads();
-      tile_apply<F, G, MODE, VAR, C, KT, SLOTS>(a, m, cur, slot, q, base, tl, twb);
+      tile_apply<F, G, MODE, VAR, C, KT, SLOTS, NT>(a, m, cur, slot, q, base, tl, twb);
     }
   }
 }
@@ -755,7 +776,7 @@ void launch_tile(const TiledTable& t, const TileArgs& a, hipStream_t s) {
   }
   uint32_t grid = std::min<uint32_t>(t.ntiles, t.cus * PL::kWgs);
   if (t.max_wgs) grid = std::min(grid, t.max_wgs);
-  k_tile<F, G, MODE, VAR, FA><<<grid, kTileThreads, lds, s>>>(a);
+  k_tile<F, G, MODE, VAR, FA><<<grid, PL::kThreads, lds, s>>>(a);
 }
 
 template <int F, int G, int MODE>
