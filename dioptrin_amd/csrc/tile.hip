@@ -694,64 +694,129 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * tile_threads(F) / 256))
 }
 
 // prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =
-// s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0. A wavefront
-// serves 16 (bin, frame) pairs with 4 lanes each: lane j adds the bin's
-// partials j, j + 4, j + 8, ... in tile order, then the four lane sums are
-// combined as (s0 + s1) + (s2 + s3). The order depends only on the table.
-template <bool PLAIN, bool VAR, class PT>
+// s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0. A workgroup
+// serves one bin: 64 segment lanes times ceil(F / 4) frame quads. Segment lane
+// l adds the bin's partials l, l + 64, l + 128, ... (tile order), four frames
+// at a time with one vector load; the 64 lane sums are then combined by a
+// fixed pairwise tree (l with l ^ 1, then l ^ 2, ... l ^ 32), inside the
+// wavefront by lane exchange and across wavefronts through LDS. The order
+// depends only on the table, never on F or the batch.
+constexpr unsigned kFinalLanes = 64;
+
+template <class PT, int FPT> struct PartVec { PT v[FPT]; };
+
+template <class PT, int FPT>
+__device__ __forceinline__ PartVec<PT, FPT> load_part(const PT* p) {
+  PartVec<PT, FPT> r;
+  if constexpr (FPT == 4 && sizeof(PT) == 4) {
+    const f32x4 x = *reinterpret_cast<const f32x4*>(p);
+    r.v[0] = x.x; r.v[1] = x.y; r.v[2] = x.z; r.v[3] = x.w;
+  } else if constexpr (FPT == 4 && sizeof(PT) == 8) {
+    const f64x2 x = reinterpret_cast<const f64x2*>(p)[0], y = reinterpret_cast<const f64x2*>(p)[1];
+    r.v[0] = x.x; r.v[1] = x.y; r.v[2] = y.x; r.v[3] = y.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < FPT; k++) r.v[k] = p[k];
+  }
+  return r;
+}
+
+template <bool PLAIN, bool VAR, class PT, int FPT>
 __global__ __launch_bounds__(256) void k_tfinal(const PT* __restrict__ part, uint32_t nseg,
-                                                uint32_t F, uint32_t logF,
-                                                const uint32_t* __restrict__ bin_ptr,
+                                                uint32_t F, const uint32_t* __restrict__ bin_ptr,
                                                 const uint32_t* __restrict__ bin_seg,
                                                 const double* __restrict__ den, uint32_t nbins,
                                                 uint32_t nf, double* __restrict__ I,
                                                 double* __restrict__ S) {
   using AT = typename std::conditional<PLAIN, float, double>::type;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t j = lane >> 4, pair = lane & 15;
-  const uint64_t b = wave * (16u >> logF) + (pair >> logF);
-  const uint32_t f = pair & (F - 1);
-  const bool ok = b < nbins && f < nf;
-  AT s = 0, v = 0;
-  if (ok) {
-    const PT* pv = part + (size_t)nseg * F;
-    const uint32_t q1 = bin_ptr[b + 1];
-    uint32_t q = bin_ptr[b] + j;
-    // Eight partials of this lane in flight at a time, added in order.
-    for (; q + 28 < q1; q += 32) {
-      uint32_t g[8];
+  constexpr int NV = VAR ? 2 : 1;
+  __shared__ AT red[4][4][NV][FPT];  // [wavefront][quad][sum kind][frame]
+  const uint32_t quads = (F + 3) / 4;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t q = tid % quads, l = tid / quads;
+  const uint32_t b = blockIdx.x;
+  const uint32_t q0 = bin_ptr[b], q1 = bin_ptr[b + 1];
+  AT acc[NV][FPT];
 #pragma unroll
-      for (int k = 0; k < 8; k++) g[k] = bin_seg[q + 4 * k];
-      PT x[8], y[8];
+  for (int k = 0; k < FPT; k++) {
+    acc[0][k] = 0;
+    if constexpr (VAR) acc[NV - 1][k] = 0;
+  }
+  const PT* pv = part + (size_t)nseg * F;
+  // Partials of this lane, four loads in flight, added in order.
+  uint32_t i = q0 + l;
+  for (; i + 3 * kFinalLanes < q1; i += 4 * kFinalLanes) {
+    uint32_t g[4];
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        x[k] = part[(size_t)g[k] * F + f];
-        if constexpr (VAR) y[k] = pv[(size_t)g[k] * F + f];
-      }
+    for (int u = 0; u < 4; u++) g[u] = bin_seg[i + u * kFinalLanes];
+    PartVec<PT, FPT> x[4], y[4];
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        s = s + static_cast<AT>(x[k]);
-        if constexpr (VAR) v = v + static_cast<AT>(y[k]);
-      }
+    for (int u = 0; u < 4; u++) {
+      x[u] = load_part<PT, FPT>(part + (size_t)g[u] * F + q * 4);
+      if constexpr (VAR) y[u] = load_part<PT, FPT>(pv + (size_t)g[u] * F + q * 4);
     }
-    for (; q < q1; q += 4) {
-      const size_t g = (size_t)bin_seg[q] * F + f;
-      s = s + static_cast<AT>(part[g]);
-      if constexpr (VAR) v = v + static_cast<AT>(pv[g]);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int k = 0; k < FPT; k++) {
+        acc[0][k] = acc[0][k] + static_cast<AT>(x[u].v[k]);
+        if constexpr (VAR) acc[NV - 1][k] = acc[NV - 1][k] + static_cast<AT>(y[u].v[k]);
+      }
+  }
+  for (; i < q1; i += kFinalLanes) {
+    const size_t g = (size_t)bin_seg[i] * F + q * 4;
+    const PartVec<PT, FPT> x = load_part<PT, FPT>(part + g);
+    PartVec<PT, FPT> y;
+    if constexpr (VAR) y = load_part<PT, FPT>(pv + g);
+#pragma unroll
+    for (int k = 0; k < FPT; k++) {
+      acc[0][k] = acc[0][k] + static_cast<AT>(x.v[k]);
+      if constexpr (VAR) acc[NV - 1][k] = acc[NV - 1][k] + static_cast<AT>(y.v[k]);
     }
   }
-  s = s + __shfl_xor(s, 16);
-  s = s + __shfl_xor(s, 32);
-  if constexpr (VAR) {
-    v = v + __shfl_xor(v, 16);
-    v = v + __shfl_xor(v, 32);
+  // Tree over segment lanes: level j pairs l with l ^ 2^j, i.e. thread tid
+  // with tid ^ (quads << j); levels inside the wavefront by lane exchange.
+  uint32_t j = 0;
+  for (; j < 6 && (quads << j) < 64; j++) {
+    const int m = (int)(quads << j);
+#pragma unroll
+    for (int k = 0; k < FPT; k++) {
+      acc[0][k] = acc[0][k] + __shfl_xor(acc[0][k], m);
+      if constexpr (VAR) acc[NV - 1][k] = acc[NV - 1][k] + __shfl_xor(acc[NV - 1][k], m);
+    }
   }
-  if (ok && j == 0) {
-    const double d = den[b];
-    I[(uint64_t)f * nbins + b] = d == 0.0 ? 0.0 : (double)s / d;
+  const uint32_t waves = quads;  // 64 * quads threads
+  if (waves > 1) {
+    const uint32_t w = tid / 64;
+    if ((tid & 63) < quads) {
+#pragma unroll
+      for (int k = 0; k < FPT; k++) {
+        red[w][q][0][k] = acc[0][k];
+        if constexpr (VAR) red[w][q][NV - 1][k] = acc[NV - 1][k];
+      }
+    }
+    __syncthreads();
+    if (tid >= quads) return;
+    // Remaining levels pair wavefront w with w ^ 1, then w ^ 2.
+#pragma unroll
+    for (int k = 0; k < FPT; k++)
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        AT x = red[0][q][n][k] + red[1][q][n][k];
+        if (waves == 4) x = x + (red[2][q][n][k] + red[3][q][n][k]);
+        acc[n][k] = x;
+      }
+  } else if (tid >= quads) {
+    return;
+  }
+  const double d = den[b];
+#pragma unroll
+  for (int k = 0; k < FPT; k++) {
+    const uint32_t f = q * 4 + k;
+    if (f >= nf || f >= F) continue;
+    I[(uint64_t)f * nbins + b] = d == 0.0 ? 0.0 : (double)acc[0][k] / d;
     if constexpr (VAR) {
-      const double vv = d == 0.0 ? 0.0 : (double)v / (d * d);
+      const double vv = d == 0.0 ? 0.0 : (double)acc[NV - 1][k] / (d * d);
       S[(uint64_t)f * nbins + b] = sqrt(fmax(vv, 0.0));
     }
   }
@@ -829,13 +894,19 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
 }
 
 template <bool PLAIN, bool VAR, class PT>
-void launch_final(const TiledTable& t, const void* part, uint32_t F, uint32_t logF, uint32_t nf,
-                  double* I, double* S, hipStream_t s) {
-  const uint64_t waves = (t.nbins + (16u >> logF) - 1) / (16u >> logF);
-  const unsigned blocks = blocks_for(waves * 64, 256);
-  k_tfinal<PLAIN, VAR, PT><<<blocks, 256, 0, s>>>(static_cast<const PT*>(part), t.nseg, F, logF,
-                                                  t.bin_ptr.p, t.bin_seg.p, t.den, t.nbins, nf,
-                                                  I, S);
+void launch_final(const TiledTable& t, const void* part, uint32_t F, uint32_t nf, double* I,
+                  double* S, hipStream_t s) {
+  const unsigned threads = kFinalLanes * ((F + 3) / 4);
+  const PT* p = static_cast<const PT*>(part);
+  if (F >= 4)
+    k_tfinal<PLAIN, VAR, PT, 4><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, t.bin_ptr.p,
+                                                             t.bin_seg.p, t.den, t.nbins, nf, I, S);
+  else if (F == 2)
+    k_tfinal<PLAIN, VAR, PT, 2><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, t.bin_ptr.p,
+                                                             t.bin_seg.p, t.den, t.nbins, nf, I, S);
+  else
+    k_tfinal<PLAIN, VAR, PT, 1><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, t.bin_ptr.p,
+                                                             t.bin_seg.p, t.den, t.nbins, nf, I, S);
 }
 
 size_t dtype_bytes(int dtype) {
@@ -847,11 +918,6 @@ size_t dtype_bytes(int dtype) {
   }
 }
 
-uint32_t log2u(uint32_t v) {
-  uint32_t l = 0;
-  while ((1u << l) < v) l++;
-  return l;
-}
 
 }  // namespace
 
@@ -1000,7 +1066,6 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   sc.ensure(t, F, var);
   const char* fb = static_cast<const char*>(frames);
   const uint64_t npix = (uint64_t)t.dim1 * t.dim2;
-  const uint32_t logF = log2u(F);
   const bool plain = precision == DIO_F32, f64 = precision == DIO_F64;
   for (uint64_t f0 = 0; f0 < n_frames; f0 += F) {
     const uint32_t nf = (uint32_t)std::min<uint64_t>(F, n_frames - f0);
@@ -1009,14 +1074,14 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
     double* I = intensity + f0 * t.nbins;
     double* S = var ? sigma + f0 * t.nbins : nullptr;
     if (f64) {
-      if (var) launch_final<false, true, double>(t, sc.ptr(), F, logF, nf, I, S, s);
-      else launch_final<false, false, double>(t, sc.ptr(), F, logF, nf, I, S, s);
+      if (var) launch_final<false, true, double>(t, sc.ptr(), F, nf, I, S, s);
+      else launch_final<false, false, double>(t, sc.ptr(), F, nf, I, S, s);
     } else if (plain) {
-      if (var) launch_final<true, true, float>(t, sc.ptr(), F, logF, nf, I, S, s);
-      else launch_final<true, false, float>(t, sc.ptr(), F, logF, nf, I, S, s);
+      if (var) launch_final<true, true, float>(t, sc.ptr(), F, nf, I, S, s);
+      else launch_final<true, false, float>(t, sc.ptr(), F, nf, I, S, s);
     } else {
-      if (var) launch_final<false, true, float>(t, sc.ptr(), F, logF, nf, I, S, s);
-      else launch_final<false, false, float>(t, sc.ptr(), F, logF, nf, I, S, s);
+      if (var) launch_final<false, true, float>(t, sc.ptr(), F, nf, I, S, s);
+      else launch_final<false, false, float>(t, sc.ptr(), F, nf, I, S, s);
     }
     DIO_HIP(hipGetLastError());
   }
