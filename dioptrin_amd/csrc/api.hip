@@ -134,6 +134,9 @@ dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& des
       const long v = std::atol(e);
       if (v > 0) p->tt.max_wgs = (uint32_t)std::min<long>(v, 1L << 20);
     }
+    // DIOPTRIN_TILE_DB=0 selects the single-buffered kernel for 16-frame passes
+    // (comparison runs; results are the same bits either way).
+    if (const char* e = std::getenv("DIOPTRIN_TILE_DB")) p->tt.db = std::atol(e) != 0;
     if (desc.precision == DIO_F64) p->tile = std::min(p->tile, dio::max_frames_for(DIO_F64));
     p->st.row_ptr = p->b.full.row_ptr.p;
     p->st.col = p->b.full.col.p;

@@ -693,6 +693,168 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * tile_threads(F) / 256))
   }
 }
 
+// ---------------------------------------------------------------- double-buffered tile kernel
+
+// Double-buffered variant for 16-frame passes of 32-bit frames: one workgroup
+// of 1024 threads per CU holds two LDS tiles. While tile k is computed from
+// one buffer, tile k + 1 (loaded into registers during the previous
+// iteration) is staged into the other and tile k + 2 is loaded into the
+// registers, so the frame loads stay in flight across the whole iteration and
+// an iteration needs one barrier. Threads 0-511 stage the first half of the
+// frames, threads 512-1023 the second half, each one pixel pair of a row.
+constexpr unsigned kDbThreads = 1024;
+
+template <int F, int MODE> struct PlanDb {
+  using LT = typename TAcc<MODE>::L;
+  static constexpr size_t kFrameBytes = Plan<F, MODE>::kFrameBytes;
+  static constexpr size_t kWBytes = MODE == 2 ? 32 : 16;
+  static constexpr size_t kRoom = (80 * 1024 - kFrameBytes) / (8 + kWBytes);
+  static constexpr uint32_t kCap = kRoom >= 1024 ? 1024 : (uint32_t)(kRoom / 8 * 8);
+  static constexpr size_t kHalf = (kFrameBytes + (size_t)kCap * (8 + kWBytes) + 255) / 256 * 256;
+  static constexpr size_t kBytes = 2 * kHalf;
+  static_assert(kBytes <= 160 * 1024, "two LDS tiles exceed one CU");
+  static_assert(kCap >= kTilePixels / kGroup, "a whole segment must fit one table chunk");
+  static_assert(kCap <= kDbThreads, "one table group per thread");
+};
+
+template <int F> struct DbFrames {
+  uint32_t v[F / 2][2];
+};
+
+template <int F, int A>
+__device__ __forceinline__ void db_load(const TileArgs& a, uint32_t t, DbFrames<F>& fr) {
+  static_assert(A == 4 || A == 8, "pair loads are 4 or 8 bytes");
+  const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
+  const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
+  const uint32_t c = (t % a.ntx) * kTileCols + cp * 2;
+  const uint32_t r = (t / a.ntx) * kTileRows + r0;
+  const uint32_t nvalid = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
+  const uint32_t off = r * a.dim2 + c;
+  // Unconditional loads from valid addresses, as in frames_load32.
+  const uint32_t o0 = nvalid > 0 ? off * 4u : 0u, o1 = nvalid > 1 ? off * 4u + 4u : 0u;
+  const unsigned char* src = static_cast<const unsigned char*>(a.frames);
+  const uint32_t flast = a.nf - 1;
+#pragma unroll
+  for (int i = 0; i < F / 2; i++) {
+    const uint32_t f = half * (F / 2) + i;
+    const unsigned char* pf = src + (uint64_t)min(f, flast) * a.npix * 4u;
+    if constexpr (A == 8) {
+      const u32x2 v = load_at<u32x2>(pf, o0);
+      fr.v[i][0] = v.x;
+      fr.v[i][1] = v.y;
+    } else {
+      fr.v[i][0] = load_at<uint32_t>(pf, o0);
+      fr.v[i][1] = load_at<uint32_t>(pf, o1);
+    }
+  }
+}
+
+template <int F, class W, class LT>
+__device__ __forceinline__ void db_store_as(const DbFrames<F>& fr, LT* fl) {
+  const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
+  const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
+#pragma unroll
+  for (int i = 0; i < F / 2; i++) {
+    const uint32_t f = half * (F / 2) + i;
+    Lds2<LT> o;
+    o.v[0] = word_to<W, LT>(fr.v[i][0]);
+    o.v[1] = word_to<W, LT>(fr.v[i][1]);
+    *reinterpret_cast<Lds2<LT>*>(fl + f * kLdsStride + r0 * kTileCols + cp * 2) = o;
+  }
+}
+
+template <int F, class LT>
+__device__ __forceinline__ void db_store(const TileArgs& a, const DbFrames<F>& fr, LT* fl) {
+  if (a.dtype == DIO_U32) db_store_as<F, uint32_t, LT>(fr, fl);
+  else if (a.dtype == DIO_I32) db_store_as<F, int32_t, LT>(fr, fl);
+  else db_store_as<F, float, LT>(fr, fl);
+}
+
+template <int F, int A, int MODE, uint32_t C, uint32_t SLOTS>
+__device__ __forceinline__ uint32_t db_prefetch(const TileArgs& a, uint32_t tt, uint4 m,
+                                                uint32_t slot, DbFrames<F>& fr,
+                                                TableRegs<1, MODE>& tr,
+                                                SegRegs<SegRounds<SLOTS>::R>& sr) {
+  if (m.x == m.y) return 0;
+  const uint32_t n = min(m.w - m.z, C);
+  seg_bounds<SLOTS>(a, m, slot, sr);
+  db_load<F, A>(a, tt, fr);
+  table_load<1, MODE, kDbThreads>(a, m.z, m.z + n, tr);
+  return n;
+}
+
+template <int F, int G, int MODE, bool VAR, int FA>
+__global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_db(
+    TileArgs a) {
+  static_assert(FA == 4 || FA == 8, "32-bit frames only");
+  static_assert(F % 2 == 0, "frames split over two thread halves");
+  using LT = typename TAcc<MODE>::L;
+  using PL = PlanDb<F, MODE>;
+  constexpr uint32_t C = PL::kCap;
+  constexpr uint32_t L = F / G;
+  constexpr uint32_t SLOTS = kDbThreads / L;
+  constexpr int R = SegRounds<SLOTS>::R;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  LT* const fl0 = reinterpret_cast<LT*>(lds_raw);
+  LT* const fl1 = reinterpret_cast<LT*>(lds_raw + PL::kHalf);
+  const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
+
+  const uint32_t grid = gridDim.x;
+  const uint32_t t0 = first_tile();
+  if (t0 >= a.ntiles) return;
+  MetaLanes ml;
+  ml.load(a, t0, grid, 0);
+  pad_slots<F>(fl0);
+  pad_slots<F>(fl1);
+
+  DbFrames<F> fr;
+  TableRegs<1, MODE> tr;
+  SegRegs<R> sr = {};
+  // Prologue: tile 0 staged into buffer 0, tile 1 loaded into registers.
+  uint4 mA = ml.get(0);
+  uint32_t pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0, mA, slot, fr, tr, sr);
+  if (mA.x != mA.y) {
+    db_store<F, LT>(a, fr, fl0);
+    table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(lds_raw + PL::kFrameBytes),
+                                     lds_raw + PL::kFrameBytes + (size_t)C * 8);
+  }
+  SegRegs<R> cA = sr;
+  uint4 mB = ml.get(1);
+  if (t0 + grid < a.ntiles) pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0 + grid, mB, slot, fr, tr, sr);
+  __syncthreads();
+
+  for (uint32_t k = 0;; k++) {
+    if (t0 + k * grid >= a.ntiles) break;
+    const uint32_t b = k & 1;
+    unsigned char* const cur = lds_raw + (b ? PL::kHalf : 0);
+    unsigned char* const nxt = lds_raw + (b ? 0 : PL::kHalf);
+    // 1. Tile k + 1 (registers) into the other buffer, free since the last
+    //    barrier.
+    if (mB.x != mB.y) {
+      db_store<F, LT>(a, fr, reinterpret_cast<LT*>(nxt));
+      table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(nxt + PL::kFrameBytes),
+                                       nxt + PL::kFrameBytes + (size_t)C * 8);
+    }
+    const SegRegs<R> cB = sr;
+    // 2. Tile k + 2 into the registers.
+    if (k + 2 - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k + 2);
+    const uint4 mC = ml.get(k + 2);
+    if (t0 + (k + 2) * grid < a.ntiles)
+      pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0 + (k + 2) * grid, mC, slot, fr, tr, sr);
+    // 3. Tile k from its buffer.
+    if (mA.x != mA.y) {
+      const LT* base = reinterpret_cast<const LT*>(cur) + q * G * kLdsStride;
+      tile_apply<F, G, MODE, VAR, C, 1, SLOTS, kDbThreads>(
+          a, mA, cA, slot, q, base, reinterpret_cast<u32x2*>(cur + PL::kFrameBytes),
+          cur + PL::kFrameBytes + (size_t)C * 8);
+    }
+    __syncthreads();
+    mA = mB;
+    cA = cB;
+    mB = mC;
+  }
+}
+
 // prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =
 // s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0. A workgroup
 // serves one bin: 64 segment lanes times ceil(F / 4) frame quads. Segment lane
@@ -844,10 +1006,40 @@ void launch_tile(const TiledTable& t, const TileArgs& a, hipStream_t s) {
   k_tile<F, G, MODE, VAR, FA><<<grid, PL::kThreads, lds, s>>>(a);
 }
 
+template <int F, int G, int MODE, bool VAR, int FA>
+void launch_tile_db(const TiledTable& t, const TileArgs& a, hipStream_t s) {
+  using PL = PlanDb<F, MODE>;
+  constexpr size_t lds = PL::kBytes;
+  static std::atomic<uint64_t> done{0};
+  int dev = 0;
+  DIO_HIP(hipGetDevice(&dev));
+  const uint64_t bit = 1ull << (dev & 63);
+  if (!(done.load() & bit)) {
+    DIO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_db<F, G, MODE, VAR, FA>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    done.fetch_or(bit);
+  }
+  uint32_t grid = std::min<uint32_t>(t.ntiles, t.cus);
+  if (t.max_wgs) grid = std::min(grid, t.max_wgs);
+  k_tile_db<F, G, MODE, VAR, FA><<<grid, kDbThreads, lds, s>>>(a);
+}
+
 template <int F, int G, int MODE>
 void launch_v(const TiledTable& t, const TileArgs& a, bool var, hipStream_t s) {
   const bool w32 = a.dtype == DIO_U32 || a.dtype == DIO_I32 || a.dtype == DIO_F32T;
   const int fa = !w32 ? 0 : a.aligned8 ? 8 : 4;
+  if constexpr (F == 16 && MODE != 2) {
+    if (t.db && fa != 0) {
+      if (var) {
+        if (fa == 8) launch_tile_db<F, G, MODE, true, 8>(t, a, s);
+        else launch_tile_db<F, G, MODE, true, 4>(t, a, s);
+      } else {
+        if (fa == 8) launch_tile_db<F, G, MODE, false, 8>(t, a, s);
+        else launch_tile_db<F, G, MODE, false, 4>(t, a, s);
+      }
+      return;
+    }
+  }
   if (var) {
     if (fa == 8) launch_tile<F, G, MODE, true, 8>(t, a, s);
     else if (fa == 4) launch_tile<F, G, MODE, true, 4>(t, a, s);
@@ -886,6 +1078,7 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
     case 4 * 8 + 4: launch_m<4, 4>(t, a, var, precision, s); break;
     case 8 * 8 + 2: launch_m<8, 2>(t, a, var, precision, s); break;
     case 8 * 8 + 4: launch_m<8, 4>(t, a, var, precision, s); break;
+    case 16 * 8 + 1: launch_m<16, 1>(t, a, var, precision, s); break;
     case 16 * 8 + 2: launch_m<16, 2>(t, a, var, precision, s); break;
     case 16 * 8 + 4: launch_m<16, 4>(t, a, var, precision, s); break;
     default: fail(DIO_ERR_INVALID, "unsupported (frames per pass, frames per lane) pair");
@@ -927,7 +1120,7 @@ bool valid_tile(int precision, uint32_t F, uint32_t G) {
   if (F > max_frames_for(precision)) return false;
   switch (F * 8 + G) {
     case 1 * 8 + 1: case 2 * 8 + 2: case 4 * 8 + 4: case 8 * 8 + 2: case 8 * 8 + 4:
-    case 16 * 8 + 2: case 16 * 8 + 4:
+    case 16 * 8 + 1: case 16 * 8 + 2: case 16 * 8 + 4:
       return true;
     default:
       return false;

@@ -146,9 +146,10 @@ int dio_integrate_device(dio_plan *plan, const void *d_frames, int dtype, uint64
 
 /* Launch shape of the fused batch kernel (no reference counterpart; the
  * reference launches one frame per kernel, src/integration/cuda/device.rs:
- * 1460-1584): frames per pass (1..32, powers of two; 1..16 for DIO_F64) and
- * frames per lane (1, 2 or 4; the lanes sharing one table segment are
- * frames_per_tile / frames_per_lane). dio_plan_autotune picks both from a
+ * 1460-1584): frames per pass and frames per lane, one of the pairs (1, 1),
+ * (2, 2), (4, 4), (8, 2), (8, 4), (16, 1), (16, 2), (16, 4) (at most 8 frames
+ * per pass for DIO_F64); the lanes sharing one table segment are
+ * frames_per_tile / frames_per_lane. dio_plan_autotune picks both from a
  * short device sweep and stores them in the plan; frames_per_lane may be
  * null. Results do not depend on either choice. */
 int dio_plan_autotune(dio_plan *plan, int dtype, uint64_t n_frames_hint, uint32_t flags,

@@ -143,7 +143,7 @@ def test_compensated_beats_plain_f32(pilatus):
     assert errs["f32_wide"] <= 1e-6 and errs["f32_compensated"] <= 1e-6
 
 
-TILE_SHAPES = [(1, 1), (2, 2), (4, 4), (8, 2), (8, 4), (16, 2), (16, 4)]
+TILE_SHAPES = [(1, 1), (2, 2), (4, 4), (8, 2), (8, 4), (16, 1), (16, 2), (16, 4)]
 
 
 @pytest.mark.parametrize("precision", ["f32", "f32_compensated", "f32_wide", "f64"])
@@ -188,6 +188,32 @@ def test_capped_grid_many_tiles_per_workgroup(pilatus, dtype, monkeypatch):
     monkeypatch.setenv("DIOPTRIN_TILE_WORKGROUPS", "8")
     capped = run()
     for (i0, s0), (i1, s1) in zip(full, capped):
+        assert np.array_equal(i0, i1)
+        assert np.array_equal(s0, s1)
+
+
+@pytest.mark.parametrize("tile", [(16, 1), (16, 2), (16, 4)], ids=["G1", "G2", "G4"])
+def test_double_buffered_equals_single_buffered(pilatus, tile, monkeypatch):
+    """16-frame passes of 32-bit frames run the double-buffered kernel (one
+    1024-thread workgroup per CU, two LDS tiles); DIOPTRIN_TILE_DB=0 selects
+    the single-buffered one. Both give the same bits, with and without
+    errors, for u32 and f32 frames."""
+    d = pilatus
+    rng = np.random.default_rng(12)
+    frames = rng.poisson(120.0, size=(21,) + d["w"]["shape"])
+
+    def run():
+        plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
+                       polarization_factor=0.99, mask=d["mask"], precision="f32_wide")
+        plan.set_tile(*tile)
+        return [plan.integrate_stack(frames.astype(dt), errors=True)[:2]
+                for dt in (np.uint32, np.float32)]
+
+    monkeypatch.delenv("DIOPTRIN_TILE_DB", raising=False)
+    db = run()
+    monkeypatch.setenv("DIOPTRIN_TILE_DB", "0")
+    single = run()
+    for (i0, s0), (i1, s1) in zip(db, single):
         assert np.array_equal(i0, i1)
         assert np.array_equal(s0, s1)
 
