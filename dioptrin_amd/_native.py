@@ -22,12 +22,12 @@ WITH_ERRORS, WITH_AZIMUTHAL_STD, DEVICE_FRAMES = 1, 2, 4
 EXPORTS = [
     "dio_last_error", "dio_version", "dio_open", "dio_close", "dio_device_info_get",
     "dio_plan_create", "dio_plan_destroy", "dio_plan_info_get", "dio_plan_radial",
-    "dio_plan_azimuthal", "dio_plan_export_csr", "dio_plan_denominators",
-    "dio_plan_normalization", "dio_integrate", "dio_integrate_device", "dio_plan_autotune",
-    "dio_plan_set_tile", "dio_graph_create", "dio_graph_launch", "dio_graph_destroy",
-    "dio_plan_time_gather", "dio_device_alloc", "dio_device_free", "dio_memcpy",
-    "dio_synchronize", "dio_synthetic_poisson_u32", "dio_server_create", "dio_server_submit",
-    "dio_server_poll", "dio_server_stats_get", "dio_server_destroy",
+    "dio_plan_azimuthal", "dio_plan_export_csr", "dio_plan_denominators", "dio_plan_normalization",
+    "dio_integrate", "dio_integrate_device", "dio_plan_autotune", "dio_plan_set_tile",
+    "dio_graph_create", "dio_graph_launch", "dio_graph_destroy", "dio_plan_time_gather",
+    "dio_plan_time_passes", "dio_device_alloc", "dio_device_free", "dio_memcpy", "dio_synchronize",
+    "dio_synthetic_poisson_u32", "dio_server_create", "dio_server_submit", "dio_server_poll",
+    "dio_server_stats_get", "dio_server_destroy",
 ]
 
 
@@ -125,6 +125,8 @@ def lib() -> C.CDLL:
             "dio_graph_destroy": (i32, [vp]),
             "dio_plan_time_gather": (i32, [vp, vp, i32, u64, u32, dp, dp, i32,
                                            C.POINTER(C.c_double)]),
+            "dio_plan_time_passes": (i32, [vp, vp, i32, u64, u32, dp, dp, C.POINTER(C.c_float),
+                                           u32, C.POINTER(u32)]),
             "dio_device_alloc": (i32, [vp, u64, C.POINTER(vp)]),
             "dio_device_free": (i32, [vp, vp]),
             "dio_memcpy": (i32, [vp, vp, vp, u64, i32]),

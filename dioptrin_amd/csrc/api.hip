@@ -88,7 +88,7 @@ namespace dio {
 // Enqueue the whole integration of device frames on `s`.
 void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
                 uint64_t n_frames, uint32_t flags, double* I, double* S, double* SD,
-                hipStream_t s) {
+                hipStream_t s, const hipEvent_t* pass_events) {
   bool var = flags & DIO_WITH_ERRORS;
   if (flags & DIO_WITH_AZIMUTHAL_STD) {
     require(p->desc.ndim == 1, DIO_ERR_UNSUPPORTED, "azimuthal std is defined for 1D plans");
@@ -98,7 +98,7 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
   }
   require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
   tiled_gather(p->tt, sc, frames, dtype, n_frames, var, p->desc.precision, p->tile, p->lanes, I,
-               S, s);
+               S, s, pass_events);
 }
 
 }  // namespace dio
@@ -459,6 +459,37 @@ int dio_graph_launch(dio_graph* g, void* stream) {
 
 int dio_graph_destroy(dio_graph* g) {
   return guarded([&] { delete g; });
+}
+
+int dio_plan_time_passes(dio_plan* p, const void* d_frames, int dtype, uint64_t n_frames,
+                         uint32_t flags, double* d_intensity, double* d_sigma, float* ms,
+                         uint32_t cap, uint32_t* n_passes) {
+  return guarded([&] {
+    require(p && d_frames && d_intensity && n_frames > 0, DIO_ERR_INVALID, "null argument");
+    require(!(flags & DIO_WITH_AZIMUTHAL_STD), DIO_ERR_UNSUPPORTED,
+            "pass timing covers the tiled gather only");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    const uint64_t passes = (n_frames + p->tile - 1) / p->tile;
+    std::vector<hipEvent_t> ev(2 * passes, nullptr);
+    struct Events {
+      std::vector<hipEvent_t>& v;
+      ~Events() {
+        for (hipEvent_t e : v)
+          if (e) (void)hipEventDestroy(e);
+      }
+    } guard{ev};
+    for (hipEvent_t& e : ev) DIO_HIP(hipEventCreate(&e));
+    run_device(p, p->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, nullptr,
+               p->stream, ev.data());
+    DIO_HIP(hipEventSynchronize(ev.back()));
+    if (n_passes) *n_passes = (uint32_t)passes;
+    for (uint64_t k = 0; k < passes && k < cap; k++) {
+      float t = 0.f;
+      DIO_HIP(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+      ms[k] = t;
+    }
+  });
 }
 
 int dio_plan_time_gather(dio_plan* p, const void* d_frames, int dtype, uint64_t n_frames,

@@ -1252,7 +1252,8 @@ void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, i
 
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t F, uint32_t G,
-                  double* intensity, double* sigma, hipStream_t s) {
+                  double* intensity, double* sigma, hipStream_t s,
+                  const hipEvent_t* pass_events) {
   if (!valid_tile(precision, F, G))
     fail(DIO_ERR_INVALID, "invalid frames per pass / frames per lane");
   const size_t esz = dtype_bytes(dtype);
@@ -1262,8 +1263,11 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   const bool plain = precision == DIO_F32, f64 = precision == DIO_F64;
   for (uint64_t f0 = 0; f0 < n_frames; f0 += F) {
     const uint32_t nf = (uint32_t)std::min<uint64_t>(F, n_frames - f0);
+    const uint64_t pass = f0 / F;
+    if (pass_events) DIO_HIP(hipEventRecord(pass_events[2 * pass], s));
     if (t.nseg)
       launch_pass(t, F, G, fb + f0 * npix * esz, dtype, nf, var, precision, sc.ptr(), s);
+    if (pass_events) DIO_HIP(hipEventRecord(pass_events[2 * pass + 1], s));
     double* I = intensity + f0 * t.nbins;
     double* S = var ? sigma + f0 * t.nbins : nullptr;
     if (f64) {

@@ -79,9 +79,12 @@ uint32_t max_frames_for(int precision);
 // Valid (frames per pass, frames per lane) pair for this precision.
 bool valid_tile(int precision, uint32_t frames, uint32_t lanes);
 
+// pass_events (optional): 2 per pass, recorded before and after each pass's
+// tile kernel.
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t frames_per_pass,
-                  uint32_t frames_per_lane, double* intensity, double* sigma, hipStream_t s);
+                  uint32_t frames_per_lane, double* intensity, double* sigma, hipStream_t s,
+                  const hipEvent_t* pass_events = nullptr);
 
 // Only the tile kernel for one pass of nf <= frames_per_pass frames (timing).
 void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,

@@ -170,6 +170,13 @@ int dio_graph_destroy(dio_graph *graph);
 int dio_plan_time_gather(dio_plan *plan, const void *d_frames, int dtype, uint64_t n_frames,
                          uint32_t flags, double *d_intensity, double *d_sigma, int reps,
                          double *avg_ms);
+/* Runs one dio_integrate_device of the whole batch with HIP events recorded
+ * on the plan's stream around each pass's gather kernel, waits, and returns
+ * the kernel durations (ms) of the passes in order: min(cap, passes) values,
+ * the pass count in *n_passes. Measurement only (no reference counterpart). */
+int dio_plan_time_passes(dio_plan *plan, const void *d_frames, int dtype, uint64_t n_frames,
+                         uint32_t flags, double *d_intensity, double *d_sigma, float *ms,
+                         uint32_t cap, uint32_t *n_passes);
 
 /* Device memory and synchronisation for callers without their own device
  * allocator (the reference's plans own their device buffers the same way,
