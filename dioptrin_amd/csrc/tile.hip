@@ -856,14 +856,19 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
 }
 
 // prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =
-// s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0. A workgroup
-// serves one bin: 64 segment lanes times ceil(F / 4) frame quads. Segment lane
-// l adds the bin's partials l, l + 64, l + 128, ... (tile order), four frames
-// at a time with one vector load; the 64 lane sums are then combined by a
-// fixed pairwise tree (l with l ^ 1, then l ^ 2, ... l ^ 32), inside the
-// wavefront by lane exchange and across wavefronts through LDS. The order
-// depends only on the table, never on F or the batch.
+// s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0. One launch
+// reduces nf frames of consecutive passes: pass j's partials are the block at
+// part + j * BS, rows of FS frames. A workgroup serves one bin: 64 segment
+// lanes times Q frame quads of FPT frames (Q = ceil(nf / FPT) rounded up to a
+// power of two, so a wavefront holds whole segment lanes; quads past nf read
+// columns that are not stored, and no quad straddles two passes as FPT
+// divides FS). Segment lane l adds the bin's partials l, l + 64, l + 128, ... (tile
+// order), FPT frames at a time with one vector load; the 64 lane sums are then
+// combined by a fixed pairwise tree (l with l ^ 1, then l ^ 2, ... l ^ 32),
+// inside the wavefront by lane exchange and across wavefronts through LDS.
+// The order depends only on the table, never on the pass or batch shape.
 constexpr unsigned kFinalLanes = 64;
+constexpr unsigned kFinalMaxQuads = 8;  // at most 32 frames per launch
 
 template <class PT, int FPT> struct PartVec { PT v[FPT]; };
 
@@ -884,16 +889,15 @@ __device__ __forceinline__ PartVec<PT, FPT> load_part(const PT* p) {
 }
 
 template <bool PLAIN, bool VAR, class PT, int FPT>
-__global__ __launch_bounds__(256) void k_tfinal(const PT* __restrict__ part, uint32_t nseg,
-                                                uint32_t F, const uint32_t* __restrict__ bin_ptr,
-                                                const uint32_t* __restrict__ bin_seg,
-                                                const double* __restrict__ den, uint32_t nbins,
-                                                uint32_t nf, double* __restrict__ I,
-                                                double* __restrict__ S) {
+__global__ __launch_bounds__(kFinalLanes * kFinalMaxQuads) void k_tfinal(
+    const PT* __restrict__ part, uint32_t nseg, uint32_t FS, uint64_t BS,
+    const uint32_t* __restrict__ bin_ptr,
+    const uint32_t* __restrict__ bin_seg, const double* __restrict__ den, uint32_t nbins,
+    uint32_t nf, double* __restrict__ I, double* __restrict__ S) {
   using AT = typename std::conditional<PLAIN, float, double>::type;
   constexpr int NV = VAR ? 2 : 1;
-  __shared__ AT red[4][4][NV][FPT];  // [wavefront][quad][sum kind][frame]
-  const uint32_t quads = (F + 3) / 4;
+  __shared__ AT red[kFinalMaxQuads][kFinalMaxQuads][NV][FPT];  // [wavefront][quad][kind][frame]
+  const uint32_t quads = blockDim.x / kFinalLanes;
   const uint32_t tid = threadIdx.x;
   const uint32_t q = tid % quads, l = tid / quads;
   const uint32_t b = blockIdx.x;
@@ -904,42 +908,41 @@ __global__ __launch_bounds__(256) void k_tfinal(const PT* __restrict__ part, uin
     acc[0][k] = 0;
     if constexpr (VAR) acc[NV - 1][k] = 0;
   }
-  const PT* pv = part + (size_t)nseg * F;
-  // Partials of this lane, four loads in flight, added in order.
-  uint32_t i = q0 + l;
-  for (; i + 3 * kFinalLanes < q1; i += 4 * kFinalLanes) {
-    uint32_t g[4];
+  const uint32_t c0 = q * FPT;
+  const PT* pr = part + (uint64_t)(c0 / FS) * BS + c0 % FS;
+  const PT* pv = pr + (size_t)nseg * FS;
+  // Partials of this lane (i = q0 + l, q0 + l + 64, ...), added in order:
+  // U segment indices are read at once, then the U partial rows they name,
+  // so a bin of up to 64 U segments costs two memory round trips.
+  constexpr int U = sizeof(PT) == 8 ? 4 : 8;
+  for (uint32_t i0 = q0 + l; i0 < q1; i0 += U * kFinalLanes) {
+    uint32_t g[U];
 #pragma unroll
-    for (int u = 0; u < 4; u++) g[u] = bin_seg[i + u * kFinalLanes];
-    PartVec<PT, FPT> x[4], y[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      x[u] = load_part<PT, FPT>(part + (size_t)g[u] * F + q * 4);
-      if constexpr (VAR) y[u] = load_part<PT, FPT>(pv + (size_t)g[u] * F + q * 4);
+    for (int u = 0; u < U; u++) {
+      const uint32_t i = i0 + u * kFinalLanes;
+      g[u] = i < q1 ? bin_seg[i] : 0u;
     }
+    PartVec<PT, FPT> x[U], y[U];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int k = 0; k < FPT; k++) {
-        acc[0][k] = acc[0][k] + static_cast<AT>(x[u].v[k]);
-        if constexpr (VAR) acc[NV - 1][k] = acc[NV - 1][k] + static_cast<AT>(y[u].v[k]);
+    for (int u = 0; u < U; u++) {
+      if (i0 + u * kFinalLanes < q1) {
+        x[u] = load_part<PT, FPT>(pr + (size_t)g[u] * FS);
+        if constexpr (VAR) y[u] = load_part<PT, FPT>(pv + (size_t)g[u] * FS);
       }
-  }
-  for (; i < q1; i += kFinalLanes) {
-    const size_t g = (size_t)bin_seg[i] * F + q * 4;
-    const PartVec<PT, FPT> x = load_part<PT, FPT>(part + g);
-    PartVec<PT, FPT> y;
-    if constexpr (VAR) y = load_part<PT, FPT>(pv + g);
-#pragma unroll
-    for (int k = 0; k < FPT; k++) {
-      acc[0][k] = acc[0][k] + static_cast<AT>(x.v[k]);
-      if constexpr (VAR) acc[NV - 1][k] = acc[NV - 1][k] + static_cast<AT>(y.v[k]);
     }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (i0 + u * kFinalLanes < q1) {
+#pragma unroll
+        for (int k = 0; k < FPT; k++) {
+          acc[0][k] = acc[0][k] + static_cast<AT>(x[u].v[k]);
+          if constexpr (VAR) acc[NV - 1][k] = acc[NV - 1][k] + static_cast<AT>(y[u].v[k]);
+        }
+      }
   }
   // Tree over segment lanes: level j pairs l with l ^ 2^j, i.e. thread tid
   // with tid ^ (quads << j); levels inside the wavefront by lane exchange.
-  uint32_t j = 0;
-  for (; j < 6 && (quads << j) < 64; j++) {
+  for (uint32_t j = 0; j < 6 && (quads << j) < 64; j++) {
     const int m = (int)(quads << j);
 #pragma unroll
     for (int k = 0; k < FPT; k++) {
@@ -959,14 +962,16 @@ __global__ __launch_bounds__(256) void k_tfinal(const PT* __restrict__ part, uin
     }
     __syncthreads();
     if (tid >= quads) return;
-    // Remaining levels pair wavefront w with w ^ 1, then w ^ 2.
+    // Remaining levels pair wavefront w with w ^ 1, then w ^ 2, then w ^ 4.
 #pragma unroll
     for (int k = 0; k < FPT; k++)
 #pragma unroll
       for (int n = 0; n < NV; n++) {
-        AT x = red[0][q][n][k] + red[1][q][n][k];
-        if (waves == 4) x = x + (red[2][q][n][k] + red[3][q][n][k]);
-        acc[n][k] = x;
+        AT x[kFinalMaxQuads];
+        for (uint32_t w2 = 0; w2 < waves; w2++) x[w2] = red[w2][q][n][k];
+        for (uint32_t h = 1; h < waves; h *= 2)
+          for (uint32_t w2 = 0; w2 < waves; w2 += 2 * h) x[w2] = x[w2] + x[w2 + h];
+        acc[n][k] = x[0];
       }
   } else if (tid >= quads) {
     return;
@@ -974,8 +979,8 @@ __global__ __launch_bounds__(256) void k_tfinal(const PT* __restrict__ part, uin
   const double d = den[b];
 #pragma unroll
   for (int k = 0; k < FPT; k++) {
-    const uint32_t f = q * 4 + k;
-    if (f >= nf || f >= F) continue;
+    const uint32_t f = q * FPT + k;
+    if (f >= nf) continue;
     I[(uint64_t)f * nbins + b] = d == 0.0 ? 0.0 : (double)acc[0][k] / d;
     if constexpr (VAR) {
       const double vv = d == 0.0 ? 0.0 : (double)acc[NV - 1][k] / (d * d);
@@ -1086,19 +1091,25 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
   DIO_HIP(hipGetLastError());
 }
 
+// Reduces nf frames of consecutive passes of F frames each; pass j's
+// partials are the block at part + j * BS (elements).
 template <bool PLAIN, bool VAR, class PT>
-void launch_final(const TiledTable& t, const void* part, uint32_t F, uint32_t nf, double* I,
-                  double* S, hipStream_t s) {
-  const unsigned threads = kFinalLanes * ((F + 3) / 4);
+void launch_final(const TiledTable& t, const void* part, uint32_t F, uint64_t BS, uint32_t nf,
+                  double* I, double* S, hipStream_t s) {
   const PT* p = static_cast<const PT*>(part);
-  if (F >= 4)
-    k_tfinal<PLAIN, VAR, PT, 4><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, t.bin_ptr.p,
+  const uint32_t fpt = std::min(F, 4u);  // F is a power of two
+  unsigned quads = 1;
+  while (quads * fpt < nf) quads *= 2;
+  if (quads > kFinalMaxQuads) fail(DIO_ERR_INVALID, "too many frames for one reduction");
+  const unsigned threads = kFinalLanes * quads;
+  if (fpt == 4)
+    k_tfinal<PLAIN, VAR, PT, 4><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
                                                              t.bin_seg.p, t.den, t.nbins, nf, I, S);
-  else if (F == 2)
-    k_tfinal<PLAIN, VAR, PT, 2><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, t.bin_ptr.p,
+  else if (fpt == 2)
+    k_tfinal<PLAIN, VAR, PT, 2><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
                                                              t.bin_seg.p, t.den, t.nbins, nf, I, S);
   else
-    k_tfinal<PLAIN, VAR, PT, 1><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, t.bin_ptr.p,
+    k_tfinal<PLAIN, VAR, PT, 1><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
                                                              t.bin_seg.p, t.den, t.nbins, nf, I, S);
 }
 
@@ -1261,24 +1272,35 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   const char* fb = static_cast<const char*>(frames);
   const uint64_t npix = (uint64_t)t.dim1 * t.dim2;
   const bool plain = precision == DIO_F32, f64 = precision == DIO_F64;
-  for (uint64_t f0 = 0; f0 < n_frames; f0 += F) {
-    const uint32_t nf = (uint32_t)std::min<uint64_t>(F, n_frames - f0);
-    const uint64_t pass = f0 / F;
-    if (pass_events) DIO_HIP(hipEventRecord(pass_events[2 * pass], s));
-    if (t.nseg)
-      launch_pass(t, F, G, fb + f0 * npix * esz, dtype, nf, var, precision, sc.ptr(), s);
-    if (pass_events) DIO_HIP(hipEventRecord(pass_events[2 * pass + 1], s));
-    double* I = intensity + f0 * t.nbins;
-    double* S = var ? sigma + f0 * t.nbins : nullptr;
+  // Float32 partials of two passes fit the scratch (F float64 values per
+  // segment and kind), so one reduction launch serves both passes.
+  const uint32_t P = f64 ? 1 : 2;
+  const uint32_t FR = F * P;
+  const uint64_t BS = (uint64_t)t.nseg * F * (var ? 2 : 1);  // elements per pass block
+  const size_t pbytes = BS * (f64 ? sizeof(double) : sizeof(float));
+  for (uint64_t g0 = 0; g0 < n_frames; g0 += FR) {
+    const uint32_t ng = (uint32_t)std::min<uint64_t>(FR, n_frames - g0);
+    for (uint32_t j = 0; j * F < ng; j++) {
+      const uint64_t f0 = g0 + (uint64_t)j * F;
+      const uint32_t nf = std::min(F, ng - j * F);
+      const uint64_t pass = f0 / F;
+      if (pass_events) DIO_HIP(hipEventRecord(pass_events[2 * pass], s));
+      if (t.nseg)
+        launch_pass(t, F, G, fb + f0 * npix * esz, dtype, nf, var, precision,
+                    reinterpret_cast<char*>(sc.ptr()) + j * pbytes, s);
+      if (pass_events) DIO_HIP(hipEventRecord(pass_events[2 * pass + 1], s));
+    }
+    double* I = intensity + g0 * t.nbins;
+    double* S = var ? sigma + g0 * t.nbins : nullptr;
     if (f64) {
-      if (var) launch_final<false, true, double>(t, sc.ptr(), F, nf, I, S, s);
-      else launch_final<false, false, double>(t, sc.ptr(), F, nf, I, S, s);
+      if (var) launch_final<false, true, double>(t, sc.ptr(), F, BS, ng, I, S, s);
+      else launch_final<false, false, double>(t, sc.ptr(), F, BS, ng, I, S, s);
     } else if (plain) {
-      if (var) launch_final<true, true, float>(t, sc.ptr(), F, nf, I, S, s);
-      else launch_final<true, false, float>(t, sc.ptr(), F, nf, I, S, s);
+      if (var) launch_final<true, true, float>(t, sc.ptr(), F, BS, ng, I, S, s);
+      else launch_final<true, false, float>(t, sc.ptr(), F, BS, ng, I, S, s);
     } else {
-      if (var) launch_final<false, true, float>(t, sc.ptr(), F, nf, I, S, s);
-      else launch_final<false, false, float>(t, sc.ptr(), F, nf, I, S, s);
+      if (var) launch_final<false, true, float>(t, sc.ptr(), F, BS, ng, I, S, s);
+      else launch_final<false, false, float>(t, sc.ptr(), F, BS, ng, I, S, s);
     }
     DIO_HIP(hipGetLastError());
   }

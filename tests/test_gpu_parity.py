@@ -155,7 +155,9 @@ def test_batch_equals_single_frame_bitwise(pilatus, tile, precision):
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
                    polarization_factor=0.99, mask=d["mask"], precision=precision)
     rng = np.random.default_rng(5)
-    frames = rng.poisson(200.0, size=(37,) + d["w"]["shape"]).astype(np.float32)
+    # 53 = 32 + 21 frames: float32 partials of two passes share one reduction,
+    # so the second pair holds a full pass and a 5-frame one.
+    frames = rng.poisson(200.0, size=(53,) + d["w"]["shape"]).astype(np.float32)
     plan.set_tile(1, 1)
     single = [plan.integrate_stack(frames[k:k + 1], errors=True) for k in range(frames.shape[0])]
     plan.set_tile(*tile)
