@@ -528,6 +528,9 @@ struct MetaLanes {
     const uint32_t i = t0 + (k + threadIdx.x % kMetaLanes) * grid;
     const u32x4 z = {0u, 0u, 0u, 0u};
     v = i < a.ntiles ? reinterpret_cast<const u32x4*>(a.tile_meta)[i] : z;
+    // Wait here, on this rare path: a wait where get() reads the lanes would
+    // also wait for every store and load issued before it.
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
   }
   __device__ __forceinline__ uint4 get(uint32_t k) const {  // k0 <= k < k0 + 64
     const int l = (int)(k - k0);
@@ -538,16 +541,37 @@ struct MetaLanes {
   }
 };
 
+// Partials of the segments a slot takes in the first R rounds of a tile,
+// held in registers (seg = ~0u: none) and written later.
+template <int R, int G, int MODE, bool VAR> struct Pending {
+  uint32_t seg[R];
+  SegVals<G, MODE, VAR> vals[R];
+  __device__ void clear() {
+#pragma unroll
+    for (int k = 0; k < R; k++) seg[k] = ~0u;
+  }
+};
+
+template <int F, int R, int G, int MODE, bool VAR>
+__device__ __forceinline__ void pending_write(const TileArgs& a, uint32_t q,
+                                              Pending<R, G, MODE, VAR>& pd) {
+#pragma unroll
+  for (int k = 0; k < R; k++)
+    if (pd.seg[k] != ~0u) seg_write<F, G, MODE, VAR>(a, pd.seg[k], q, pd.vals[k]);
+  pd.clear();
+}
+
 // Computes one staged tile (m = its tile_meta): segments [s0, s1) in chunks
 // of at most C table groups; the first chunk is already in LDS, later chunks
 // of a large tile are loaded here. cur holds the group bounds of this slot's
-// first two segments.
+// first R segments. With pd, the partials of those R segments are left in
+// pd (the caller writes them) instead of being stored here.
 template <int F, int G, int MODE, bool VAR, uint32_t C, int KT, uint32_t SLOTS, unsigned NT,
           class LT>
-__device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m,
-                                           const SegRegs<SegRounds<SLOTS>::R>& cur,
-                                           uint32_t slot, uint32_t q, const LT* base, u32x2* tl,
-                                           unsigned char* twb) {
+__device__ __forceinline__ void tile_apply(
+    const TileArgs& a, uint4 m, const SegRegs<SegRounds<SLOTS>::R>& cur, uint32_t slot,
+    uint32_t q, const LT* base, u32x2* tl, unsigned char* twb,
+    Pending<SegRounds<SLOTS>::R, G, MODE, VAR>* pd = nullptr) {
   // Chunk bounds are workgroup-uniform: read them through the scalar cache so
   // that waiting for them never waits on the vector loads in flight.
   typedef const __attribute__((address_space(4))) uint32_t CWord;
@@ -593,7 +617,19 @@ __device__ __forceinline__ void tile_apply(const TileArgs& a, uint4 m,
           hi = r == (uint32_t)k ? cur.hi[k] : hi;
         }
         const uint32_t s_r = sa + seg_of_slot<SLOTS>(slot, r);
-        if (s_r < sb) seg_apply<F, G, MODE, VAR>(a, base, tl, twb, lo - gc, hi - gc, s_r, q);
+        if (s_r < sb) {
+          if (pd) {
+            const SegVals<G, MODE, VAR> v = seg_eval<G, MODE, VAR>(base, tl, twb, lo - gc, hi - gc);
+#pragma unroll
+            for (int k = 0; k < R; k++)
+              if (r == (uint32_t)k) {
+                pd->seg[k] = s_r;
+                pd->vals[k] = v;
+              }
+          } else {
+            seg_apply<F, G, MODE, VAR>(a, base, tl, twb, lo - gc, hi - gc, s_r, q);
+          }
+        }
       }
       r0 = R;
     }
@@ -702,6 +738,10 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * tile_threads(F) / 256))
 // registers, so the frame loads stay in flight across the whole iteration and
 // an iteration needs one barrier. Threads 0-511 stage the first half of the
 // frames, threads 512-1023 the second half, each one pixel pair of a row.
+// A tile's partials (those of its first two rounds of segments) are written
+// in the next iteration, after the wait for the staged frames and before the
+// next loads: the memory counter also counts stores, so stores issued
+// between a load and its wait would be waited for too.
 constexpr unsigned kDbThreads = 1024;
 
 template <int F, int MODE> struct PlanDb {
@@ -810,6 +850,8 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   DbFrames<F> fr;
   TableRegs<1, MODE> tr;
   SegRegs<R> sr = {};
+  Pending<R, G, MODE, VAR> pd;
+  pd.clear();
   // Prologue: tile 0 staged into buffer 0, tile 1 loaded into registers.
   uint4 mA = ml.get(0);
   uint32_t pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0, mA, slot, fr, tr, sr);
@@ -836,23 +878,30 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
                                        nxt + PL::kFrameBytes + (size_t)C * 8);
     }
     const SegRegs<R> cB = sr;
-    // 2. Tile k + 2 into the registers.
+    // Every load of the last prefetch has landed on every path (the store
+    // above waited for them): said explicitly, so that no later wait in this
+    // iteration also waits for the partial stores issued next.
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    // 2. Partials of tile k - 1.
+    pending_write<F>(a, q, pd);
+    // 3. Tile k + 2 into the registers.
     if (k + 2 - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k + 2);
     const uint4 mC = ml.get(k + 2);
     if (t0 + (k + 2) * grid < a.ntiles)
       pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0 + (k + 2) * grid, mC, slot, fr, tr, sr);
-    // 3. Tile k from its buffer.
+    // 4. Tile k from its buffer.
     if (mA.x != mA.y) {
       const LT* base = reinterpret_cast<const LT*>(cur) + q * G * kLdsStride;
       tile_apply<F, G, MODE, VAR, C, 1, SLOTS, kDbThreads>(
           a, mA, cA, slot, q, base, reinterpret_cast<u32x2*>(cur + PL::kFrameBytes),
-          cur + PL::kFrameBytes + (size_t)C * 8);
+          cur + PL::kFrameBytes + (size_t)C * 8, &pd);
     }
     __syncthreads();
     mA = mB;
     cA = cB;
     mB = mC;
   }
+  pending_write<F>(a, q, pd);
 }
 
 // prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =

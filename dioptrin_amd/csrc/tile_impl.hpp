@@ -135,12 +135,34 @@ template <int G, int MODE> struct SegGroups {
   static constexpr int P = 1;
 };
 
-// Applies segment s (table groups [ga, gb) of the LDS chunk) to this lane's G
-// frames and writes its partials.
-template <int F, int G, int MODE, bool VAR, class LT>
-__device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, const u32x2* tl,
-                                          const unsigned char* twb, uint32_t ga, uint32_t gb,
-                                          uint32_t s, uint32_t q) {
+// One segment's partials of this lane's G frames (and their variance terms).
+template <int G, int MODE, bool VAR> struct SegVals {
+  using PT = typename TAcc<MODE>::P;
+  PT v[G];
+  PT e[VAR ? G : 1];
+};
+
+// Writes segment s's partials of this lane's frames.
+template <int F, int G, int MODE, bool VAR>
+__device__ __forceinline__ void seg_write(const TileArgs& a, uint32_t s, uint32_t q,
+                                          const SegVals<G, MODE, VAR>& o) {
+  using PT = typename TAcc<MODE>::P;
+  PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * F + q * G;
+#pragma unroll
+  for (int j = 0; j < G; j++) {
+    if (q * G + j < a.nf) {
+      out[j] = o.v[j];
+      if constexpr (VAR) out[(size_t)a.nseg * F + j] = o.e[j];
+    }
+  }
+}
+
+// Applies a segment (table groups [ga, gb) of the LDS chunk) to this lane's G
+// frames.
+template <int G, int MODE, bool VAR, class LT>
+__device__ __forceinline__ SegVals<G, MODE, VAR> seg_eval(const LT* base, const u32x2* tl,
+                                                          const unsigned char* twb, uint32_t ga,
+                                                          uint32_t gb) {
   using Acc = typename TAcc<MODE>::T;
   using PT = typename TAcc<MODE>::P;
   Acc acc[G], var[G];
@@ -204,14 +226,21 @@ __device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, con
           if constexpr (VAR) var[j].add_var(wf[p][e], wd[p][e], x[p][e][j]);
         }
   }
-  PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * F + q * G;
+  SegVals<G, MODE, VAR> o;
 #pragma unroll
   for (int j = 0; j < G; j++) {
-    if (q * G + j < a.nf) {
-      out[j] = static_cast<PT>(acc[j].value());
-      if constexpr (VAR) out[(size_t)a.nseg * F + j] = static_cast<PT>(var[j].value());
-    }
+    o.v[j] = static_cast<PT>(acc[j].value());
+    if constexpr (VAR) o.e[j] = static_cast<PT>(var[j].value());
   }
+  return o;
+}
+
+// Applies segment s to this lane's G frames and writes its partials.
+template <int F, int G, int MODE, bool VAR, class LT>
+__device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, const u32x2* tl,
+                                          const unsigned char* twb, uint32_t ga, uint32_t gb,
+                                          uint32_t s, uint32_t q) {
+  seg_write<F, G, MODE, VAR>(a, s, q, seg_eval<G, MODE, VAR>(base, tl, twb, ga, gb));
 }
 
 // Persistent-grid tile order: workgroups are dealt to the XCDs round-robin
