@@ -65,7 +65,8 @@ class PlanInfo(C.Structure):
         ("nnz", C.c_uint64), ("nnz_masked", C.c_uint64), ("weighted", C.c_uint32),
         ("n_segments", C.c_uint32), ("uncertain_angles", C.c_uint64),
         ("singular_pixels", C.c_uint32), ("build_ms", C.c_double), ("hist_min", C.c_double),
-        ("hist_step", C.c_double), ("device_bytes", C.c_uint64)]
+        ("hist_step", C.c_double), ("device_bytes", C.c_uint64),
+        ("max_tile_groups", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):
@@ -126,6 +127,7 @@ def lib() -> C.CDLL:
             "dio_plan_time_gather": (i32, [vp, vp, i32, u64, u32, dp, dp, i32,
                                            C.POINTER(C.c_double)]),
             "dio_plan_time_passes": (i32, [vp, vp, i32, u64, u32, dp, dp, C.POINTER(C.c_float),
+                                           C.POINTER(u32),
                                            u32, C.POINTER(u32)]),
             "dio_device_alloc": (i32, [vp, u64, C.POINTER(vp)]),
             "dio_device_free": (i32, [vp, vp]),

@@ -88,7 +88,7 @@ namespace dio {
 // Enqueue the whole integration of device frames on `s`.
 void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
                 uint64_t n_frames, uint32_t flags, double* I, double* S, double* SD,
-                hipStream_t s, const hipEvent_t* pass_events) {
+                hipStream_t s, LaunchEvents* launches) {
   bool var = flags & DIO_WITH_ERRORS;
   if (flags & DIO_WITH_AZIMUTHAL_STD) {
     require(p->desc.ndim == 1, DIO_ERR_UNSUPPORTED, "azimuthal std is defined for 1D plans");
@@ -98,7 +98,7 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
   }
   require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
   tiled_gather(p->tt, sc, frames, dtype, n_frames, var, p->desc.precision, p->tile, p->lanes, I,
-               S, s, pass_events);
+               S, s, launches);
 }
 
 }  // namespace dio
@@ -236,6 +236,7 @@ int dio_plan_info_get(dio_plan* p, dio_plan_info* out) {
     out->nnz_masked = p->b.exec_table().nnz;
     out->weighted = p->b.weighted ? 1 : 0;
     out->n_segments = p->tt.nseg;
+    out->max_tile_groups = p->tt.max_tile_groups;
     out->uncertain_angles = p->b.uncertain;
     out->singular_pixels = p->b.singular;
     out->build_ms = p->build_ms;
@@ -463,14 +464,14 @@ int dio_graph_destroy(dio_graph* g) {
 
 int dio_plan_time_passes(dio_plan* p, const void* d_frames, int dtype, uint64_t n_frames,
                          uint32_t flags, double* d_intensity, double* d_sigma, float* ms,
-                         uint32_t cap, uint32_t* n_passes) {
+                         uint32_t* frames, uint32_t cap, uint32_t* n_launches) {
   return guarded([&] {
     require(p && d_frames && d_intensity && n_frames > 0, DIO_ERR_INVALID, "null argument");
     require(!(flags & DIO_WITH_AZIMUTHAL_STD), DIO_ERR_UNSUPPORTED,
             "pass timing covers the tiled gather only");
     std::lock_guard<std::mutex> lk(p->mu);
     DIO_HIP(hipSetDevice(p->device));
-    const uint64_t passes = (n_frames + p->tile - 1) / p->tile;
+    const uint64_t passes = (n_frames + p->tile - 1) / p->tile;  // launches <= passes
     std::vector<hipEvent_t> ev(2 * passes, nullptr);
     struct Events {
       std::vector<hipEvent_t>& v;
@@ -480,14 +481,21 @@ int dio_plan_time_passes(dio_plan* p, const void* d_frames, int dtype, uint64_t 
       }
     } guard{ev};
     for (hipEvent_t& e : ev) DIO_HIP(hipEventCreate(&e));
+    std::vector<uint32_t> fr(passes, 0);
+    dio::LaunchEvents le;
+    le.ev = ev.data();
+    le.cap = passes;
+    le.frames = fr.data();
     run_device(p, p->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, nullptr,
-               p->stream, ev.data());
-    DIO_HIP(hipEventSynchronize(ev.back()));
-    if (n_passes) *n_passes = (uint32_t)passes;
-    for (uint64_t k = 0; k < passes && k < cap; k++) {
+               p->stream, &le);
+    require(le.n > 0, DIO_ERR_DEVICE, "no tile kernel launch was timed");
+    DIO_HIP(hipEventSynchronize(ev[2 * le.n - 1]));
+    if (n_launches) *n_launches = (uint32_t)le.n;
+    for (uint64_t k = 0; k < le.n && k < cap; k++) {
       float t = 0.f;
       DIO_HIP(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
       ms[k] = t;
+      if (frames) frames[k] = fr[k];
     }
   });
 }

@@ -50,7 +50,7 @@ namespace dio {
 // Enqueues the integration of device frames on stream s (no synchronisation).
 void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype, uint64_t n_frames,
                 uint32_t flags, double* I, double* S, double* SD, hipStream_t s,
-                const hipEvent_t* pass_events = nullptr);
+                dio::LaunchEvents* launches = nullptr);
 // Plan construction behind dio_plan_create (validates, builds on the device).
 dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& desc);
 }  // namespace dio

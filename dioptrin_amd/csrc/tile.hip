@@ -562,16 +562,17 @@ __device__ __forceinline__ void pending_write(const TileArgs& a, uint32_t q,
 }
 
 // Computes one staged tile (m = its tile_meta): segments [s0, s1) in chunks
-// of at most C table groups; the first chunk is already in LDS, later chunks
-// of a large tile are loaded here. cur holds the group bounds of this slot's
-// first R segments. With pd, the partials of those R segments are left in
-// pd (the caller writes them) instead of being stored here.
+// of at most C table groups; the first chunk is already in LDS (unless
+// first_in_lds is false), later chunks of a large tile are loaded here. cur
+// holds the group bounds of this slot's first R segments. With pd, the
+// partials of those R segments are left in pd (the caller writes them)
+// instead of being stored here.
 template <int F, int G, int MODE, bool VAR, uint32_t C, int KT, uint32_t SLOTS, unsigned NT,
           class LT>
 __device__ __forceinline__ void tile_apply(
     const TileArgs& a, uint4 m, const SegRegs<SegRounds<SLOTS>::R>& cur, uint32_t slot,
     uint32_t q, const LT* base, u32x2* tl, unsigned char* twb,
-    Pending<SegRounds<SLOTS>::R, G, MODE, VAR>* pd = nullptr) {
+    Pending<SegRounds<SLOTS>::R, G, MODE, VAR>* pd = nullptr, bool first_in_lds = true) {
   // Chunk bounds are workgroup-uniform: read them through the scalar cache so
   // that waiting for them never waits on the vector loads in flight.
   typedef const __attribute__((address_space(4))) uint32_t CWord;
@@ -589,7 +590,7 @@ __device__ __forceinline__ void tile_apply(
       }
       sb = lo;
     }
-    if (sa != s0) {  // a later chunk of a large tile: load it now
+    if (sa != s0 || !first_in_lds) {  // a chunk not in LDS (a later one of a large tile)
       __syncthreads();
       TableRegs<KT, MODE> td;
       const uint32_t n = sg[sb] - gc;
@@ -732,13 +733,17 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * tile_threads(F) / 256))
 // ---------------------------------------------------------------- double-buffered tile kernel
 
 // Double-buffered variant for 16-frame passes of 32-bit frames: one workgroup
-// of 1024 threads per CU holds two LDS tiles. While tile k is computed from
-// one buffer, tile k + 1 (loaded into registers during the previous
-// iteration) is staged into the other and tile k + 2 is loaded into the
-// registers, so the frame loads stay in flight across the whole iteration and
-// an iteration needs one barrier. Threads 0-511 stage the first half of the
-// frames, threads 512-1023 the second half, each one pixel pair of a row.
-// A tile's partials (those of its first two rounds of segments) are written
+// of 1024 threads per CU holds two LDS frame tiles and two table chunks.
+// The work of a workgroup is a sequence of items, an item being one tile and
+// one pass: with two passes per launch (nf2 > 0), the two passes of a tile are
+// consecutive items that share one table chunk in LDS, so the table is read
+// from HBM once per two passes. While item i is computed from one frame
+// buffer, item i + 1 (loaded into registers during the previous iteration) is
+// staged into the other, and item i + 2 is loaded into the registers, so the
+// frame loads stay in flight across the whole iteration and an iteration
+// needs one barrier. Threads 0-511 stage the first half of the frames,
+// threads 512-1023 the second half, each one pixel pair of a row.
+// An item's partials (those of its first two rounds of segments) are written
 // in the next iteration, after the wait for the staged frames and before the
 // next loads: the memory counter also counts stores, so stores issued
 // between a load and its wait would be waited for too.
@@ -746,12 +751,14 @@ constexpr unsigned kDbThreads = 1024;
 
 template <int F, int MODE> struct PlanDb {
   using LT = typename TAcc<MODE>::L;
-  static constexpr size_t kFrameBytes = Plan<F, MODE>::kFrameBytes;
+  static constexpr size_t kFrameBytes = (Plan<F, MODE>::kFrameBytes + 255) / 256 * 256;
   static constexpr size_t kWBytes = MODE == 2 ? 32 : 16;
   static constexpr size_t kRoom = (80 * 1024 - kFrameBytes) / (8 + kWBytes);
   static constexpr uint32_t kCap = kRoom >= 1024 ? 1024 : (uint32_t)(kRoom / 8 * 8);
-  static constexpr size_t kHalf = (kFrameBytes + (size_t)kCap * (8 + kWBytes) + 255) / 256 * 256;
-  static constexpr size_t kBytes = 2 * kHalf;
+  static constexpr size_t kTableBytes = ((size_t)kCap * (8 + kWBytes) + 255) / 256 * 256;
+  // [frames 0][frames 1][table 0][table 1]
+  static constexpr size_t kTable0 = 2 * kFrameBytes;
+  static constexpr size_t kBytes = 2 * kFrameBytes + 2 * kTableBytes;
   static_assert(kBytes <= 160 * 1024, "two LDS tiles exceed one CU");
   static_assert(kCap >= kTilePixels / kGroup, "a whole segment must fit one table chunk");
   static_assert(kCap <= kDbThreads, "one table group per thread");
@@ -760,6 +767,18 @@ template <int F, int MODE> struct PlanDb {
 template <int F> struct DbFrames {
   uint32_t v[F / 2][2];
 };
+
+// Arguments of pass h (0 or 1) of a two-pass launch.
+template <int F>
+__device__ __forceinline__ TileArgs db_pass(const TileArgs& a, uint32_t h) {
+  TileArgs r = a;
+  if (h) {
+    r.frames = static_cast<const unsigned char*>(a.frames) + (uint64_t)F * a.npix * 4u;
+    r.nf = a.nf2;
+    r.part = static_cast<unsigned char*>(a.part) + a.pblock;
+  }
+  return r;
+}
 
 template <int F, int A>
 __device__ __forceinline__ void db_load(const TileArgs& a, uint32_t t, DbFrames<F>& fr) {
@@ -810,16 +829,18 @@ __device__ __forceinline__ void db_store(const TileArgs& a, const DbFrames<F>& f
   else db_store_as<F, float, LT>(fr, fl);
 }
 
+// Loads item (tile tt, pass arguments ap): segment bounds, frames and, for
+// the first pass of a tile, its first table chunk (returns its groups).
 template <int F, int A, int MODE, uint32_t C, uint32_t SLOTS>
-__device__ __forceinline__ uint32_t db_prefetch(const TileArgs& a, uint32_t tt, uint4 m,
-                                                uint32_t slot, DbFrames<F>& fr,
+__device__ __forceinline__ uint32_t db_prefetch(const TileArgs& ap, uint32_t tt, uint4 m,
+                                                bool table, uint32_t slot, DbFrames<F>& fr,
                                                 TableRegs<1, MODE>& tr,
                                                 SegRegs<SegRounds<SLOTS>::R>& sr) {
   if (m.x == m.y) return 0;
   const uint32_t n = min(m.w - m.z, C);
-  seg_bounds<SLOTS>(a, m, slot, sr);
-  db_load<F, A>(a, tt, fr);
-  table_load<1, MODE, kDbThreads>(a, m.z, m.z + n, tr);
+  seg_bounds<SLOTS>(ap, m, slot, sr);
+  db_load<F, A>(ap, tt, fr);
+  if (table) table_load<1, MODE, kDbThreads>(ap, m.z, m.z + n, tr);
   return n;
 }
 
@@ -835,73 +856,87 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   constexpr uint32_t SLOTS = kDbThreads / L;
   constexpr int R = SegRounds<SLOTS>::R;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  LT* const fl0 = reinterpret_cast<LT*>(lds_raw);
-  LT* const fl1 = reinterpret_cast<LT*>(lds_raw + PL::kHalf);
   const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
-
+  // Item i: the workgroup's tile i / H, pass i % H.
+  const uint32_t H = a.nf2 ? 2u : 1u;
   const uint32_t grid = gridDim.x;
   const uint32_t t0 = first_tile();
   if (t0 >= a.ntiles) return;
+  auto tile_of = [&](uint32_t i) { return t0 + (i / H) * grid; };
+  auto frames_buf = [&](uint32_t i) { return lds_raw + (i & 1) * PL::kFrameBytes; };
+  auto table_buf = [&](uint32_t i) {
+    return lds_raw + PL::kTable0 + ((i / H) & 1) * PL::kTableBytes;
+  };
   MetaLanes ml;
   ml.load(a, t0, grid, 0);
-  pad_slots<F>(fl0);
-  pad_slots<F>(fl1);
+  pad_slots<F>(reinterpret_cast<LT*>(frames_buf(0)));
+  pad_slots<F>(reinterpret_cast<LT*>(frames_buf(1)));
 
   DbFrames<F> fr;
   TableRegs<1, MODE> tr;
   SegRegs<R> sr = {};
   Pending<R, G, MODE, VAR> pd;
   pd.clear();
-  // Prologue: tile 0 staged into buffer 0, tile 1 loaded into registers.
+  uint32_t ph = 0;  // pass of the item whose partials are pending
+  // Prologue: item 0 staged, item 1 loaded into the registers.
   uint4 mA = ml.get(0);
-  uint32_t pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0, mA, slot, fr, tr, sr);
+  uint32_t pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0, mA, true, slot, fr, tr, sr);
   if (mA.x != mA.y) {
-    db_store<F, LT>(a, fr, fl0);
-    table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(lds_raw + PL::kFrameBytes),
-                                     lds_raw + PL::kFrameBytes + (size_t)C * 8);
+    db_store<F, LT>(a, fr, reinterpret_cast<LT*>(frames_buf(0)));
+    table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(0)),
+                                     table_buf(0) + (size_t)C * 8);
   }
   SegRegs<R> cA = sr;
-  uint4 mB = ml.get(1);
-  if (t0 + grid < a.ntiles) pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0 + grid, mB, slot, fr, tr, sr);
+  uint4 mB = ml.get(1 / H);
+  if (tile_of(1) < a.ntiles)
+    pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, 1 % H), tile_of(1), mB, H == 1,
+                                            slot, fr, tr, sr);
   __syncthreads();
 
-  for (uint32_t k = 0;; k++) {
-    if (t0 + k * grid >= a.ntiles) break;
-    const uint32_t b = k & 1;
-    unsigned char* const cur = lds_raw + (b ? PL::kHalf : 0);
-    unsigned char* const nxt = lds_raw + (b ? 0 : PL::kHalf);
-    // 1. Tile k + 1 (registers) into the other buffer, free since the last
-    //    barrier.
+  for (uint32_t i = 0;; i++) {
+    if (tile_of(i) >= a.ntiles) break;
+    const uint32_t h = i % H;
+    // 1. Item i + 1 (registers) into the other frame buffer, free since the
+    //    last barrier, and its table chunk if it opens a tile (the other
+    //    table buffer, last read by the previous tile).
     if (mB.x != mB.y) {
-      db_store<F, LT>(a, fr, reinterpret_cast<LT*>(nxt));
-      table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(nxt + PL::kFrameBytes),
-                                       nxt + PL::kFrameBytes + (size_t)C * 8);
+      const uint32_t i1 = i + 1;
+      db_store<F, LT>(a, fr, reinterpret_cast<LT*>(frames_buf(i1)));
+      if (i1 % H == 0)
+        table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(i1)),
+                                         table_buf(i1) + (size_t)C * 8);
     }
     const SegRegs<R> cB = sr;
     // Every load of the last prefetch has landed on every path (the store
     // above waited for them): said explicitly, so that no later wait in this
     // iteration also waits for the partial stores issued next.
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
-    // 2. Partials of tile k - 1.
-    pending_write<F>(a, q, pd);
-    // 3. Tile k + 2 into the registers.
-    if (k + 2 - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k + 2);
-    const uint4 mC = ml.get(k + 2);
-    if (t0 + (k + 2) * grid < a.ntiles)
-      pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0 + (k + 2) * grid, mC, slot, fr, tr, sr);
-    // 4. Tile k from its buffer.
+    // 2. Partials of item i - 1.
+    pending_write<F>(db_pass<F>(a, ph), q, pd);
+    // 3. Item i + 2 into the registers.
+    const uint32_t i2 = i + 2, k2 = i2 / H;
+    if (k2 - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k2);
+    const uint4 mC = ml.get(k2);
+    if (tile_of(i2) < a.ntiles)
+      pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, i2 % H), tile_of(i2), mC,
+                                              i2 % H == 0, slot, fr, tr, sr);
+    // 4. Item i from its buffers. The second pass of a tile whose table
+    //    takes several chunks reloads the first (the last chunk is in LDS).
     if (mA.x != mA.y) {
-      const LT* base = reinterpret_cast<const LT*>(cur) + q * G * kLdsStride;
+      const LT* base = reinterpret_cast<const LT*>(frames_buf(i)) + q * G * kLdsStride;
+      unsigned char* tb = table_buf(i);
+      const bool first_in_lds = h == 0 || mA.w - mA.z <= C;
       tile_apply<F, G, MODE, VAR, C, 1, SLOTS, kDbThreads>(
-          a, mA, cA, slot, q, base, reinterpret_cast<u32x2*>(cur + PL::kFrameBytes),
-          cur + PL::kFrameBytes + (size_t)C * 8, &pd);
+          db_pass<F>(a, h), mA, cA, slot, q, base, reinterpret_cast<u32x2*>(tb),
+          tb + (size_t)C * 8, &pd, first_in_lds);
     }
+    ph = h;
     __syncthreads();
     mA = mB;
     cA = cB;
     mB = mC;
   }
-  pending_write<F>(a, q, pd);
+  pending_write<F>(db_pass<F>(a, ph), q, pd);
 }
 
 // prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =
@@ -1119,12 +1154,19 @@ void launch_m(const TiledTable& t, const TileArgs& a, bool var, int precision, h
   }
 }
 
+// nf2 > 0 (double-buffered kernel only): a second pass of nf2 frames, at
+// fr + F frames, partials at part + pblock bytes, in the same launch.
 void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, int dtype,
-                 uint32_t nf, bool var, int precision, void* part, hipStream_t s) {
+                 uint32_t nf, bool var, int precision, void* part, hipStream_t s,
+                 uint32_t nf2 = 0, uint64_t pblock = 0) {
   const uint32_t al8 = (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % 8 == 0) ? 1u : 0u;
   TileArgs a{fr,     dtype,   t.lidx.p, t.w32.p, t.w64.p, t.seg_group.p,
              t.tile_meta.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
              (uint64_t)t.dim1 * t.dim2, nf, al8, part};
+  a.nf2 = nf2;
+  a.pblock = pblock;
+  if (nf2 && !double_buffered(t, F, dtype, precision))
+    fail(DIO_ERR_INVALID, "two passes per launch need the double-buffered kernel");
   if (precision == DIO_F64 && !t.w64.p) fail(DIO_ERR_INVALID, "plan has no float64 weights");
   switch (F * 8 + G) {
     case 1 * 8 + 1: launch_m<1, 1>(t, a, var, precision, s); break;
@@ -1297,7 +1339,12 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
   k_count_u32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(bin_n.p, nseg, hist.p);
   scan_u32(hist.p, t.bin_ptr.p, nbins + 1, s, tmp);
   DIO_HIP(hipGetLastError());
+  std::vector<uint32_t> meta(4 * (size_t)t.ntiles);
+  DIO_HIP(hipMemcpyAsync(meta.data(), t.tile_meta.p, meta.size() * 4, hipMemcpyDeviceToHost, s));
   DIO_HIP(hipStreamSynchronize(s));
+  t.max_tile_groups = 0;
+  for (uint32_t k = 0; k < t.ntiles; k++)
+    t.max_tile_groups = std::max(t.max_tile_groups, meta[4 * k + 3] - meta[4 * k + 2]);
 }
 
 void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
@@ -1310,10 +1357,14 @@ void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, i
   if (t.nseg) launch_pass(t, F, G, frames, dtype, nf, var, precision, sc.ptr(), s);
 }
 
+bool double_buffered(const TiledTable& t, uint32_t F, int dtype, int precision) {
+  const bool w32 = dtype == DIO_U32 || dtype == DIO_I32 || dtype == DIO_F32T;
+  return F == 16 && precision != DIO_F64 && t.db && w32;
+}
+
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t F, uint32_t G,
-                  double* intensity, double* sigma, hipStream_t s,
-                  const hipEvent_t* pass_events) {
+                  double* intensity, double* sigma, hipStream_t s, LaunchEvents* le) {
   if (!valid_tile(precision, F, G))
     fail(DIO_ERR_INVALID, "invalid frames per pass / frames per lane");
   const size_t esz = dtype_bytes(dtype);
@@ -1327,17 +1378,35 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   const uint32_t FR = F * P;
   const uint64_t BS = (uint64_t)t.nseg * F * (var ? 2 : 1);  // elements per pass block
   const size_t pbytes = BS * (f64 ? sizeof(double) : sizeof(float));
+  // The double-buffered kernel takes both passes of a reduction in one
+  // launch and reads each table tile once for them.
+  const bool pair = P == 2 && double_buffered(t, F, dtype, precision);
+  auto timed = [&](uint32_t frames_in_launch, auto&& launch) {
+    const bool rec = le && le->n < le->cap;
+    if (rec) DIO_HIP(hipEventRecord(le->ev[2 * le->n], s));
+    if (t.nseg) launch();
+    if (rec) {
+      DIO_HIP(hipEventRecord(le->ev[2 * le->n + 1], s));
+      if (le->frames) le->frames[le->n] = frames_in_launch;
+      le->n++;
+    }
+  };
   for (uint64_t g0 = 0; g0 < n_frames; g0 += FR) {
     const uint32_t ng = (uint32_t)std::min<uint64_t>(FR, n_frames - g0);
-    for (uint32_t j = 0; j * F < ng; j++) {
-      const uint64_t f0 = g0 + (uint64_t)j * F;
-      const uint32_t nf = std::min(F, ng - j * F);
-      const uint64_t pass = f0 / F;
-      if (pass_events) DIO_HIP(hipEventRecord(pass_events[2 * pass], s));
-      if (t.nseg)
-        launch_pass(t, F, G, fb + f0 * npix * esz, dtype, nf, var, precision,
-                    reinterpret_cast<char*>(sc.ptr()) + j * pbytes, s);
-      if (pass_events) DIO_HIP(hipEventRecord(pass_events[2 * pass + 1], s));
+    if (pair && ng > F) {
+      timed(ng, [&] {
+        launch_pass(t, F, G, fb + g0 * npix * esz, dtype, F, var, precision, sc.ptr(), s,
+                    ng - F, pbytes);
+      });
+    } else {
+      for (uint32_t j = 0; j * F < ng; j++) {
+        const uint64_t f0 = g0 + (uint64_t)j * F;
+        const uint32_t nf = std::min(F, ng - j * F);
+        timed(nf, [&] {
+          launch_pass(t, F, G, fb + f0 * npix * esz, dtype, nf, var, precision,
+                      reinterpret_cast<char*>(sc.ptr()) + j * pbytes, s);
+        });
+      }
     }
     double* I = intensity + g0 * t.nbins;
     double* S = var ? sigma + g0 * t.nbins : nullptr;

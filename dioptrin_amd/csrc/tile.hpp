@@ -32,6 +32,7 @@ struct TiledTable {
   bool db = true;              // double-buffered kernel for 16-frame passes of 32-bit frames
   uint64_t nnz = 0;
   uint32_t nseg = 0, ngroups = 0;
+  uint32_t max_tile_groups = 0;  // largest table of one tile, in groups
   DevBuf<uint16_t> lidx;       // 4 * ngroups: pixel index inside its tile (kPadSlot = padding)
   DevBuf<float> w32;           // 4 * ngroups
   DevBuf<double> w64;          // 4 * ngroups (F64 precision only)
@@ -79,12 +80,24 @@ uint32_t max_frames_for(int precision);
 // Valid (frames per pass, frames per lane) pair for this precision.
 bool valid_tile(int precision, uint32_t frames, uint32_t lanes);
 
-// pass_events (optional): 2 per pass, recorded before and after each pass's
-// tile kernel.
+// Events recorded around each launch of the tile kernel (measurement): 2 per
+// launch in ev (room for cap launches); n counts the launches, frames[k] (if
+// given) receives the frames of launch k.
+struct LaunchEvents {
+  const hipEvent_t* ev = nullptr;
+  uint64_t cap = 0;
+  uint64_t n = 0;
+  uint32_t* frames = nullptr;
+};
+
+// True when 16-frame passes of this frame type and precision run the
+// double-buffered kernel, which takes two passes per launch.
+bool double_buffered(const TiledTable& t, uint32_t frames_per_pass, int dtype, int precision);
+
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t frames_per_pass,
                   uint32_t frames_per_lane, double* intensity, double* sigma, hipStream_t s,
-                  const hipEvent_t* pass_events = nullptr);
+                  LaunchEvents* launches = nullptr);
 
 // Only the tile kernel for one pass of nf <= frames_per_pass frames (timing).
 void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,

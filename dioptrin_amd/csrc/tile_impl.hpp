@@ -108,6 +108,11 @@ struct TileArgs {
   uint32_t nf;
   uint32_t aligned8;  // frame rows and blocks are 8-byte aligned (32-bit frames)
   void* part;
+  // Double-buffered kernel only: frames of a second pass over the same table
+  // (0: none); its frames follow the first pass's F frames and its partials
+  // start pblock bytes after the first pass's.
+  uint32_t nf2 = 0;
+  uint64_t pblock = 0;
 };
 
 // Segment (relative to the chunk's first) a slot takes in round r. Segments

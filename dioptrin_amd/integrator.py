@@ -299,16 +299,20 @@ class GpuPlan:
         return ms.value
 
     def time_passes(self, frames, out_i, out_s=None, errors=False) -> list:
-        """Gather-kernel duration (ms) of each pass of one device-resident
-        integration of the batch, timed with HIP events on the plan's stream."""
+        """(duration in ms, frames) of each gather-kernel launch of one
+        device-resident integration of the batch, timed with HIP events on
+        the plan's stream (a launch covers one pass, or two passes sharing
+        the table)."""
         dt = _NP_DTYPES[frames.dtype]
-        passes = frames.shape[0]  # at least one frame per pass
-        buf = (C.c_float * passes)()
+        cap = frames.shape[0]  # at least one frame per launch
+        buf = (C.c_float * cap)()
+        nfr = (C.c_uint32 * cap)()
         n = C.c_uint32()
         N.check(N.lib().dio_plan_time_passes(
             self._h, frames.ptr, N.DTYPES[dt], frames.shape[0], N.WITH_ERRORS if errors else 0,
-            out_i.ptr, None if out_s is None else out_s.ptr, buf, passes, C.byref(n)))
-        return list(buf)[:min(passes, n.value)]
+            out_i.ptr, None if out_s is None else out_s.ptr, buf, nfr, cap, C.byref(n)))
+        k = min(cap, n.value)
+        return list(zip(list(buf)[:k], list(nfr)[:k]))
 
     def capture_graph(self, frames, out_i, out_s=None, out_d=None, errors=False,
                       azimuthal_std=False):

@@ -84,6 +84,7 @@ typedef struct {
   double build_ms;              /* plan build wall time */
   double hist_min, hist_step;   /* 1D radial axis */
   uint64_t device_bytes;        /* device memory held by the plan */
+  uint32_t max_tile_groups;     /* largest table of one detector tile, in 4-entry groups */
 } dio_plan_info;
 
 typedef struct {
@@ -171,12 +172,15 @@ int dio_plan_time_gather(dio_plan *plan, const void *d_frames, int dtype, uint64
                          uint32_t flags, double *d_intensity, double *d_sigma, int reps,
                          double *avg_ms);
 /* Runs one dio_integrate_device of the whole batch with HIP events recorded
- * on the plan's stream around each pass's gather kernel, waits, and returns
- * the kernel durations (ms) of the passes in order: min(cap, passes) values,
- * the pass count in *n_passes. Measurement only (no reference counterpart). */
+ * on the plan's stream around each launch of the gather kernel (one pass, or
+ * two passes sharing the table for the double-buffered kernel), waits, and
+ * returns the kernel durations (ms) of the launches in order and, if frames
+ * is not NULL, the frames each launch covered: min(cap, launches) values,
+ * the launch count in *n_launches. Measurement only (no reference
+ * counterpart). */
 int dio_plan_time_passes(dio_plan *plan, const void *d_frames, int dtype, uint64_t n_frames,
                          uint32_t flags, double *d_intensity, double *d_sigma, float *ms,
-                         uint32_t cap, uint32_t *n_passes);
+                         uint32_t *frames, uint32_t cap, uint32_t *n_launches);
 
 /* Device memory and synchronisation for callers without their own device
  * allocator (the reference's plans own their device buffers the same way,

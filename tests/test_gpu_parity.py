@@ -292,9 +292,10 @@ def test_device_resident_path_and_graph_replay(pilatus):
 
 
 def test_pass_timing_matches_plain_run(pilatus):
-    """The timed integration (HIP events around each pass's gather kernel)
-    writes the same outputs as the plain device run and reports one positive
-    kernel duration per pass."""
+    """The timed integration (HIP events around each launch of the gather
+    kernel) writes the same outputs as the plain device run and reports one
+    positive kernel duration per launch: one pass per launch for 2-frame
+    passes, two passes per launch for the double-buffered 16-frame kernel."""
     from dioptrin_amd.device import DeviceArray, synchronize
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
@@ -308,8 +309,42 @@ def test_pass_timing_matches_plain_run(pilatus):
     synchronize()
     for rep in range(2):
         t = plan.time_passes(frames, out)
-        assert len(t) == 3 and all(x > 0 for x in t), t
+        assert [f for _, f in t] == [2, 2, 1] and all(ms > 0 for ms, _ in t), t
         assert np.array_equal(out.numpy(), ref.numpy()), rep
+    frames = DeviceArray.from_numpy(
+        np.random.default_rng(4).poisson(90.0, size=(37,) + d["w"]["shape"]).astype(np.uint32))
+    ref = DeviceArray((37, plan.nbins), np.float64)
+    out = DeviceArray((37, plan.nbins), np.float64)
+    plan.set_tile(16, 2)
+    plan.integrate_device(frames, ref)
+    synchronize()
+    t = plan.time_passes(frames, out)
+    assert [f for _, f in t] == [32, 5] and all(ms > 0 for ms, _ in t), t
+    assert np.array_equal(out.numpy(), ref.numpy())
+
+
+def test_paired_passes_with_multi_chunk_tiles(monkeypatch):
+    """The double-buffered kernel runs two 16-frame passes per launch over one
+    LDS copy of each tile's table. A 500 x 360 2D table has tiles whose table
+    takes several LDS chunks (up to 4208 groups); the second pass of such a
+    tile reloads the first chunk. Results equal the single-buffered kernel's
+    (one pass per launch, every chunk loaded per pass) bit for bit."""
+    g = GEOMETRIES_2D[-1]
+    name, poni, shape, nr, na = g[:5]
+    frames = np.random.default_rng(13).poisson(60.0, size=(29,) + shape).astype(np.uint32)
+
+    def run():
+        plan = GpuPlan(poni, shape, n_radial=nr, n_azimuthal=na, precision="f32_wide")
+        assert plan.info["max_tile_groups"] > 3 * 672  # several chunks of 672 groups
+        plan.set_tile(16, 2)
+        return plan.integrate_stack(frames, errors=True)[:2]
+
+    monkeypatch.delenv("DIOPTRIN_TILE_DB", raising=False)
+    i0, s0 = run()
+    monkeypatch.setenv("DIOPTRIN_TILE_DB", "0")
+    i1, s1 = run()
+    assert np.array_equal(i0, i1)
+    assert np.array_equal(s0, s1)
 
 
 def test_device_synthetic_frames_are_poisson():
