@@ -735,9 +735,9 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * tile_threads(F) / 256))
 // Double-buffered variant for 16-frame passes of 32-bit frames: one workgroup
 // of 1024 threads per CU holds two LDS frame tiles and two table chunks.
 // The work of a workgroup is a sequence of items, an item being one tile and
-// one pass: with two passes per launch (nf2 > 0), the two passes of a tile are
-// consecutive items that share one table chunk in LDS, so the table is read
-// from HBM once per two passes. While item i is computed from one frame
+// one pass: with several passes per launch (npass, up to four), the passes of
+// a tile are consecutive items that share one table chunk in LDS, so the table
+// is read from HBM once per launch. While item i is computed from one frame
 // buffer, item i + 1 (loaded into registers during the previous iteration) is
 // staged into the other, and item i + 2 is loaded into the registers, so the
 // frame loads stay in flight across the whole iteration and an iteration
@@ -768,14 +768,14 @@ template <int F> struct DbFrames {
   uint32_t v[F / 2][2];
 };
 
-// Arguments of pass h (0 or 1) of a two-pass launch.
+// Arguments of pass h < npass of a launch.
 template <int F>
 __device__ __forceinline__ TileArgs db_pass(const TileArgs& a, uint32_t h) {
   TileArgs r = a;
-  if (h) {
-    r.frames = static_cast<const unsigned char*>(a.frames) + (uint64_t)F * a.npix * 4u;
-    r.nf = a.nf2;
-    r.part = static_cast<unsigned char*>(a.part) + a.pblock;
+  if (a.npass > 1) {
+    r.frames = static_cast<const unsigned char*>(a.frames) + (uint64_t)h * F * a.npix * 4u;
+    r.nf = h + 1 == a.npass ? a.nf_last : (uint32_t)F;
+    r.part = static_cast<unsigned char*>(a.part) + h * a.pblock;
   }
   return r;
 }
@@ -858,7 +858,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
   // Item i: the workgroup's tile i / H, pass i % H.
-  const uint32_t H = a.nf2 ? 2u : 1u;
+  const uint32_t H = a.npass;
   const uint32_t grid = gridDim.x;
   const uint32_t t0 = first_tile();
   if (t0 >= a.ntiles) return;
@@ -1154,19 +1154,24 @@ void launch_m(const TiledTable& t, const TileArgs& a, bool var, int precision, h
   }
 }
 
-// nf2 > 0 (double-buffered kernel only): a second pass of nf2 frames, at
-// fr + F frames, partials at part + pblock bytes, in the same launch.
+// npass > 1 (double-buffered kernel only): npass passes in one launch, the
+// last of nf frames and the others of F; pass h reads frames h * F onwards
+// and writes its partials at part + h * pblock bytes.
 void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, int dtype,
                  uint32_t nf, bool var, int precision, void* part, hipStream_t s,
-                 uint32_t nf2 = 0, uint64_t pblock = 0) {
+                 uint32_t npass = 1, uint64_t pblock = 0) {
   const uint32_t al8 = (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % 8 == 0) ? 1u : 0u;
   TileArgs a{fr,     dtype,   t.lidx.p, t.w32.p, t.w64.p, t.seg_group.p,
              t.tile_meta.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
              (uint64_t)t.dim1 * t.dim2, nf, al8, part};
-  a.nf2 = nf2;
+  if (npass < 1 || npass > kMaxPassesPerLaunch || nf < 1 || nf > F)
+    fail(DIO_ERR_INVALID, "invalid passes per launch");
+  a.npass = npass;
+  a.nf_last = nf;
+  a.nf = npass > 1 ? F : nf;
   a.pblock = pblock;
-  if (nf2 && !double_buffered(t, F, dtype, precision))
-    fail(DIO_ERR_INVALID, "two passes per launch need the double-buffered kernel");
+  if (npass > 1 && !double_buffered(t, F, dtype, precision))
+    fail(DIO_ERR_INVALID, "several passes per launch need the double-buffered kernel");
   if (precision == DIO_F64 && !t.w64.p) fail(DIO_ERR_INVALID, "plan has no float64 weights");
   switch (F * 8 + G) {
     case 1 * 8 + 1: launch_m<1, 1>(t, a, var, precision, s); break;
@@ -1372,15 +1377,16 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   const char* fb = static_cast<const char*>(frames);
   const uint64_t npix = (uint64_t)t.dim1 * t.dim2;
   const bool plain = precision == DIO_F32, f64 = precision == DIO_F64;
-  // Float32 partials of two passes fit the scratch (F float64 values per
-  // segment and kind), so one reduction launch serves both passes.
-  const uint32_t P = f64 ? 1 : 2;
+  // The scratch holds the float32 partials of kMaxPassesPerLaunch passes (or
+  // the float64 ones of one). The double-buffered kernel runs that many
+  // passes in one launch over one read of the table; otherwise a launch is a
+  // pass. A reduction launch serves up to 32 frames (two 16-frame passes).
+  const bool db = !f64 && double_buffered(t, F, dtype, precision);
+  const uint32_t P = f64 ? 1 : db ? kMaxPassesPerLaunch : 2;
   const uint32_t FR = F * P;
+  const uint32_t RF = std::min(FR, 32u);  // frames per reduction launch
   const uint64_t BS = (uint64_t)t.nseg * F * (var ? 2 : 1);  // elements per pass block
   const size_t pbytes = BS * (f64 ? sizeof(double) : sizeof(float));
-  // The double-buffered kernel takes both passes of a reduction in one
-  // launch and reads each table tile once for them.
-  const bool pair = P == 2 && double_buffered(t, F, dtype, precision);
   auto timed = [&](uint32_t frames_in_launch, auto&& launch) {
     const bool rec = le && le->n < le->cap;
     if (rec) DIO_HIP(hipEventRecord(le->ev[2 * le->n], s));
@@ -1393,10 +1399,11 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   };
   for (uint64_t g0 = 0; g0 < n_frames; g0 += FR) {
     const uint32_t ng = (uint32_t)std::min<uint64_t>(FR, n_frames - g0);
-    if (pair && ng > F) {
+    if (db) {
+      const uint32_t np = (ng + F - 1) / F;
       timed(ng, [&] {
-        launch_pass(t, F, G, fb + g0 * npix * esz, dtype, F, var, precision, sc.ptr(), s,
-                    ng - F, pbytes);
+        launch_pass(t, F, G, fb + g0 * npix * esz, dtype, ng - (np - 1) * F, var, precision,
+                    sc.ptr(), s, np, pbytes);
       });
     } else {
       for (uint32_t j = 0; j * F < ng; j++) {
@@ -1408,17 +1415,21 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
         });
       }
     }
-    double* I = intensity + g0 * t.nbins;
-    double* S = var ? sigma + g0 * t.nbins : nullptr;
-    if (f64) {
-      if (var) launch_final<false, true, double>(t, sc.ptr(), F, BS, ng, I, S, s);
-      else launch_final<false, false, double>(t, sc.ptr(), F, BS, ng, I, S, s);
-    } else if (plain) {
-      if (var) launch_final<true, true, float>(t, sc.ptr(), F, BS, ng, I, S, s);
-      else launch_final<true, false, float>(t, sc.ptr(), F, BS, ng, I, S, s);
-    } else {
-      if (var) launch_final<false, true, float>(t, sc.ptr(), F, BS, ng, I, S, s);
-      else launch_final<false, false, float>(t, sc.ptr(), F, BS, ng, I, S, s);
+    for (uint32_t r0 = 0; r0 < ng; r0 += RF) {
+      const uint32_t nr = std::min(RF, ng - r0);
+      const void* pr = reinterpret_cast<const char*>(sc.ptr()) + (r0 / F) * pbytes;
+      double* I = intensity + (g0 + r0) * t.nbins;
+      double* S = var ? sigma + (g0 + r0) * t.nbins : nullptr;
+      if (f64) {
+        if (var) launch_final<false, true, double>(t, pr, F, BS, nr, I, S, s);
+        else launch_final<false, false, double>(t, pr, F, BS, nr, I, S, s);
+      } else if (plain) {
+        if (var) launch_final<true, true, float>(t, pr, F, BS, nr, I, S, s);
+        else launch_final<true, false, float>(t, pr, F, BS, nr, I, S, s);
+      } else {
+        if (var) launch_final<false, true, float>(t, pr, F, BS, nr, I, S, s);
+        else launch_final<false, false, float>(t, pr, F, BS, nr, I, S, s);
+      }
     }
     DIO_HIP(hipGetLastError());
   }

@@ -24,6 +24,8 @@ constexpr uint32_t kMaxFrames = 16;
 // Longest segment, in groups: a (tile, bin) run beyond it is cut into several
 // segments, so no lane of a tile walks far more groups than the others.
 constexpr uint32_t kMaxSegGroups = 12;
+// Passes over one read of the table in a launch of the double-buffered kernel.
+constexpr uint32_t kMaxPassesPerLaunch = 4;
 
 struct TiledTable {
   uint32_t dim1 = 0, dim2 = 0, nbins = 0, ntx = 0, nty = 0, ntiles = 0;
@@ -53,8 +55,9 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
                  const uint32_t* row_ptr, const uint32_t* col, const float* w32,
                  const double* w64, bool keep_w64, hipStream_t s);
 
-// Segment partials of one pass: [1 + errors][nseg][frames per pass], float32
-// (float64 for the F64 precision).
+// Segment partials: per pass a block [1 + errors][nseg][frames per pass],
+// float32 (float64 for the F64 precision), for up to kMaxPassesPerLaunch
+// passes of float32 partials (or one pass of float64 ones).
 // Either owned (grown on demand) or external memory of a stated size (the
 // serving loop's stream-ordered slot buffers).
 struct TileScratch {
@@ -62,7 +65,7 @@ struct TileScratch {
   double* ext = nullptr;
   size_t ext_n = 0;
   static size_t need(const TiledTable& t, uint32_t frames, bool var) {
-    return (size_t)t.nseg * frames * (var ? 2 : 1);
+    return (size_t)t.nseg * frames * (var ? 2 : 1) * (kMaxPassesPerLaunch / 2);
   }
   double* ptr() const { return ext ? ext : part.p; }
   void ensure(const TiledTable& t, uint32_t frames, bool var) {

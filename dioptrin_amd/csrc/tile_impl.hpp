@@ -108,10 +108,12 @@ struct TileArgs {
   uint32_t nf;
   uint32_t aligned8;  // frame rows and blocks are 8-byte aligned (32-bit frames)
   void* part;
-  // Double-buffered kernel only: frames of a second pass over the same table
-  // (0: none); its frames follow the first pass's F frames and its partials
-  // start pblock bytes after the first pass's.
-  uint32_t nf2 = 0;
+  // Double-buffered kernel only: npass passes over the same table in one
+  // launch. Pass h reads frames h * F onwards and writes its partials pblock
+  // bytes after pass h - 1's; passes before the last hold F frames, the last
+  // nf_last (a single pass holds nf).
+  uint32_t npass = 1;
+  uint32_t nf_last = 0;
   uint64_t pblock = 0;
 };
 

@@ -155,8 +155,9 @@ def test_batch_equals_single_frame_bitwise(pilatus, tile, precision):
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
                    polarization_factor=0.99, mask=d["mask"], precision=precision)
     rng = np.random.default_rng(5)
-    # 53 = 32 + 21 frames: float32 partials of two passes share one reduction,
-    # so the second pair holds a full pass and a 5-frame one.
+    # 53 frames: the last pass is partial (5 frames); 16-frame passes of
+    # 32-bit frames run as one 4-pass launch with reductions of 32 and 21
+    # frames, smaller passes reduce two passes (or one float64 pass) at a time.
     frames = rng.poisson(200.0, size=(53,) + d["w"]["shape"]).astype(np.float32)
     plan.set_tile(1, 1)
     single = [plan.integrate_stack(frames[k:k + 1], errors=True) for k in range(frames.shape[0])]
@@ -295,7 +296,8 @@ def test_pass_timing_matches_plain_run(pilatus):
     """The timed integration (HIP events around each launch of the gather
     kernel) writes the same outputs as the plain device run and reports one
     positive kernel duration per launch: one pass per launch for 2-frame
-    passes, two passes per launch for the double-buffered 16-frame kernel."""
+    passes, up to four passes per launch for the double-buffered 16-frame
+    kernel."""
     from dioptrin_amd.device import DeviceArray, synchronize
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
@@ -319,19 +321,20 @@ def test_pass_timing_matches_plain_run(pilatus):
     plan.integrate_device(frames, ref)
     synchronize()
     t = plan.time_passes(frames, out)
-    assert [f for _, f in t] == [32, 5] and all(ms > 0 for ms, _ in t), t
+    assert [f for _, f in t] == [37] and all(ms > 0 for ms, _ in t), t
     assert np.array_equal(out.numpy(), ref.numpy())
 
 
 def test_paired_passes_with_multi_chunk_tiles(monkeypatch):
-    """The double-buffered kernel runs two 16-frame passes per launch over one
-    LDS copy of each tile's table. A 500 x 360 2D table has tiles whose table
-    takes several LDS chunks (up to 4208 groups); the second pass of such a
-    tile reloads the first chunk. Results equal the single-buffered kernel's
-    (one pass per launch, every chunk loaded per pass) bit for bit."""
+    """The double-buffered kernel runs up to four 16-frame passes per launch
+    over one LDS copy of each tile's table. A 500 x 360 2D table has tiles
+    whose table takes several LDS chunks (up to 4208 groups); every pass after
+    the first of such a tile reloads the first chunk. Results equal the
+    single-buffered kernel's (one pass per launch, every chunk loaded per
+    pass) bit for bit, for 4 + 2 passes (16, 16, 16, 16 | 16, 3 frames)."""
     g = GEOMETRIES_2D[-1]
     name, poni, shape, nr, na = g[:5]
-    frames = np.random.default_rng(13).poisson(60.0, size=(29,) + shape).astype(np.uint32)
+    frames = np.random.default_rng(13).poisson(60.0, size=(83,) + shape).astype(np.uint32)
 
     def run():
         plan = GpuPlan(poni, shape, n_radial=nr, n_azimuthal=na, precision="f32_wide")
