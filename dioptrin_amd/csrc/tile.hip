@@ -829,18 +829,19 @@ __device__ __forceinline__ void db_store(const TileArgs& a, const DbFrames<F>& f
   else db_store_as<F, float, LT>(fr, fl);
 }
 
-// Loads item (tile tt, pass arguments ap): segment bounds, frames and, for
-// the first pass of a tile, its first table chunk (returns its groups).
+// Loads item (tile tt, pass arguments ap): frames and, for the first pass of
+// a tile, its segment bounds and first table chunk (returns its groups). The
+// later passes of the tile keep the bounds already in sr.
 template <int F, int A, int MODE, uint32_t C, uint32_t SLOTS>
 __device__ __forceinline__ uint32_t db_prefetch(const TileArgs& ap, uint32_t tt, uint4 m,
-                                                bool table, uint32_t slot, DbFrames<F>& fr,
+                                                bool first, uint32_t slot, DbFrames<F>& fr,
                                                 TableRegs<1, MODE>& tr,
                                                 SegRegs<SegRounds<SLOTS>::R>& sr) {
   if (m.x == m.y) return 0;
   const uint32_t n = min(m.w - m.z, C);
-  seg_bounds<SLOTS>(ap, m, slot, sr);
+  if (first) seg_bounds<SLOTS>(ap, m, slot, sr);
   db_load<F, A>(ap, tt, fr);
-  if (table) table_load<1, MODE, kDbThreads>(ap, m.z, m.z + n, tr);
+  if (first) table_load<1, MODE, kDbThreads>(ap, m.z, m.z + n, tr);
   return n;
 }
 

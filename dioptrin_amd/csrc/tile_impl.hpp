@@ -15,6 +15,7 @@ constexpr unsigned kXcds = 8;  // accelerator complex dies of an MI355X
 // can become byte copies through scratch memory.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
@@ -155,6 +156,22 @@ __device__ __forceinline__ void seg_write(const TileArgs& a, uint32_t s, uint32_
                                           const SegVals<G, MODE, VAR>& o) {
   using PT = typename TAcc<MODE>::P;
   PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * F + q * G;
+  if constexpr (sizeof(PT) == 4 && (G == 2 || G == 4)) {
+    // All G frames present: one vector store per kind (the row offset is a
+    // multiple of G floats).
+    using V = typename std::conditional<G == 2, f32x2, f32x4>::type;
+    if (q * G + G <= a.nf) {
+      V v, e;
+#pragma unroll
+      for (int j = 0; j < G; j++) {
+        v[j] = o.v[j];
+        if constexpr (VAR) e[j] = o.e[j];
+      }
+      *reinterpret_cast<V*>(out) = v;
+      if constexpr (VAR) *reinterpret_cast<V*>(out + (size_t)a.nseg * F) = e;
+      return;
+    }
+  }
 #pragma unroll
   for (int j = 0; j < G; j++) {
     if (q * G + j < a.nf) {
