@@ -541,6 +541,17 @@ struct MetaLanes {
   }
 };
 
+// Metadata of one tile, read by every lane (rare path: it waits for its load
+// at once, for the reason MetaLanes::load does).
+__device__ __forceinline__ uint4 tile_meta_of(const TileArgs& a, uint32_t t) {
+  const u32x4 v = reinterpret_cast<const u32x4*>(a.tile_meta)[t];
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
+  return make_uint4((uint32_t)__builtin_amdgcn_readfirstlane((int)v.x),
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)v.y),
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)v.z),
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)v.w));
+}
+
 // Partials of the segments a slot takes in the first R rounds of a tile,
 // held in registers (seg = ~0u: none) and written later.
 template <int R, int G, int MODE, bool VAR> struct Pending {
@@ -858,18 +869,57 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   constexpr int R = SegRounds<SLOTS>::R;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
-  // Item i: the workgroup's tile i / H, pass i % H.
+  // Items: K = ntiles / grid full rounds give the workgroup's tiles
+  // t0 + k * grid, each for H passes in a row (item k * H + h). The H passes
+  // of the rem = ntiles - K * grid remaining tiles are dealt one item per
+  // workgroup (item K * H + e is item j = blockIdx.x + e * grid of the
+  // remainder), so the last round costs about one item, not H.
   const uint32_t H = a.npass;
   const uint32_t grid = gridDim.x;
   const uint32_t t0 = first_tile();
   if (t0 >= a.ntiles) return;
-  auto tile_of = [&](uint32_t i) { return t0 + (i / H) * grid; };
+  const uint32_t K = a.ntiles / grid, rem = a.ntiles - K * grid;
+  struct Item {
+    uint32_t tile, pass, tbuf;
+    bool valid, first, round;  // round: a full-round item (metadata in lanes)
+  };
+  auto item = [&](uint32_t i) {
+    Item it;
+    if (i < K * H) {
+      const uint32_t k = i / H;
+      it.tile = t0 + k * grid;
+      it.pass = i % H;
+      it.tbuf = k & 1;
+      it.valid = true;
+      it.first = it.pass == 0;
+      it.round = true;
+    } else {
+      const uint32_t e = i - K * H, j = blockIdx.x + e * grid;
+      it.tile = K * grid + j / H;
+      it.pass = j % H;
+      it.tbuf = (K + e) & 1;
+      it.valid = j < rem * H;
+      it.first = true;
+      it.round = false;
+    }
+    return it;
+  };
   auto frames_buf = [&](uint32_t i) { return lds_raw + (i & 1) * PL::kFrameBytes; };
-  auto table_buf = [&](uint32_t i) {
-    return lds_raw + PL::kTable0 + ((i / H) & 1) * PL::kTableBytes;
+  auto table_buf = [&](const Item& it) {
+    return lds_raw + PL::kTable0 + it.tbuf * PL::kTableBytes;
   };
   MetaLanes ml;
   ml.load(a, t0, grid, 0);
+  // Metadata of an item (zero past the last one).
+  auto meta = [&](uint32_t i, const Item& it) {
+    if (!it.valid) return make_uint4(0u, 0u, 0u, 0u);
+    if (it.round) {
+      const uint32_t k = i / H;
+      if (k - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k);
+      return ml.get(k);
+    }
+    return tile_meta_of(a, it.tile);
+  };
   pad_slots<F>(reinterpret_cast<LT*>(frames_buf(0)));
   pad_slots<F>(reinterpret_cast<LT*>(frames_buf(1)));
 
@@ -880,32 +930,34 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   pd.clear();
   uint32_t ph = 0;  // pass of the item whose partials are pending
   // Prologue: item 0 staged, item 1 loaded into the registers.
-  uint4 mA = ml.get(0);
-  uint32_t pn = db_prefetch<F, FA, MODE, C, SLOTS>(a, t0, mA, true, slot, fr, tr, sr);
+  Item iA = item(0);
+  if (!iA.valid) return;
+  uint4 mA = meta(0, iA);
+  uint32_t pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, iA.pass), iA.tile, mA, true,
+                                                   slot, fr, tr, sr);
   if (mA.x != mA.y) {
     db_store<F, LT>(a, fr, reinterpret_cast<LT*>(frames_buf(0)));
-    table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(0)),
-                                     table_buf(0) + (size_t)C * 8);
+    table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(iA)),
+                                     table_buf(iA) + (size_t)C * 8);
   }
   SegRegs<R> cA = sr;
-  uint4 mB = ml.get(1 / H);
-  if (tile_of(1) < a.ntiles)
-    pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, 1 % H), tile_of(1), mB, H == 1,
+  Item iB = item(1);
+  uint4 mB = meta(1, iB);
+  if (iB.valid)
+    pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, iB.pass), iB.tile, mB, iB.first,
                                             slot, fr, tr, sr);
   __syncthreads();
 
   for (uint32_t i = 0;; i++) {
-    if (tile_of(i) >= a.ntiles) break;
-    const uint32_t h = i % H;
+    if (!iA.valid) break;
     // 1. Item i + 1 (registers) into the other frame buffer, free since the
     //    last barrier, and its table chunk if it opens a tile (the other
     //    table buffer, last read by the previous tile).
     if (mB.x != mB.y) {
-      const uint32_t i1 = i + 1;
-      db_store<F, LT>(a, fr, reinterpret_cast<LT*>(frames_buf(i1)));
-      if (i1 % H == 0)
-        table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(i1)),
-                                         table_buf(i1) + (size_t)C * 8);
+      db_store<F, LT>(a, fr, reinterpret_cast<LT*>(frames_buf(i + 1)));
+      if (iB.first)
+        table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(iB)),
+                                         table_buf(iB) + (size_t)C * 8);
     }
     const SegRegs<R> cB = sr;
     // Every load of the last prefetch has landed on every path (the store
@@ -915,26 +967,27 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
     // 2. Partials of item i - 1.
     pending_write<F>(db_pass<F>(a, ph), q, pd);
     // 3. Item i + 2 into the registers.
-    const uint32_t i2 = i + 2, k2 = i2 / H;
-    if (k2 - ml.k0 >= kMetaLanes) ml.load(a, t0, grid, k2);
-    const uint4 mC = ml.get(k2);
-    if (tile_of(i2) < a.ntiles)
-      pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, i2 % H), tile_of(i2), mC,
-                                              i2 % H == 0, slot, fr, tr, sr);
-    // 4. Item i from its buffers. The second pass of a tile whose table
-    //    takes several chunks reloads the first (the last chunk is in LDS).
+    const Item iC = item(i + 2);
+    const uint4 mC = meta(i + 2, iC);
+    if (iC.valid)
+      pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, iC.pass), iC.tile, mC, iC.first,
+                                              slot, fr, tr, sr);
+    // 4. Item i from its buffers. A later pass of a tile whose table takes
+    //    several chunks reloads the first (the last chunk is in LDS).
     if (mA.x != mA.y) {
       const LT* base = reinterpret_cast<const LT*>(frames_buf(i)) + q * G * kLdsStride;
-      unsigned char* tb = table_buf(i);
-      const bool first_in_lds = h == 0 || mA.w - mA.z <= C;
+      unsigned char* tb = table_buf(iA);
+      const bool first_in_lds = iA.first || mA.w - mA.z <= C;
       tile_apply<F, G, MODE, VAR, C, 1, SLOTS, kDbThreads>(
-          db_pass<F>(a, h), mA, cA, slot, q, base, reinterpret_cast<u32x2*>(tb),
+          db_pass<F>(a, iA.pass), mA, cA, slot, q, base, reinterpret_cast<u32x2*>(tb),
           tb + (size_t)C * 8, &pd, first_in_lds);
     }
-    ph = h;
+    ph = iA.pass;
     __syncthreads();
+    iA = iB;
     mA = mB;
     cA = cB;
+    iB = iC;
     mB = mC;
   }
   pending_write<F>(db_pass<F>(a, ph), q, pd);
