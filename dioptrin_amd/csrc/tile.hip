@@ -1031,8 +1031,14 @@ __global__ __launch_bounds__(kFinalLanes * kFinalMaxQuads) void k_tfinal(
     const PT* __restrict__ part, uint32_t nseg, uint32_t FS, uint64_t BS,
     const uint32_t* __restrict__ bin_ptr,
     const uint32_t* __restrict__ bin_seg, const double* __restrict__ den, uint32_t nbins,
-    uint32_t nf, double* __restrict__ I, double* __restrict__ S) {
+    uint32_t ntot, uint32_t RF, double* __restrict__ I, double* __restrict__ S) {
   using AT = typename std::conditional<PLAIN, float, double>::type;
+  // Frame group blockIdx.y: frames [f0, f0 + nf) of the ntot, RF per group
+  // (a multiple of FS, so a group starts on a pass block).
+  const uint32_t f0 = blockIdx.y * RF, nf = min(RF, ntot - f0);
+  part += (uint64_t)(f0 / FS) * BS;
+  I += (uint64_t)f0 * nbins;
+  if constexpr (VAR) S += (uint64_t)f0 * nbins;
   constexpr int NV = VAR ? 2 : 1;
   __shared__ AT red[kFinalMaxQuads][kFinalMaxQuads][NV][FPT];  // [wavefront][quad][kind][frame]
   const uint32_t quads = blockDim.x / kFinalLanes;
@@ -1241,26 +1247,32 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
   DIO_HIP(hipGetLastError());
 }
 
-// Reduces nf frames of consecutive passes of F frames each; pass j's
-// partials are the block at part + j * BS (elements).
+// Reduces ntot frames of consecutive passes of F frames each (pass j's
+// partials are the block at part + j * BS elements), in groups of RF frames
+// (a multiple of F, at most 32), one launch for all groups.
 template <bool PLAIN, bool VAR, class PT>
-void launch_final(const TiledTable& t, const void* part, uint32_t F, uint64_t BS, uint32_t nf,
-                  double* I, double* S, hipStream_t s) {
+void launch_final(const TiledTable& t, const void* part, uint32_t F, uint64_t BS, uint32_t ntot,
+                  uint32_t RF, double* I, double* S, hipStream_t s) {
   const PT* p = static_cast<const PT*>(part);
   const uint32_t fpt = std::min(F, 4u);  // F is a power of two
+  const uint32_t nf = std::min(ntot, RF);
   unsigned quads = 1;
   while (quads * fpt < nf) quads *= 2;
-  if (quads > kFinalMaxQuads) fail(DIO_ERR_INVALID, "too many frames for one reduction");
+  if (quads > kFinalMaxQuads || RF % F) fail(DIO_ERR_INVALID, "invalid reduction shape");
   const unsigned threads = kFinalLanes * quads;
+  const dim3 grid(t.nbins, (ntot + RF - 1) / RF);
   if (fpt == 4)
-    k_tfinal<PLAIN, VAR, PT, 4><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
-                                                             t.bin_seg.p, t.den, t.nbins, nf, I, S);
+    k_tfinal<PLAIN, VAR, PT, 4><<<grid, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
+                                                          t.bin_seg.p, t.den, t.nbins, ntot, RF,
+                                                          I, S);
   else if (fpt == 2)
-    k_tfinal<PLAIN, VAR, PT, 2><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
-                                                             t.bin_seg.p, t.den, t.nbins, nf, I, S);
+    k_tfinal<PLAIN, VAR, PT, 2><<<grid, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
+                                                          t.bin_seg.p, t.den, t.nbins, ntot, RF,
+                                                          I, S);
   else
-    k_tfinal<PLAIN, VAR, PT, 1><<<t.nbins, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
-                                                             t.bin_seg.p, t.den, t.nbins, nf, I, S);
+    k_tfinal<PLAIN, VAR, PT, 1><<<grid, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
+                                                          t.bin_seg.p, t.den, t.nbins, ntot, RF,
+                                                          I, S);
 }
 
 size_t dtype_bytes(int dtype) {
@@ -1438,7 +1450,7 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   const bool db = !f64 && double_buffered(t, F, dtype, precision);
   const uint32_t P = f64 ? 1 : db ? kMaxPassesPerLaunch : 2;
   const uint32_t FR = F * P;
-  const uint32_t RF = std::min(FR, 32u);  // frames per reduction launch
+  const uint32_t RF = std::min(FR, 32u);  // frames per reduction workgroup
   const uint64_t BS = (uint64_t)t.nseg * F * (var ? 2 : 1);  // elements per pass block
   const size_t pbytes = BS * (f64 ? sizeof(double) : sizeof(float));
   auto timed = [&](uint32_t frames_in_launch, auto&& launch) {
@@ -1469,20 +1481,19 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
         });
       }
     }
-    for (uint32_t r0 = 0; r0 < ng; r0 += RF) {
-      const uint32_t nr = std::min(RF, ng - r0);
-      const void* pr = reinterpret_cast<const char*>(sc.ptr()) + (r0 / F) * pbytes;
-      double* I = intensity + (g0 + r0) * t.nbins;
-      double* S = var ? sigma + (g0 + r0) * t.nbins : nullptr;
+    {
+      const void* pr = sc.ptr();
+      double* I = intensity + g0 * t.nbins;
+      double* S = var ? sigma + g0 * t.nbins : nullptr;
       if (f64) {
-        if (var) launch_final<false, true, double>(t, pr, F, BS, nr, I, S, s);
-        else launch_final<false, false, double>(t, pr, F, BS, nr, I, S, s);
+        if (var) launch_final<false, true, double>(t, pr, F, BS, ng, RF, I, S, s);
+        else launch_final<false, false, double>(t, pr, F, BS, ng, RF, I, S, s);
       } else if (plain) {
-        if (var) launch_final<true, true, float>(t, pr, F, BS, nr, I, S, s);
-        else launch_final<true, false, float>(t, pr, F, BS, nr, I, S, s);
+        if (var) launch_final<true, true, float>(t, pr, F, BS, ng, RF, I, S, s);
+        else launch_final<true, false, float>(t, pr, F, BS, ng, RF, I, S, s);
       } else {
-        if (var) launch_final<false, true, float>(t, pr, F, BS, nr, I, S, s);
-        else launch_final<false, false, float>(t, pr, F, BS, nr, I, S, s);
+        if (var) launch_final<false, true, float>(t, pr, F, BS, ng, RF, I, S, s);
+        else launch_final<false, false, float>(t, pr, F, BS, ng, RF, I, S, s);
       }
     }
     DIO_HIP(hipGetLastError());
