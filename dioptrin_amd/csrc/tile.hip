@@ -995,18 +995,20 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
 
 // prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =
 // s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0. One launch
-// reduces nf frames of consecutive passes: pass j's partials are the block at
-// part + j * BS, rows of FS frames. A workgroup serves one bin: 64 segment
-// lanes times Q frame quads of FPT frames (Q = ceil(nf / FPT) rounded up to a
-// power of two, so a wavefront holds whole segment lanes; quads past nf read
-// columns that are not stored, and no quad straddles two passes as FPT
-// divides FS). Segment lane l adds the bin's partials l, l + 64, l + 128, ... (tile
-// order), FPT frames at a time with one vector load; the 64 lane sums are then
-// combined by a fixed pairwise tree (l with l ^ 1, then l ^ 2, ... l ^ 32),
-// inside the wavefront by lane exchange and across wavefronts through LDS.
-// The order depends only on the table, never on the pass or batch shape.
+// reduces the frames of consecutive passes: pass j's partials are the block
+// at part + j * BS, rows of FS frames. A workgroup serves one bin and one
+// group of at most 32 frames (blockIdx.y): 64 segment lanes times Q frame
+// quads of FPT frames (Q = ceil(frames / FPT) rounded up to a power of two,
+// so a wavefront holds whole segment lanes; quads past the group's frames
+// read columns that are not stored, and no quad straddles two passes as FPT
+// divides FS). Segment lane l adds the bin's partials l, l + 64, l + 128, ...
+// (tile order), FPT frames at a time with one vector load; the 64 lane sums
+// are then combined by a fixed pairwise tree (l with l ^ 1, then l ^ 2, ...
+// l ^ 32), inside the wavefront by lane exchange and across wavefronts
+// through LDS. The order depends only on the table, never on the pass or
+// batch shape.
 constexpr unsigned kFinalLanes = 64;
-constexpr unsigned kFinalMaxQuads = 8;  // at most 32 frames per launch
+constexpr unsigned kFinalMaxQuads = 8;  // at most 32 frames per workgroup
 
 template <class PT, int FPT> struct PartVec { PT v[FPT]; };
 
