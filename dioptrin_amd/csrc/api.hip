@@ -389,12 +389,18 @@ int dio_plan_autotune(dio_plan* p, int dtype, uint64_t n_frames_hint, uint32_t f
     hipEvent_t e0, e1;
     DIO_HIP(hipEventCreate(&e0));
     DIO_HIP(hipEventCreate(&e1));
-    double best = 1e300;
-    uint32_t best_tile = p->tile, best_lanes = p->lanes;
+    // Two sweeps over the candidates; each keeps its faster sweep, so a
+    // candidate timed during a transient slowdown is not ruled out by it.
+    std::vector<std::pair<uint32_t, uint32_t>> cand;
     for (uint32_t tile : {1u, 2u, 4u, 8u, 16u, 32u}) {
       if (tile > 1 && tile > nf * 2) break;
-      for (uint32_t lanes : {1u, 2u, 4u}) {
-        if (!dio::valid_tile(p->desc.precision, tile, lanes)) continue;
+      for (uint32_t lanes : {1u, 2u, 4u})
+        if (dio::valid_tile(p->desc.precision, tile, lanes)) cand.emplace_back(tile, lanes);
+    }
+    std::vector<double> t_best(cand.size(), 1e300);
+    for (int sweep = 0; sweep < 2; sweep++) {
+      for (size_t c = 0; c < cand.size(); c++) {
+        const uint32_t tile = cand[c].first, lanes = cand[c].second;
         dio::tiled_gather(p->tt, p->sc, fr.p, dtype, nf, var, p->desc.precision, tile, lanes,
                           oi.p, os.p, p->stream);  // warm-up
         DIO_HIP(hipEventRecord(e0, p->stream));
@@ -405,13 +411,17 @@ int dio_plan_autotune(dio_plan* p, int dtype, uint64_t n_frames_hint, uint32_t f
         DIO_HIP(hipEventSynchronize(e1));
         float ms = 0;
         DIO_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-          best = ms;
-          best_tile = tile;
-          best_lanes = lanes;
-        }
+        t_best[c] = std::min(t_best[c], (double)ms);
       }
     }
+    uint32_t best_tile = p->tile, best_lanes = p->lanes;
+    double best = 1e300;
+    for (size_t c = 0; c < cand.size(); c++)
+      if (t_best[c] < best) {
+        best = t_best[c];
+        best_tile = cand[c].first;
+        best_lanes = cand[c].second;
+      }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     p->tile = best_tile;
