@@ -265,9 +265,10 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if (tr.get("tile") == [tile, lanes] and tr.get("precision") == args.precision
-                and tr.get("frames_per_launch") == fl):
-            traffic = tr.get("hbm_bytes_per_launch")
+        if tr.get("precision") == args.precision and tr.get("frames_per_launch") == fl:
+            for e in tr.get("entries", []):
+                if e.get("tile") == [tile, lanes]:
+                    traffic = e.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
