@@ -15,6 +15,7 @@ DIO_OK, DIO_ERR_UNAVAILABLE, DIO_ERR_UNSUPPORTED, DIO_ERR_INVALID, DIO_ERR_DEVIC
 METHODS = {"counting": 0, "pixel_split": 1, "pyfai_full": 2, "supersampled": 3}
 UNITS = {"2th_rad": 0, "2th_deg": 1, "q_A^-1": 2, "q_nm^-1": 3}
 PRECISIONS = {"f32": 0, "f32_compensated": 1, "f64": 2, "f32_wide": 3}
+ORDERS = {"detector": 0, "flip_vertical": 1}
 DTYPES = {"uint16": 0, "uint32": 1, "int32": 2, "float32": 3, "float64": 4}
 WITH_ERRORS, WITH_AZIMUTHAL_STD, DEVICE_FRAMES = 1, 2, 4
 
@@ -56,7 +57,8 @@ class PlanDesc(C.Structure):
         ("radial_min", C.c_double), ("radial_max", C.c_double),
         ("has_azimuthal_range", C.c_int32), ("azimuthal_min", C.c_double),
         ("azimuthal_max", C.c_double), ("has_polarization", C.c_int32),
-        ("polarization_factor", C.c_double), ("mask", C.c_void_p), ("precision", C.c_int32)]
+        ("polarization_factor", C.c_double), ("mask", C.c_void_p), ("precision", C.c_int32),
+        ("input_order", C.c_int32)]
 
 
 class PlanInfo(C.Structure):

@@ -67,6 +67,8 @@ void check_desc(const dio_plan_desc& d) {
   require(d.unit >= DIO_UNIT_TTH_RAD && d.unit <= DIO_UNIT_Q_NM, DIO_ERR_INVALID, "unknown unit");
   require(d.precision >= DIO_F32 && d.precision <= DIO_F32_WIDE, DIO_ERR_INVALID,
           "unknown precision");
+  require(d.input_order == DIO_ORDER_DETECTOR || d.input_order == DIO_ORDER_FLIP_VERTICAL,
+          DIO_ERR_INVALID, "unknown input order");
   if (d.has_radial_range)
     require(std::isfinite(d.radial_min) && std::isfinite(d.radial_max) &&
                 d.radial_min < d.radial_max,
@@ -127,6 +129,7 @@ dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& des
     dio::build_tiled(p->tt, desc.dim1, desc.dim2, t.nbins, t.nnz, t.row_ptr.p, t.col.p,
                      t.w32.p, t.w64.p, desc.precision == DIO_F64, p->stream);
     p->tt.den = p->b.den.p;
+    p->tt.flip = desc.input_order == DIO_ORDER_FLIP_VERTICAL;
     p->tt.cus = (uint32_t)std::max(1, ctx->prop.multiProcessorCount);
     // DIOPTRIN_TILE_WORKGROUPS caps the gather's persistent grid (tuning, and
     // tests of the many-tiles-per-workgroup path on small detectors).
@@ -145,6 +148,9 @@ dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& des
     p->st.norm = p->b.norm.p;
     p->st.nbins = p->b.full.nbins;
     p->st.npix = p->gt.npix;
+    p->st.dim1 = desc.dim1;
+    p->st.dim2 = desc.dim2;
+    p->st.flip = desc.input_order == DIO_ORDER_FLIP_VERTICAL;
     p->build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return p.release();

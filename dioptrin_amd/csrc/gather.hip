@@ -31,7 +31,8 @@ __global__ __launch_bounds__(kBlock) void k_std(const uint32_t* __restrict__ row
                                                 const uint8_t* __restrict__ mask,
                                                 const double* __restrict__ norm,
                                                 const P* __restrict__ frames, uint64_t npix,
-                                                uint32_t nbins, int errors, double* I, double* S,
+                                                uint32_t nbins, uint32_t dim1, uint32_t dim2,
+                                                int flip, int errors, double* I, double* S,
                                                 double* SD) {
   uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   uint32_t r = blockIdx.x * kWavesPerBlock + wave;
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_std(const uint32_t* __restrict__ row
     if (mask && mask[j]) return false;
     double n = norm[j];
     c = (double)w32[k];
-    s = (double)fr[j];
+    s = (double)fr[flip ? (dim1 - 1 - j / dim2) * dim2 + j % dim2 : j];
     x = s / n;
     w = c * n;
     return isfinite(s) && isfinite(n) && n > 0.0 && isfinite(c) && c > 0.0 && isfinite(x) &&
@@ -111,7 +112,8 @@ void azimuthal_std_frames(const StdTable& t, const void* frames, int dtype, uint
   switch (dtype) {
 #define DIO_STD(T)                                                                            \
   k_std<T><<<grid, kBlock, 0, s>>>(t.row_ptr, t.col, t.w32, t.mask, t.norm,                  \
-                                   static_cast<const T*>(frames), t.npix, t.nbins, errors,     \
+                                   static_cast<const T*>(frames), t.npix, t.nbins, t.dim1,     \
+                                   t.dim2, t.flip ? 1 : 0, errors,                             \
                                    intensity, sigma, std_out);                                 \
   break;
     case DIO_U16: DIO_STD(uint16_t)

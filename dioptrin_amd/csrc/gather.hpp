@@ -23,6 +23,8 @@ struct StdTable {
   const double* norm = nullptr;
   uint32_t nbins = 0;
   uint64_t npix = 0;
+  uint32_t dim1 = 0, dim2 = 0;
+  bool flip = false;  // frames hold the detector's rows bottom-up
 };
 
 // azimuthal.rs:48-118 for n_frames device frames: intensity, sigma (when

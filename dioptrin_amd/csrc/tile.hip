@@ -288,7 +288,7 @@ __device__ __forceinline__ TileGeom<NT> tile_geom(const TileArgs& a, uint32_t t)
   for (int h = 0; h < Staging<NT>::kPasses; h++) {
     const uint32_t r = (t / a.ntx) * kTileRows + g.r0 + Staging<NT>::kRowsPerPass * h;
     g.nvalid[h] = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
-    g.off[h] = r * a.dim2 + c;
+    g.off[h] = (a.flip && g.nvalid[h] ? a.dim1 - 1 - r : r) * a.dim2 + c;
   }
   return g;
 }
@@ -799,7 +799,7 @@ __device__ __forceinline__ void db_load(const TileArgs& a, uint32_t t, DbFrames<
   const uint32_t c = (t % a.ntx) * kTileCols + cp * 2;
   const uint32_t r = (t / a.ntx) * kTileRows + r0;
   const uint32_t nvalid = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
-  const uint32_t off = r * a.dim2 + c;
+  const uint32_t off = (a.flip && nvalid ? a.dim1 - 1 - r : r) * a.dim2 + c;
   // Unconditional loads from valid addresses, as in frames_load32.
   const uint32_t o0 = nvalid > 0 ? off * 4u : 0u, o1 = nvalid > 1 ? off * 4u + 4u : 0u;
   const unsigned char* src = static_cast<const unsigned char*>(a.frames);
@@ -1230,6 +1230,7 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
     fail(DIO_ERR_INVALID, "invalid passes per launch");
   a.npass = npass;
   a.nf_last = nf;
+  a.flip = t.flip ? 1u : 0u;
   a.nf = npass > 1 ? F : nf;
   a.pblock = pblock;
   if (npass > 1 && !double_buffered(t, F, dtype, precision))

@@ -109,6 +109,7 @@ struct TileArgs {
   uint32_t nf;
   uint32_t aligned8;  // frame rows and blocks are 8-byte aligned (32-bit frames)
   void* part;
+  uint32_t flip = 0;  // frames hold the detector's rows bottom-up (row r is frame row dim1-1-r)
   // Double-buffered kernel only: npass passes over the same table in one
   // launch. Pass h reads frames h * F onwards and writes its partials pblock
   // bytes after pass h - 1's; passes before the last hold F frames, the last

@@ -103,9 +103,12 @@ def device_info(device: int = 0) -> dict:
 
 def plan_args(poni: dict, shape, method="pixel_split", n_ss=None, unit="2th_rad",
               n_radial=1000, n_azimuthal=None, radial_range=None, azimuthal_range=None,
-              polarization_factor=None, mask=None, precision="f32_compensated"):
+              polarization_factor=None, mask=None, precision="f32_compensated",
+              input_order="detector"):
     """The (dio_poni, dio_plan_desc, mask array) triple naming one plan. The
-    mask array must stay alive while the descriptor is in use."""
+    mask array must stay alive while the descriptor is in use. input_order
+    "flip_vertical" (InputOrder::FlipVertical, fixed.rs:20-24) reads frames
+    whose rows arrive bottom-up."""
     if method not in N.METHODS:
         raise ValueError(f"unknown method {method!r}")
     if unit not in N.UNITS:
@@ -113,6 +116,9 @@ def plan_args(poni: dict, shape, method="pixel_split", n_ss=None, unit="2th_rad"
     if precision not in N.PRECISIONS:
         raise ValueError(f"unknown precision {precision!r}; expected one of "
                          f"{sorted(N.PRECISIONS)}")
+    if input_order not in N.ORDERS:
+        raise ValueError(f"unknown input order {input_order!r}; expected one of "
+                         f"{sorted(N.ORDERS)}")
     dim1, dim2 = int(shape[0]), int(shape[1])
     p = N.Poni(*(float(poni.get(k, 0.0)) for k in (
         "pixel1", "pixel2", "distance", "poni1", "poni2", "rot1", "rot2", "rot3",
@@ -141,6 +147,7 @@ def plan_args(poni: dict, shape, method="pixel_split", n_ss=None, unit="2th_rad"
             raise ValueError(f"mask shape {m.shape} does not match image shape {(dim1, dim2)}")
         d.mask = m.ctypes.data
     d.precision = N.PRECISIONS[precision]
+    d.input_order = N.ORDERS[input_order]
     return p, d, m
 
 
@@ -150,14 +157,16 @@ class GpuPlan:
 
     def __init__(self, poni: dict, shape, method="pixel_split", n_ss=None, unit="2th_rad",
                  n_radial=1000, n_azimuthal=None, radial_range=None, azimuthal_range=None,
-                 polarization_factor=None, mask=None, precision="f32_compensated", device=0):
+                 polarization_factor=None, mask=None, precision="f32_compensated", device=0,
+                 input_order="detector"):
         self.shape = (int(shape[0]), int(shape[1]))
+        self.input_order = input_order
         self.ndim = 1 if n_azimuthal is None else 2
         self.precision = precision
         self.device = device
         p, d, self._mask = plan_args(poni, shape, method, n_ss, unit, n_radial, n_azimuthal,
                                      radial_range, azimuthal_range, polarization_factor, mask,
-                                     precision)
+                                     precision, input_order)
         h = C.c_void_p()
         N.check(N.lib().dio_plan_create(_Context.get(device), C.byref(p), C.byref(d), C.byref(h)))
         self._h = h
@@ -426,17 +435,20 @@ class Integrator:
         return plan
 
     def prepare_gpu1d(self, shape, n_points: int, *, precision: str = "f32_compensated",
-                      device: Optional[int] = None) -> GpuPlan:
-        """prepare_cuda1d counterpart: an independent plan snapshot."""
+                      device: Optional[int] = None, input_order: str = "detector") -> GpuPlan:
+        """prepare_cuda1d counterpart: an independent plan snapshot
+        (input_order as Cuda1dOptions::with_input_order, cuda.rs:174-181)."""
         return GpuPlan(self.poni, shape, self.method, self.n_ss, self.unit, n_points, None,
                        self.radial_range, None, self.polarization_factor, self.mask, precision,
-                       self.device if device is None else device)
+                       self.device if device is None else device, input_order)
 
     def prepare_gpu2d(self, shape, n_radial: int, n_azimuthal: int, *, azimuthal_range=None,
-                      precision: str = "f32_compensated", device: Optional[int] = None) -> GpuPlan:
+                      precision: str = "f32_compensated", device: Optional[int] = None,
+                      input_order: str = "detector") -> GpuPlan:
         return GpuPlan(self.poni, shape, self.method, self.n_ss, self.unit, n_radial,
                        n_azimuthal, self.radial_range, azimuthal_range, self.polarization_factor,
-                       self.mask, precision, self.device if device is None else device)
+                       self.mask, precision, self.device if device is None else device,
+                       input_order)
 
     def integrate1d(self, image: np.ndarray, n_points: int, errors: bool = False, *,
                     azimuthal_std: bool = False) -> IntegrationResult:

@@ -37,6 +37,12 @@ enum { DIO_UNIT_TTH_RAD = 0, DIO_UNIT_TTH_DEG = 1, DIO_UNIT_Q_A = 2, DIO_UNIT_Q_
 enum { DIO_F32 = 0, DIO_F32_COMPENSATED = 1, DIO_F64 = 2, DIO_F32_WIDE = 3 };
 /* Frame element types (CudaPixel, src/integration/cuda.rs:354-367). */
 enum { DIO_U16 = 0, DIO_U32 = 1, DIO_I32 = 2, DIO_F32T = 3, DIO_F64T = 4 };
+/* Row order of incoming frames (InputOrder, src/integration/fixed.rs:20-24;
+ * Cuda1dOptions::with_input_order, src/integration/cuda.rs:174-181):
+ * DIO_ORDER_FLIP_VERTICAL frames hold the detector's rows bottom-up. The plan
+ * reads them in that order; results are the bits of the upright frames with
+ * a DIO_ORDER_DETECTOR plan. Exported tables stay in detector order. */
+enum { DIO_ORDER_DETECTOR = 0, DIO_ORDER_FLIP_VERTICAL = 1 };
 /* Output selection flags; DIO_DEVICE_FRAMES (dio_server_submit only): the
  * frames pointer is device memory of the server's context. */
 enum { DIO_WITH_ERRORS = 1, DIO_WITH_AZIMUTHAL_STD = 2, DIO_DEVICE_FRAMES = 4 };
@@ -70,6 +76,7 @@ typedef struct {
   double polarization_factor;
   const uint8_t *mask;          /* host, dim1*dim2, nonzero = masked; may be NULL */
   int32_t precision;            /* DIO_F32 .. DIO_F32_WIDE */
+  int32_t input_order;          /* DIO_ORDER_DETECTOR (0) or DIO_ORDER_FLIP_VERTICAL */
 } dio_plan_desc;
 
 typedef struct {

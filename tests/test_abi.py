@@ -67,3 +67,23 @@ def test_invalid_configuration_fails_before_the_device():
     d.dim1, d.dim2, d.ndim, d.n_radial = 0, 10, 1, 10
     rc = L.dio_plan_create(None, ctypes.byref(p), ctypes.byref(d), ctypes.byref(ctypes.c_void_p()))
     assert rc == N.DIO_ERR_INVALID
+
+
+def test_input_order_descriptor():
+    # InputOrder (fixed.rs:20-24): the header's values, the Python names and
+    # the descriptor field agree; an unknown order is rejected on the host.
+    import re
+    from dioptrin_amd import _native as N
+    from dioptrin_amd.integrator import plan_args
+    hdr = open(os.path.join(ROOT, "include", "dioptrin_hip.h")).read()
+    m = re.search(r"DIO_ORDER_DETECTOR = (\d+), DIO_ORDER_FLIP_VERTICAL = (\d+)", hdr)
+    assert m and N.ORDERS == {"detector": int(m.group(1)), "flip_vertical": int(m.group(2))}
+    assert re.search(r"int32_t precision;[^\n]*\n\s*int32_t input_order;", hdr)
+    assert [f for f, _ in N.PlanDesc._fields_][-2:] == ["precision", "input_order"]
+    poni = dict(pixel1=1e-4, pixel2=1e-4, distance=0.1, wavelength=1e-10)
+    _, d, _ = plan_args(poni, (8, 9), input_order="flip_vertical")
+    assert d.input_order == N.ORDERS["flip_vertical"]
+    _, d, _ = plan_args(poni, (8, 9))
+    assert d.input_order == N.ORDERS["detector"]
+    with pytest.raises(ValueError):
+        plan_args(poni, (8, 9), input_order="sideways")

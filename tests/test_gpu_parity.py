@@ -221,6 +221,52 @@ def test_double_buffered_equals_single_buffered(pilatus, tile, monkeypatch):
         assert np.array_equal(s0, s1)
 
 
+FLIP_CASES = [(1, 1, np.uint32), (16, 2, np.uint32), (16, 4, np.float32), (8, 2, np.uint32),
+              (16, 2, np.uint16), (4, 4, np.float64)]
+
+
+@pytest.mark.parametrize("tile,lanes,dtype", FLIP_CASES,
+                         ids=[f"F{f}G{g}-{np.dtype(t).name}" for f, g, t in FLIP_CASES])
+def test_flip_vertical_input_is_bitwise_detector_order(pilatus, tile, lanes, dtype):
+    """InputOrder::FlipVertical (fixed.rs:20-24): frames whose rows arrive
+    bottom-up, integrated by a flip-order plan, give the bits of the upright
+    frames with a detector-order plan (cuda/tests.rs:528-562), for intensity,
+    sigma and azimuthal std, through the per-frame, single- and
+    double-buffered tile kernels and every staging path."""
+    d = pilatus
+    rng = np.random.default_rng(21)
+    frames = rng.poisson(150.0, size=(21,) + d["w"]["shape"]).astype(dtype)
+    flipped = np.ascontiguousarray(frames[:, ::-1, :])
+    out = {}
+    for order, fr in (("detector", frames), ("flip_vertical", flipped)):
+        plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
+                       polarization_factor=0.99, mask=d["mask"], precision="f32_wide",
+                       input_order=order)
+        plan.set_tile(tile, lanes)
+        out[order] = (plan.integrate_stack(fr, errors=True)[:2],
+                      plan.integrate_stack(fr[:3], errors=True, azimuthal_std=True))
+    (i0, s0), std0 = out["detector"]
+    (i1, s1), std1 = out["flip_vertical"]
+    assert np.array_equal(i0, i1) and np.array_equal(s0, s1)
+    for a, b in zip(std0, std1):
+        assert np.array_equal(a, b, equal_nan=True)
+
+
+def test_flip_vertical_2d():
+    """The flipped row order for a 2D table (500 x 360 bins, tiles of several
+    LDS chunks): the same bits as the upright frames."""
+    g = GEOMETRIES_2D[-1]
+    name, poni, shape, nr, na = g[:5]
+    frames = np.random.default_rng(22).poisson(40.0, size=(17,) + shape).astype(np.uint32)
+    res = []
+    for order, fr in (("detector", frames),
+                      ("flip_vertical", np.ascontiguousarray(frames[:, ::-1, :]))):
+        plan = GpuPlan(poni, shape, n_radial=nr, n_azimuthal=na, precision="f32_wide",
+                       input_order=order)
+        res.append(plan.integrate_stack(fr, errors=True)[:2])
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
 def test_pixel_dtypes_equal_float64(pilatus):
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
