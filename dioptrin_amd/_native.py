@@ -17,13 +17,14 @@ UNITS = {"2th_rad": 0, "2th_deg": 1, "q_A^-1": 2, "q_nm^-1": 3}
 PRECISIONS = {"f32": 0, "f32_compensated": 1, "f64": 2, "f32_wide": 3}
 ORDERS = {"detector": 0, "flip_vertical": 1}
 DTYPES = {"uint16": 0, "uint32": 1, "int32": 2, "float32": 3, "float64": 4}
-WITH_ERRORS, WITH_AZIMUTHAL_STD, DEVICE_FRAMES = 1, 2, 4
+WITH_ERRORS, WITH_AZIMUTHAL_STD, DEVICE_FRAMES, RAW_SUMS = 1, 2, 4, 8
 
 # Exported symbols, in include/dioptrin_hip.h order.
 EXPORTS = [
     "dio_last_error", "dio_version", "dio_open", "dio_close", "dio_device_info_get",
     "dio_plan_create", "dio_plan_destroy", "dio_plan_info_get", "dio_plan_radial",
     "dio_plan_azimuthal", "dio_plan_export_csr", "dio_plan_denominators", "dio_plan_normalization",
+    "dio_plan_counts",
     "dio_integrate", "dio_integrate_device", "dio_plan_autotune", "dio_plan_set_tile",
     "dio_graph_create", "dio_graph_launch", "dio_graph_destroy", "dio_plan_time_gather",
     "dio_plan_time_passes", "dio_device_alloc", "dio_device_free", "dio_memcpy", "dio_synchronize",
@@ -119,6 +120,7 @@ def lib() -> C.CDLL:
             "dio_plan_export_csr": (i32, [vp, i32, vp, vp, vp, vp, u64]),
             "dio_plan_denominators": (i32, [vp, dp]),
             "dio_plan_normalization": (i32, [vp, dp]),
+            "dio_plan_counts": (i32, [vp, dp]),
             "dio_integrate": (i32, [vp, vp, i32, u64, u32, dp, dp, dp]),
             "dio_integrate_device": (i32, [vp, vp, i32, u64, u32, dp, dp, dp, vp]),
             "dio_plan_autotune": (i32, [vp, i32, u64, u32, C.POINTER(u32), C.POINTER(u32)]),

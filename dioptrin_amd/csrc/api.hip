@@ -93,6 +93,8 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
                 hipStream_t s, LaunchEvents* launches) {
   bool var = flags & DIO_WITH_ERRORS;
   if (flags & DIO_WITH_AZIMUTHAL_STD) {
+    require(!(flags & DIO_RAW_SUMS), DIO_ERR_UNSUPPORTED,
+            "raw sums are not defined with the azimuthal std");
     require(p->desc.ndim == 1, DIO_ERR_UNSUPPORTED, "azimuthal std is defined for 1D plans");
     require(SD != nullptr, DIO_ERR_INVALID, "azimuthal std output missing");
     azimuthal_std_frames(p->st, frames, dtype, n_frames, var, I, S, SD, s);
@@ -100,7 +102,7 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
   }
   require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
   tiled_gather(p->tt, sc, frames, dtype, n_frames, var, p->desc.precision, p->tile, p->lanes, I,
-               S, s, launches);
+               S, s, launches, (flags & DIO_RAW_SUMS) != 0);
 }
 
 }  // namespace dio
@@ -300,6 +302,18 @@ int dio_plan_denominators(dio_plan* p, double* out) {
     require(p && out, DIO_ERR_INVALID, "null argument");
     DIO_HIP(hipSetDevice(p->device));
     DIO_HIP(hipMemcpy(out, p->b.den.p, p->b.den.bytes(), hipMemcpyDeviceToHost));
+  });
+}
+
+int dio_plan_counts(dio_plan* p, double* out) {
+  return guarded([&] {
+    require(p && out, DIO_ERR_INVALID, "null argument");
+    DIO_HIP(hipSetDevice(p->device));
+    dio::DevBuf<double> c(std::max<uint32_t>(p->gt.nbins, 1));
+    dio::bin_counts(p->b, c.p, p->stream);
+    DIO_HIP(hipMemcpyAsync(out, c.p, (size_t)p->gt.nbins * sizeof(double),
+                           hipMemcpyDeviceToHost, p->stream));
+    DIO_HIP(hipStreamSynchronize(p->stream));
   });
 }
 

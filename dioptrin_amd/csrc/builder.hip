@@ -1069,6 +1069,17 @@ __global__ void k_denominators(const uint32_t* row_ptr, const uint32_t* col, con
   den[r] = n;
 }
 
+// lut/integrate.rs:52-110 (row_strict, COUNT): count = count + w in row
+// order over the execution table (+ 1 per entry for unweighted tables).
+__global__ void k_counts(const uint32_t* row_ptr, const double* w64, int weighted,
+                         uint32_t nbins, double* cnt) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nbins) return;
+  double c = 0.0;
+  for (uint32_t k = row_ptr[r]; k < row_ptr[r + 1]; k++) c = c + (weighted ? w64[k] : 1.0);
+  cnt[r] = c;
+}
+
 // ---------------------------------------------------------------- host
 
 struct Scratch {
@@ -1951,6 +1962,13 @@ void build_tables(const Geo& g, const dio_plan_desc& d, hipStream_t s, Built& ou
                                                         t.nbins, out.den.p);
   DIO_HIP(hipGetLastError());
   DIO_HIP(hipStreamSynchronize(s));
+}
+
+void bin_counts(const Built& b, double* cnt, hipStream_t s) {
+  const Table& t = b.exec_table();
+  k_counts<<<blocks_for(t.nbins, 64), 64, 0, s>>>(t.row_ptr.p, t.w64.p, b.weighted ? 1 : 0,
+                                                  t.nbins, cnt);
+  DIO_HIP(hipGetLastError());
 }
 
 }  // namespace dio

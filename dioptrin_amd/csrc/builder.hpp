@@ -36,5 +36,8 @@ struct Built {
 
 Geo make_geo(const dio_poni& p, const dio_plan_desc& d);
 void build_tables(const Geo& geo, const dio_plan_desc& d, hipStream_t s, Built& out);
+// Per bin of the execution table: sum of its weights in row order (entry
+// count for unweighted tables), enqueued on s.
+void bin_counts(const Built& b, double* cnt, hipStream_t s);
 
 }  // namespace dio

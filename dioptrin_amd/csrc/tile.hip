@@ -1033,7 +1033,7 @@ __global__ __launch_bounds__(kFinalLanes * kFinalMaxQuads) void k_tfinal(
     const PT* __restrict__ part, uint32_t nseg, uint32_t FS, uint64_t BS,
     const uint32_t* __restrict__ bin_ptr,
     const uint32_t* __restrict__ bin_seg, const double* __restrict__ den, uint32_t nbins,
-    uint32_t ntot, uint32_t RF, double* __restrict__ I, double* __restrict__ S) {
+    uint32_t ntot, uint32_t RF, int raw, double* __restrict__ I, double* __restrict__ S) {
   using AT = typename std::conditional<PLAIN, float, double>::type;
   // Frame group blockIdx.y: frames [f0, f0 + nf) of the ntot, RF per group
   // (a multiple of FS, so a group starts on a pass block).
@@ -1127,6 +1127,11 @@ __global__ __launch_bounds__(kFinalLanes * kFinalMaxQuads) void k_tfinal(
   for (int k = 0; k < FPT; k++) {
     const uint32_t f = q * FPT + k;
     if (f >= nf) continue;
+    if (raw) {  // sums before division (lut/integrate.rs:372-500)
+      I[(uint64_t)f * nbins + b] = (double)acc[0][k];
+      if constexpr (VAR) S[(uint64_t)f * nbins + b] = (double)acc[NV - 1][k];
+      continue;
+    }
     I[(uint64_t)f * nbins + b] = d == 0.0 ? 0.0 : (double)acc[0][k] / d;
     if constexpr (VAR) {
       const double vv = d == 0.0 ? 0.0 : (double)acc[NV - 1][k] / (d * d);
@@ -1252,10 +1257,11 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
 
 // Reduces ntot frames of consecutive passes of F frames each (pass j's
 // partials are the block at part + j * BS elements), in groups of RF frames
-// (a multiple of F, at most 32), one launch for all groups.
+// (a multiple of F, at most 32), one launch for all groups. raw: the sums
+// before division instead of I and sigma.
 template <bool PLAIN, bool VAR, class PT>
 void launch_final(const TiledTable& t, const void* part, uint32_t F, uint64_t BS, uint32_t ntot,
-                  uint32_t RF, double* I, double* S, hipStream_t s) {
+                  uint32_t RF, bool raw, double* I, double* S, hipStream_t s) {
   const PT* p = static_cast<const PT*>(part);
   const uint32_t fpt = std::min(F, 4u);  // F is a power of two
   const uint32_t nf = std::min(ntot, RF);
@@ -1267,15 +1273,15 @@ void launch_final(const TiledTable& t, const void* part, uint32_t F, uint64_t BS
   if (fpt == 4)
     k_tfinal<PLAIN, VAR, PT, 4><<<grid, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
                                                           t.bin_seg.p, t.den, t.nbins, ntot, RF,
-                                                          I, S);
+                                                          raw ? 1 : 0, I, S);
   else if (fpt == 2)
     k_tfinal<PLAIN, VAR, PT, 2><<<grid, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
                                                           t.bin_seg.p, t.den, t.nbins, ntot, RF,
-                                                          I, S);
+                                                          raw ? 1 : 0, I, S);
   else
     k_tfinal<PLAIN, VAR, PT, 1><<<grid, threads, 0, s>>>(p, t.nseg, F, BS, t.bin_ptr.p,
                                                           t.bin_seg.p, t.den, t.nbins, ntot, RF,
-                                                          I, S);
+                                                          raw ? 1 : 0, I, S);
 }
 
 size_t dtype_bytes(int dtype) {
@@ -1438,7 +1444,8 @@ bool double_buffered(const TiledTable& t, uint32_t F, int dtype, int precision) 
 
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t F, uint32_t G,
-                  double* intensity, double* sigma, hipStream_t s, LaunchEvents* le) {
+                  double* intensity, double* sigma, hipStream_t s, LaunchEvents* le,
+                  bool raw) {
   if (!valid_tile(precision, F, G))
     fail(DIO_ERR_INVALID, "invalid frames per pass / frames per lane");
   const size_t esz = dtype_bytes(dtype);
@@ -1489,14 +1496,14 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
       double* I = intensity + g0 * t.nbins;
       double* S = var ? sigma + g0 * t.nbins : nullptr;
       if (f64) {
-        if (var) launch_final<false, true, double>(t, pr, F, BS, ng, RF, I, S, s);
-        else launch_final<false, false, double>(t, pr, F, BS, ng, RF, I, S, s);
+        if (var) launch_final<false, true, double>(t, pr, F, BS, ng, RF, raw, I, S, s);
+        else launch_final<false, false, double>(t, pr, F, BS, ng, RF, raw, I, S, s);
       } else if (plain) {
-        if (var) launch_final<true, true, float>(t, pr, F, BS, ng, RF, I, S, s);
-        else launch_final<true, false, float>(t, pr, F, BS, ng, RF, I, S, s);
+        if (var) launch_final<true, true, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
+        else launch_final<true, false, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
       } else {
-        if (var) launch_final<false, true, float>(t, pr, F, BS, ng, RF, I, S, s);
-        else launch_final<false, false, float>(t, pr, F, BS, ng, RF, I, S, s);
+        if (var) launch_final<false, true, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
+        else launch_final<false, false, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
       }
     }
     DIO_HIP(hipGetLastError());

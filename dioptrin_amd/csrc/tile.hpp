@@ -101,7 +101,7 @@ bool double_buffered(const TiledTable& t, uint32_t frames_per_pass, int dtype, i
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t frames_per_pass,
                   uint32_t frames_per_lane, double* intensity, double* sigma, hipStream_t s,
-                  LaunchEvents* launches = nullptr);
+                  LaunchEvents* launches = nullptr, bool raw = false);
 
 // Only the tile kernel for one pass of nf <= frames_per_pass frames (timing).
 void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,

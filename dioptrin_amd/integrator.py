@@ -220,6 +220,29 @@ class GpuPlan:
         N.check(N.lib().dio_plan_normalization(self._h, out.ctypes.data))
         return out
 
+    def counts(self):
+        """Per bin: the sum of the execution table's weights (count)."""
+        out = np.empty(self.nbins)
+        N.check(N.lib().dio_plan_counts(self._h, out.ctypes.data))
+        return out
+
+    def integrate_raw(self, stack, errors=False) -> dict:
+        """Sums before division for multi-detector merging (integrate1d_raw /
+        IntermediateResult1d, lut/integrate.rs:372-500): per frame and bin
+        sum_signal = sum w*x and, with errors, sum_variance = sum w^2*max(x, 0);
+        per bin sum_norm (the denominators) and count."""
+        stack = self._stack(stack)
+        n = stack.shape[0]
+        flags = N.RAW_SUMS | (N.WITH_ERRORS if errors else 0)
+        sig = np.empty((n, self.nbins))
+        var = np.empty((n, self.nbins)) if errors else None
+        if n:
+            N.check(N.lib().dio_integrate(self._h, stack.ctypes.data, N.DTYPES[_NP_DTYPES[
+                stack.dtype]], n, flags, sig.ctypes.data,
+                None if var is None else var.ctypes.data, None))
+        return {"bins": self.radial, "sum_signal": sig, "sum_norm": self.denominators(),
+                "sum_variance": var, "count": self.counts()}
+
     # -- integration
     def _stack(self, images) -> np.ndarray:
         if isinstance(images, np.ndarray) and images.ndim == 3:

@@ -44,8 +44,12 @@ enum { DIO_U16 = 0, DIO_U32 = 1, DIO_I32 = 2, DIO_F32T = 3, DIO_F64T = 4 };
  * a DIO_ORDER_DETECTOR plan. Exported tables stay in detector order. */
 enum { DIO_ORDER_DETECTOR = 0, DIO_ORDER_FLIP_VERTICAL = 1 };
 /* Output selection flags; DIO_DEVICE_FRAMES (dio_server_submit only): the
- * frames pointer is device memory of the server's context. */
-enum { DIO_WITH_ERRORS = 1, DIO_WITH_AZIMUTHAL_STD = 2, DIO_DEVICE_FRAMES = 4 };
+ * frames pointer is device memory of the server's context. DIO_RAW_SUMS:
+ * the sums before division (IntermediateResult1d/2d, lut/integrate.rs:372-500):
+ * intensity receives sum_signal = sum w*x per bin and sigma (with
+ * DIO_WITH_ERRORS) sum_variance = sum w^2*max(x, 0) (Poisson variance);
+ * sum_norm is dio_plan_denominators and count dio_plan_counts. */
+enum { DIO_WITH_ERRORS = 1, DIO_WITH_AZIMUTHAL_STD = 2, DIO_DEVICE_FRAMES = 4, DIO_RAW_SUMS = 8 };
 
 typedef struct dio_ctx dio_ctx;
 typedef struct dio_plan dio_plan;
@@ -136,6 +140,10 @@ int dio_plan_export_csr(dio_plan *plan, int masked, uint32_t *row_ptr, uint32_t 
 int dio_plan_denominators(dio_plan *plan, double *out);
 /* Per-pixel normalization (polarization * solid angle, corrections.rs:57-93). */
 int dio_plan_normalization(dio_plan *plan, double *out);
+/* Per bin: the sum of the execution table's weights in row order (entries
+ * for unweighted tables), the count of IntermediateResult1d/2d
+ * (lut/integrate.rs:52-110 with COUNT). nbins doubles. */
+int dio_plan_counts(dio_plan *plan, double *out);
 
 /* Cuda1dPlan::integrate_frames / integrate_stack and Cuda2dPlan
  * equivalents (cuda.rs:480-548, 770-863): host frames in, host results out.

@@ -267,6 +267,36 @@ def test_flip_vertical_2d():
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
 
 
+def test_raw_sums_match_oracle(pilatus):
+    """Sums before division (integrate1d_raw, lut/integrate.rs:372-500):
+    sum_signal and the Poisson sum_variance within the gather tolerance of the
+    FP64 oracle's sums, sum_norm and count bit-exact (sequential row sums over
+    the masked table), and the batched kernel's raw sums equal the per-frame
+    path's bit for bit."""
+    d = pilatus
+    plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
+                   polarization_factor=0.99, mask=d["mask"], precision="f32_wide")
+    npix = d["w"]["shape"][0] * d["w"]["shape"][1]
+    ones = np.ones(plan.nbins)
+    frames = np.random.default_rng(23).poisson(120.0, size=(19,) + d["w"]["shape"]).astype(
+        np.uint32)
+    plan.set_tile(16, 4)
+    raw = plan.integrate_raw(frames, errors=True)
+    # The normalization (cos, powers) is not bit-critical: the device's
+    # denominators agree with the oracle's to 1e-12.
+    assert np.allclose(raw["sum_norm"], d["den"], rtol=1e-12, atol=0)
+    assert np.array_equal(raw["count"], O.denominators(d["fixed"], np.ones(npix)))
+    for k in (0, 7, 18):
+        s_ref, sig_ref = O.gather(d["fixed"], ones, frames[k].astype(np.float64), True)
+        assert np.all(np.abs(raw["sum_signal"][k] - s_ref) <= 2e-6 * np.abs(s_ref))
+        v_ref = sig_ref ** 2
+        assert np.all(np.abs(raw["sum_variance"][k] - v_ref) <= 2e-6 * v_ref + 1e-300)
+    plan.set_tile(1, 1)
+    one = plan.integrate_raw(frames, errors=True)
+    assert np.array_equal(one["sum_signal"], raw["sum_signal"])
+    assert np.array_equal(one["sum_variance"], raw["sum_variance"])
+
+
 def test_pixel_dtypes_equal_float64(pilatus):
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
