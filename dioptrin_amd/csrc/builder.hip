@@ -1057,12 +1057,31 @@ __global__ void k_normalization(Geo g, const double* tth, const double* chi, int
 }
 
 // prepared.rs:229-262, row<FAST = false>: n = n + w * norm[j], row order.
+// The entries are read 16 at a time (indices and weights, then their
+// normalization values) so that a thread waits for memory once per 16
+// additions; the additions stay sequential in row order.
 __global__ void k_denominators(const uint32_t* row_ptr, const uint32_t* col, const double* w64,
                                int weighted, const double* norm, uint32_t nbins, double* den) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nbins) return;
+  constexpr uint32_t B = 16;
   double n = 0.0;
-  for (uint32_t k = row_ptr[r]; k < row_ptr[r + 1]; k++) {
+  uint32_t k = row_ptr[r];
+  const uint32_t e = row_ptr[r + 1];
+  for (; k + B <= e; k += B) {
+    uint32_t c[B];
+    double w[B], v[B];
+#pragma unroll
+    for (uint32_t i = 0; i < B; i++) {
+      c[i] = col[k + i];
+      w[i] = weighted ? w64[k + i] : 1.0;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < B; i++) v[i] = norm[c[i]];
+#pragma unroll
+    for (uint32_t i = 0; i < B; i++) n = weighted ? n + w[i] * v[i] : n + v[i];
+  }
+  for (; k < e; k++) {
     if (weighted) n = n + w64[k] * norm[col[k]];
     else n = n + norm[col[k]];
   }

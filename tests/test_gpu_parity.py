@@ -285,6 +285,8 @@ def test_raw_sums_match_oracle(pilatus):
     # The normalization (cos, powers) is not bit-critical: the device's
     # denominators agree with the oracle's to 1e-12.
     assert np.allclose(raw["sum_norm"], d["den"], rtol=1e-12, atol=0)
+    # With the device's own normalization the sequential row sums are exact.
+    assert np.array_equal(raw["sum_norm"], O.denominators(d["fixed"], plan.normalization()))
     assert np.array_equal(raw["count"], O.denominators(d["fixed"], np.ones(npix)))
     for k in (0, 7, 18):
         s_ref, sig_ref = O.gather(d["fixed"], ones, frames[k].astype(np.float64), True)
