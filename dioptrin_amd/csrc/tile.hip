@@ -978,9 +978,13 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
       const LT* base = reinterpret_cast<const LT*>(frames_buf(i)) + q * G * kLdsStride;
       unsigned char* tb = table_buf(iA);
       const bool first_in_lds = iA.first || mA.w - mA.z <= C;
+      // The first slots hold the tile's longest segments, which set the
+      // iteration's length: their wavefronts issue first while they compute.
+      if (threadIdx.x < kDbThreads / 2) __builtin_amdgcn_s_setprio(2);
       tile_apply<F, G, MODE, VAR, C, 1, SLOTS, kDbThreads>(
           db_pass<F>(a, iA.pass), mA, cA, slot, q, base, reinterpret_cast<u32x2*>(tb),
           tb + (size_t)C * 8, &pd, first_in_lds);
+      __builtin_amdgcn_s_setprio(0);
     }
     ph = iA.pass;
     __syncthreads();
