@@ -37,6 +37,25 @@ def algorithmic_bytes_per_frame(npix, nnz, bins, tile, px_bytes=4, outputs=1):
     return px_bytes * npix + (8 * nnz + 4 * (bins + 1) + 8 * bins) / tile + 8 * bins * outputs
 
 
+def host_frame_rates(plan, frames_dev, n, errors):
+    """frames/s of plan.integrate_stack on n frames held in host memory,
+    pageable and page-locked (best of two runs each)."""
+    from dioptrin_amd.device import PinnedArray
+    host = frames_dev[:n].numpy()
+    pin = PinnedArray(host.shape, host.dtype)
+    pin.array[...] = host
+    out = {}
+    for name, src in (("pageable", host), ("pinned", pin.array)):
+        best = 0.0
+        for _ in range(2):
+            t0 = time.perf_counter()
+            plan.integrate_stack(src, errors=errors)
+            best = max(best, n / (time.perf_counter() - t0))
+        out[name] = round(best, 1)
+    del pin
+    return out
+
+
 def cpu_baseline(frames_dev, poni, w, seconds=12.0):
     """Restated reference CPU batch1d (CSC, 4-frame tiles, all host cores)."""
     import oracle as O
@@ -265,6 +284,11 @@ def main():
     plan.set_tile(tile, lanes)
     log(f"[bench] per-frame: {per_frame_fps:.0f} frames/s")
 
+    # Frames handed over in host memory (dio_integrate): the PCIe-inclusive
+    # rate of the batched path, from pageable and from page-locked buffers.
+    host_fps = host_frame_rates(plan, frames, min(nfr, 64), args.errors)
+    log(f"[bench] host frames: {host_fps}")
+
     fl = max(f for _, f in launches)  # frames of a full launch = frames per table read
     full = [ms for ms, f in launches if f == fl]
     seg_ms = sum(full) / len(full)
@@ -320,6 +344,7 @@ def main():
         "hbm_gbs_algorithmic": round(bytes_frame * fps / 1e9, 1),
         "hbm_pct_peak": round(100 * bytes_frame * fps / 1e9 / HBM_PEAK_GBS, 1),
         "per_frame_fps": round(per_frame_fps, 1),
+        "host_frames_fps_pcie": host_fps,
         "batched_vs_per_frame": round(fps / per_frame_fps, 2),
         "lut_build_ms_gpu": round(info["build_ms"], 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -27,7 +27,8 @@ EXPORTS = [
     "dio_plan_counts",
     "dio_integrate", "dio_integrate_device", "dio_plan_autotune", "dio_plan_set_tile",
     "dio_graph_create", "dio_graph_launch", "dio_graph_destroy", "dio_plan_time_gather",
-    "dio_plan_time_passes", "dio_device_alloc", "dio_device_free", "dio_memcpy", "dio_synchronize",
+    "dio_plan_time_passes", "dio_device_alloc", "dio_device_free", "dio_memcpy",
+    "dio_host_alloc", "dio_host_free", "dio_synchronize",
     "dio_synthetic_poisson_u32", "dio_server_create", "dio_server_submit", "dio_server_poll",
     "dio_server_stats_get", "dio_server_destroy",
 ]
@@ -134,6 +135,8 @@ def lib() -> C.CDLL:
                                            C.POINTER(u32),
                                            u32, C.POINTER(u32)]),
             "dio_device_alloc": (i32, [vp, u64, C.POINTER(vp)]),
+            "dio_host_alloc": (i32, [vp, u64, C.POINTER(vp)]),
+            "dio_host_free": (i32, [vp, vp]),
             "dio_device_free": (i32, [vp, vp]),
             "dio_memcpy": (i32, [vp, vp, vp, u64, i32]),
             "dio_synchronize": (i32, [vp]),

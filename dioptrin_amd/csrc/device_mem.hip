@@ -117,6 +117,23 @@ int dio_device_free(dio_ctx* ctx, void* d_ptr) {
   });
 }
 
+int dio_host_alloc(dio_ctx* ctx, uint64_t bytes, void** h_ptr) {
+  return guarded_local([&] {
+    if (!ctx || !h_ptr) dio::fail(DIO_ERR_INVALID, "null argument");
+    DIO_HIP(hipSetDevice(dio_ctx_device(ctx)));
+    *h_ptr = nullptr;
+    if (bytes) DIO_HIP(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault));
+  });
+}
+
+int dio_host_free(dio_ctx* ctx, void* h_ptr) {
+  return guarded_local([&] {
+    if (!ctx) dio::fail(DIO_ERR_INVALID, "null context");
+    DIO_HIP(hipSetDevice(dio_ctx_device(ctx)));
+    if (h_ptr) DIO_HIP(hipHostFree(h_ptr));
+  });
+}
+
 int dio_memcpy(dio_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind) {
   return guarded_local([&] {
     if (!ctx || (bytes && (!dst || !src))) dio::fail(DIO_ERR_INVALID, "null argument");

@@ -71,6 +71,31 @@ class DeviceArray:
             self.ptr = 0
 
 
+class PinnedArray:
+    """A numpy array in page-locked host memory (the pinned staging of the
+    reference's GPU path, cuda/device.rs:1063-1079): dio_integrate copies
+    such frames to the device by DMA. `.array` is the numpy view; it is
+    valid while this object lives."""
+
+    def __init__(self, shape, dtype, device: int = 0):
+        from .integrator import _Context
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self._ctx = _Context.get(device)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = C.c_void_p()
+        N.check(N.lib().dio_host_alloc(self._ctx, self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+        buf = (C.c_char * self.nbytes).from_address(self.ptr) if self.nbytes else bytearray()
+        self.array = np.frombuffer(buf, dtype=self.dtype).reshape(self.shape)
+
+    def __del__(self):
+        if getattr(self, "ptr", 0):
+            self.array = None
+            N.lib().dio_host_free(self._ctx, self.ptr)
+            self.ptr = 0
+
+
 def synchronize(device: int = 0):
     from .integrator import _Context
     N.check(N.lib().dio_synchronize(_Context.get(device)))

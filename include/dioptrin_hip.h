@@ -204,6 +204,11 @@ int dio_device_alloc(dio_ctx *ctx, uint64_t bytes, void **d_ptr);
 int dio_device_free(dio_ctx *ctx, void *d_ptr);
 /* kind: 0 host to device, 1 device to host, 2 device to device. */
 int dio_memcpy(dio_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind);
+/* Page-locked host memory (the pinned staging of the reference's GPU path,
+ * src/integration/cuda/device.rs:1063-1079): frames placed here reach the
+ * device by DMA in dio_integrate, without a pageable staging copy. */
+int dio_host_alloc(dio_ctx *ctx, uint64_t bytes, void **h_ptr);
+int dio_host_free(dio_ctx *ctx, void *h_ptr);
 int dio_synchronize(dio_ctx *ctx);
 
 /* Seeded synthetic detector frames for benchmarks and tests (no reference

@@ -299,6 +299,23 @@ def test_raw_sums_match_oracle(pilatus):
     assert np.array_equal(one["sum_variance"], raw["sum_variance"])
 
 
+def test_pinned_host_frames_equal_pageable(pilatus):
+    """Frames in page-locked host memory (dio_host_alloc) integrate to the
+    same bits as the same frames in pageable memory."""
+    from dioptrin_amd.device import PinnedArray
+    d = pilatus
+    plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
+                   polarization_factor=0.99, mask=d["mask"], precision="f32_wide")
+    frames = np.random.default_rng(24).poisson(80.0, size=(20,) + d["w"]["shape"]).astype(
+        np.uint32)
+    pin = PinnedArray(frames.shape, frames.dtype)
+    pin.array[...] = frames
+    a = plan.integrate_stack(frames, errors=True)
+    b = plan.integrate_stack(pin.array, errors=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    del pin
+
+
 def test_pixel_dtypes_equal_float64(pilatus):
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
