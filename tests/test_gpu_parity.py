@@ -343,6 +343,25 @@ def test_azimuthal_std_matches_oracle(pilatus):
             assert rel.max() <= 1e-6, (name, rel.max())
 
 
+def test_azimuthal_std_batch_equals_single_frame_bitwise(pilatus):
+    """The spread kernel reads the table once for 4 frames; each frame's
+    results are the bits it gets alone, for batches that fill 4-frame groups
+    and ones that do not (7 = 4 + 3), including frames with NaN and
+    infinite pixels."""
+    d = pilatus
+    plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
+                   polarization_factor=0.99, mask=d["mask"], precision="f32_wide")
+    frames = np.random.default_rng(25).poisson(90.0, size=(7,) + d["w"]["shape"]).astype(
+        np.float32)
+    frames[2, :40, :] = np.nan
+    frames[5, 100:110, 200:300] = np.inf
+    I, S, D = plan.integrate_stack(frames, errors=True, azimuthal_std=True)
+    for k in range(7):
+        i, s, dd = plan.integrate_stack(frames[k:k + 1], errors=True, azimuthal_std=True)
+        assert np.array_equal(I[k], i[0]) and np.array_equal(S[k], s[0])
+        assert np.array_equal(D[k], dd[0], equal_nan=True)
+
+
 def test_edge_cases_masked_inf_empty():
     poni = dict(pixel1=1e-4, pixel2=1e-4, distance=0.05, poni1=4.5e-4, poni2=5.5e-4,
                 wavelength=1e-10)
