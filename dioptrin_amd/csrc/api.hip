@@ -146,6 +146,8 @@ dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& des
     p->st.row_ptr = p->b.full.row_ptr.p;
     p->st.col = p->b.full.col.p;
     p->st.w32 = p->b.full.w32.p;
+    p->st.w64 = p->b.full.w64.p;
+    p->st.weighted = p->b.weighted ? 1 : 0;
     p->st.mask = p->b.mask.p;
     p->st.norm = p->b.norm.p;
     p->st.nbins = p->b.full.nbins;
@@ -376,6 +378,69 @@ int dio_integrate_device(dio_plan* p, const void* d_frames, int dtype, uint64_t 
     DIO_HIP(hipSetDevice(p->device));
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
     run_device(p, p->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std, s);
+  });
+}
+
+static void check_masked(dio_plan* p, uint32_t flags) {
+  require((flags & ~(uint32_t)DIO_WITH_ERRORS) == 0, DIO_ERR_UNSUPPORTED,
+          "per-frame masks take DIO_WITH_ERRORS only");
+  require(p->desc.ndim == 1, DIO_ERR_UNSUPPORTED, "per-frame masks are defined for 1D plans");
+}
+
+int dio_integrate_masked(dio_plan* p, const void* frames, int dtype, uint64_t n_frames,
+                         const uint8_t* masks, uint32_t flags, double* intensity, double* sigma) {
+  return guarded([&] {
+    require(p != nullptr, DIO_ERR_INVALID, "null plan");
+    check_masked(p, flags);
+    if (n_frames == 0) return;
+    require(frames && masks && intensity, DIO_ERR_INVALID, "null frames, masks or output");
+    const bool var = flags & DIO_WITH_ERRORS;
+    require(!var || sigma, DIO_ERR_INVALID, "sigma output missing");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    const uint64_t npix = p->gt.npix, fbytes = npix * dtype_bytes(dtype);
+    const uint32_t nb = p->st.nbins;
+    // Chunks of at most 1 GiB of frames and masks (and at least one frame).
+    uint64_t chunk = std::max<uint64_t>(1, (1ull << 30) / (fbytes + npix));
+    chunk = std::min<uint64_t>(chunk, n_frames);
+    if (p->stage.n < chunk * fbytes) p->stage.alloc(chunk * fbytes);
+    if (p->mstage.n < chunk * npix) p->mstage.alloc(chunk * npix);
+    ensure_out(p->out_i, chunk * nb);
+    ensure_out(p->out_s, chunk * nb);
+    const char* src = static_cast<const char*>(frames);
+    for (uint64_t f0 = 0; f0 < n_frames; f0 += chunk) {
+      const uint64_t nf = std::min<uint64_t>(chunk, n_frames - f0);
+      DIO_HIP(hipMemcpyAsync(p->stage.p, src + f0 * fbytes, nf * fbytes, hipMemcpyHostToDevice,
+                             p->stream));
+      DIO_HIP(hipMemcpyAsync(p->mstage.p, masks + f0 * npix, nf * npix, hipMemcpyHostToDevice,
+                             p->stream));
+      masked_frames(p->st, p->stage.p, dtype, p->mstage.p, nf, var, p->out_i.p, p->out_s.p,
+                    p->stream);
+      DIO_HIP(hipMemcpyAsync(intensity + f0 * nb, p->out_i.p, nf * nb * 8, hipMemcpyDeviceToHost,
+                             p->stream));
+      if (var)
+        DIO_HIP(hipMemcpyAsync(sigma + f0 * nb, p->out_s.p, nf * nb * 8, hipMemcpyDeviceToHost,
+                               p->stream));
+      DIO_HIP(hipStreamSynchronize(p->stream));
+    }
+  });
+}
+
+int dio_integrate_masked_device(dio_plan* p, const void* d_frames, int dtype, uint64_t n_frames,
+                                const uint8_t* d_masks, uint32_t flags, double* d_intensity,
+                                double* d_sigma, void* stream) {
+  return guarded([&] {
+    require(p != nullptr, DIO_ERR_INVALID, "null plan");
+    check_masked(p, flags);
+    if (n_frames == 0) return;
+    require(d_frames && d_masks && d_intensity, DIO_ERR_INVALID, "null frames, masks or output");
+    const bool var = flags & DIO_WITH_ERRORS;
+    require(!var || d_sigma, DIO_ERR_INVALID, "sigma output missing");
+    dtype_bytes(dtype);
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    masked_frames(p->st, d_frames, dtype, d_masks, n_frames, var, d_intensity, d_sigma, s);
   });
 }
 

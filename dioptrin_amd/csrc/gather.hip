@@ -146,6 +146,60 @@ __global__ __launch_bounds__(kBlock) void k_std(const uint32_t* __restrict__ row
   }
 }
 
+// prepared.rs:1011-1121 (row, strict order per lane) with a per-frame mask:
+// one wavefront per row and NF frames; entry k's index, weight and
+// normalization are read once for the NF frames. FP64 sums of FP64 weights;
+// lane partial sums combined by the fixed tree of wave_sum.
+template <class P, int NF>
+__global__ __launch_bounds__(kBlock) void k_masked(const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ col,
+                                                   const double* __restrict__ w64, int weighted,
+                                                   const double* __restrict__ norm,
+                                                   const P* __restrict__ frames,
+                                                   const uint8_t* __restrict__ masks,
+                                                   uint64_t npix, uint64_t n_frames,
+                                                   uint32_t nbins, uint32_t dim1, uint32_t dim2,
+                                                   int flip, int errors, double* I, double* S) {
+  uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  uint32_t r = blockIdx.x * kWavesPerBlock + wave;
+  if (r >= nbins) return;
+  const uint64_t f0 = (uint64_t)blockIdx.y * NF;
+  const int nf = (int)min<uint64_t>(NF, n_frames - f0);
+  const P* fr = frames + f0 * npix;
+  const uint8_t* mk = masks + f0 * npix;
+  const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+  double sx[NF], sn[NF], sv[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) sx[f] = sn[f] = sv[f] = 0.0;
+  for (uint32_t k = k0 + lane; k < k1; k += kWave) {
+    const uint32_t j = col[k];
+    const double w = weighted ? w64[k] : 1.0;
+    const double n = norm[j];
+    // Frames and masks hold the rows in input order (detector order flipped
+    // for bottom-up input).
+    const uint64_t jf = flip ? (uint64_t)(dim1 - 1 - j / dim2) * dim2 + j % dim2 : j;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      if (f >= nf || mk[(uint64_t)f * npix + jf]) continue;
+      const double x = (double)fr[(uint64_t)f * npix + jf];
+      sx[f] = sx[f] + w * x;
+      sn[f] = sn[f] + w * n;
+      if (errors) sv[f] = sv[f] + w * w * fmax(x, 0.0);
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    sx[f] = wave_sum(sx[f]);
+    sn[f] = wave_sum(sn[f]);
+    if (errors) sv[f] = wave_sum(sv[f]);
+    if (f >= nf || lane != 0) continue;
+    const uint64_t ob = (f0 + f) * nbins + r;
+    const double d = sn[f];
+    I[ob] = d == 0.0 ? 0.0 : sx[f] / d;
+    if (errors) S[ob] = d == 0.0 ? 0.0 : sqrt(fmax(sv[f] / (d * d), 0.0));
+  }
+}
+
 }  // namespace
 
 void azimuthal_std_frames(const StdTable& t, const void* frames, int dtype, uint64_t n_frames,
@@ -167,6 +221,30 @@ void azimuthal_std_frames(const StdTable& t, const void* frames, int dtype, uint
     case DIO_F32T: DIO_STD(float)
     case DIO_F64T: DIO_STD(double)
 #undef DIO_STD
+    default: fail(DIO_ERR_INVALID, "unknown frame dtype");
+  }
+  DIO_HIP(hipGetLastError());
+}
+
+void masked_frames(const StdTable& t, const void* frames, int dtype, const uint8_t* masks,
+                   uint64_t n_frames, bool errors, double* intensity, double* sigma,
+                   hipStream_t s) {
+  if (n_frames == 0) return;
+  constexpr int NF = 4;  // frames per wavefront
+  dim3 grid((t.nbins + kWavesPerBlock - 1) / kWavesPerBlock, (uint32_t)((n_frames + NF - 1) / NF));
+  switch (dtype) {
+#define DIO_MASKED(T)                                                                         \
+  k_masked<T, NF><<<grid, kBlock, 0, s>>>(t.row_ptr, t.col, t.w64, t.weighted, t.norm,       \
+                                          static_cast<const T*>(frames), masks, t.npix,        \
+                                          n_frames, t.nbins, t.dim1, t.dim2, t.flip ? 1 : 0,   \
+                                          errors, intensity, sigma);                           \
+  break;
+    case DIO_U16: DIO_MASKED(uint16_t)
+    case DIO_U32: DIO_MASKED(uint32_t)
+    case DIO_I32: DIO_MASKED(int32_t)
+    case DIO_F32T: DIO_MASKED(float)
+    case DIO_F64T: DIO_MASKED(double)
+#undef DIO_MASKED
     default: fail(DIO_ERR_INVALID, "unknown frame dtype");
   }
   DIO_HIP(hipGetLastError());

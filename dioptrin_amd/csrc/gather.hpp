@@ -19,6 +19,8 @@ struct StdTable {
   const uint32_t* row_ptr = nullptr;  // unmasked table
   const uint32_t* col = nullptr;
   const float* w32 = nullptr;
+  const double* w64 = nullptr;        // FP64 weights
+  int weighted = 1;                   // 0: unit weights (counting tables)
   const uint8_t* mask = nullptr;      // may be null
   const double* norm = nullptr;
   uint32_t nbins = 0;
@@ -32,5 +34,15 @@ struct StdTable {
 void azimuthal_std_frames(const StdTable& t, const void* frames, int dtype, uint64_t n_frames,
                           bool errors, double* intensity, double* sigma, double* std_out,
                           hipStream_t s);
+
+// Dynamic masks (Dynamic1dPlan::integrate, integration/dynamic.rs:143-186;
+// prepared.rs:1011-1121 with a per-frame mask): n_frames device frames, each
+// with its own device mask (nonzero = masked), on the unmasked table; the
+// plan's own mask is not applied. Per bin: I = sum w*x / sum w*n and
+// sigma = sqrt(max(sum w^2*max(x, 0) / (sum w*n)^2, 0)) over unmasked
+// entries, 0 where the denominator is 0.
+void masked_frames(const StdTable& t, const void* frames, int dtype, const uint8_t* masks,
+                   uint64_t n_frames, bool errors, double* intensity, double* sigma,
+                   hipStream_t s);
 
 }  // namespace dio

@@ -29,6 +29,7 @@ struct dio_plan {
   double build_ms = 0;
   std::mutex mu;
   dio::DevBuf<uint8_t> stage;
+  dio::DevBuf<uint8_t> mstage;  // per-frame masks of dio_integrate_masked
   dio::DevBuf<double> out_i, out_s, out_d;
   ~dio_plan() {
     if (stream) (void)hipStreamDestroy(stream);

@@ -277,6 +277,36 @@ class GpuPlan:
                 None if D is None else D.ctypes.data))
         return I, S, D
 
+    def integrate_masked(self, stack, masks, errors=False):
+        """Per-frame (dynamic) masks, Dynamic1dPlan::integrate
+        (integration/dynamic.rs:143-186): frame k is integrated on the
+        unmasked table with masks[k] (nonzero = masked) instead of the plan's
+        mask. Returns intensity and sigma, (n, nbins) each (sigma None
+        without errors)."""
+        stack = self._stack(stack)
+        n = stack.shape[0]
+        masks = np.ascontiguousarray(masks, dtype=np.uint8)
+        if masks.shape != stack.shape:
+            raise ValueError(f"masks of shape {masks.shape} for frames {stack.shape}")
+        I = np.empty((n, self.nbins))
+        S = np.empty((n, self.nbins)) if errors else None
+        N.check(N.lib().dio_integrate_masked(
+            self._h, stack.ctypes.data, N.DTYPES[_NP_DTYPES[stack.dtype]], n, masks.ctypes.data,
+            N.WITH_ERRORS if errors else 0, I.ctypes.data, None if S is None else S.ctypes.data))
+        return I, S
+
+    def integrate_masked_device(self, frames, masks, out_i, out_s=None, errors=False,
+                                stream=None):
+        """Device variant of integrate_masked: frames (n, dim1, dim2) and masks
+        (n, dim1, dim2) uint8 DeviceArrays, (n, nbins) float64 outputs;
+        enqueued on `stream` (None = the plan's stream)."""
+        if tuple(masks.shape) != tuple(frames.shape) or np.dtype(masks.dtype) != np.uint8:
+            raise ValueError("masks must be uint8 of the frames' shape")
+        N.check(N.lib().dio_integrate_masked_device(
+            self._h, frames.ptr, N.DTYPES[_NP_DTYPES[frames.dtype]], frames.shape[0], masks.ptr,
+            N.WITH_ERRORS if errors else 0, out_i.ptr, None if out_s is None else out_s.ptr,
+            stream))
+
     def _result(self, i, s, d):
         if self.ndim == 2:
             shape = (self.n_azimuthal, self.n_radial)

@@ -160,6 +160,23 @@ int dio_integrate_device(dio_plan *plan, const void *d_frames, int dtype, uint64
                          uint32_t flags, double *d_intensity, double *d_sigma, double *d_std,
                          void *stream);
 
+/* Dynamic (per-frame) masks: Dynamic1dPlan::integrate(data, Some(mask),
+ * errors, false) (src/integration/dynamic.rs:143-186; row sums
+ * src/lut/prepared.rs:1011-1121, normalize_bin 985-996). Each frame comes
+ * with its own mask (masks: n_frames * dim1 * dim2 bytes, nonzero = masked,
+ * in the plan's input order) and is integrated on the plan's unmasked
+ * table; the plan's own mask is not applied. Per bin: intensity =
+ * sum w*x / sum w*norm, sigma = sqrt(max(sum w^2*max(x,0) / (sum w*norm)^2,
+ * 0)), both 0 where the denominator is 0. flags: 0 or DIO_WITH_ERRORS;
+ * 1D plans only. The host variant is synchronous; the device variant is
+ * enqueued on `stream` (NULL = the plan's stream). */
+int dio_integrate_masked(dio_plan *plan, const void *frames, int dtype, uint64_t n_frames,
+                         const uint8_t *masks, uint32_t flags, double *intensity,
+                         double *sigma);
+int dio_integrate_masked_device(dio_plan *plan, const void *d_frames, int dtype,
+                                uint64_t n_frames, const uint8_t *d_masks, uint32_t flags,
+                                double *d_intensity, double *d_sigma, void *stream);
+
 /* Launch shape of the fused batch kernel (no reference counterpart; the
  * reference launches one frame per kernel, src/integration/cuda/device.rs:
  * 1460-1584): frames per pass and frames per lane, one of the pairs (1, 1),

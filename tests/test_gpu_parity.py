@@ -621,3 +621,122 @@ def test_pyfai_table_bit_exact(g):
     else:
         lut, _ = O.build_lut2d_pyfai(P, *shape, nr, na, unit, rr)
     assert_same_table(plan, lut)
+
+
+# ----------------------------------------------------------- dynamic masks
+
+def dynamic_masks(d, n, seed):
+    """Per-frame masks: the plan's mask, random 3% pixels, a masked radial
+    band (bins whose denominator is 0), nothing masked, and random mixes."""
+    shape = d["w"]["shape"]
+    rng = np.random.default_rng(seed)
+    masks = np.zeros((n,) + shape, np.uint8)
+    for k in range(n):
+        kind = k % 4
+        if kind == 0:
+            masks[k] = d["mask"]
+        elif kind == 1:
+            masks[k] = rng.random(shape) < 0.03
+        elif kind == 2:
+            masks[k, :, 300:420] = 1   # whole columns: empties bins of the inner rings
+            masks[k, 400:700, :] = 1
+        else:
+            masks[k] = (rng.random(shape) < 0.5) if k >= 7 else 0
+    return masks
+
+
+def oracle_dynamic(d, frame, mask, errors=True):
+    fixed = O.fixed_table(d["lut"], mask)
+    den = O.denominators(fixed, d["norm"])
+    return O.gather(fixed, den, frame.astype(np.float64), errors)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.uint32, np.uint16])
+def test_dynamic_masks_match_oracle(pilatus, dtype):
+    """Dynamic1dPlan::integrate with a mask per frame (integration/dynamic.rs:
+    143-186): each frame's result is the oracle's strict gather on the table
+    compacted by that frame's mask with that mask's denominators; the plan's
+    own mask plays no part. FP64 sums of FP64 weights: rel 1e-10 (sum order
+    differs from the oracle's sequential one). Bins emptied by a mask give 0."""
+    d = pilatus
+    print(f"case dynamic masks {np.dtype(dtype).name}", flush=True)
+    n = 9
+    frames = np.stack([synth.frame_pilatus(s) for s in range(n)]).astype(dtype)
+    masks = dynamic_masks(d, n, 31)
+    plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
+                   polarization_factor=0.99, mask=d["mask"], precision="f32_wide")
+    I, S = plan.integrate_masked(frames, masks, errors=True)
+    I0, S0 = plan.integrate_masked(frames, masks, errors=False)
+    assert S0 is None and np.array_equal(I0, I)
+    empty = 0
+    for k in range(n):
+        i_ref, s_ref = oracle_dynamic(d, frames[k], masks[k])
+        scale = np.abs(i_ref).max()
+        assert np.allclose(I[k], i_ref, rtol=1e-10, atol=1e-12 * scale), k
+        assert np.allclose(S[k], s_ref, rtol=1e-10, atol=1e-12 * np.abs(s_ref).max()), k
+        z = O.denominators(O.fixed_table(d["lut"], masks[k]), d["norm"]) == 0
+        assert np.all(I[k][z] == 0) and np.all(S[k][z] == 0)
+        empty += int(z.sum())
+    assert empty > 0   # the band masks empty some bins
+
+
+def test_dynamic_mask_equal_to_fixed_mask(pilatus):
+    """A frame masked dynamically by the plan's own fixed mask gives the fixed
+    path's results within the gather tolerance (fixed_table(mask) is the same
+    table; the fixed path sums float32 weights)."""
+    d = pilatus
+    plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
+                   polarization_factor=0.99, mask=d["mask"], precision="f64")
+    frames = d["frames"]
+    I, S = plan.integrate_masked(frames, np.broadcast_to(d["mask"], frames.shape), errors=True)
+    If, Sf, _ = plan.integrate_stack(frames, errors=True)
+    assert np.allclose(I, If, rtol=2e-6, atol=0) and np.allclose(S, Sf, rtol=2e-6, atol=0)
+
+
+def test_dynamic_masks_batch_single_device_and_flip_bitwise(pilatus):
+    """Frame k's bits do not depend on the batch it comes in (4 frames per
+    wavefront, 11 frames = 2 full groups and a ragged one), the device entry
+    point gives the host entry point's bits, and flipped-row frames and masks
+    on a flip-order plan give the detector-order bits."""
+    from dioptrin_amd.device import DeviceArray, synchronize
+    d = pilatus
+    n = 11
+    frames = np.random.default_rng(5).poisson(90.0, size=(n,) + d["w"]["shape"]).astype(
+        np.uint32)
+    masks = dynamic_masks(d, n, 7)
+    kw = dict(n_radial=d["w"]["n_radial"], polarization_factor=0.99)
+    plan = GpuPlan(d["poni"], d["w"]["shape"], **kw)
+    I, S = plan.integrate_masked(frames, masks, errors=True)
+    for k in (0, 3, 4, 10):
+        i1, s1 = plan.integrate_masked(frames[k:k + 1], masks[k:k + 1], errors=True)
+        assert np.array_equal(i1[0], I[k]) and np.array_equal(s1[0], S[k]), k
+    fr, mk = DeviceArray.from_numpy(frames), DeviceArray.from_numpy(masks)
+    oi, os_ = DeviceArray((n, plan.nbins), np.float64), DeviceArray((n, plan.nbins), np.float64)
+    plan.integrate_masked_device(fr, mk, oi, os_, errors=True)
+    synchronize()
+    assert np.array_equal(oi.numpy(), I) and np.array_equal(os_.numpy(), S)
+    flip = GpuPlan(d["poni"], d["w"]["shape"], input_order="flip_vertical", **kw)
+    If, Sf = flip.integrate_masked(np.ascontiguousarray(frames[:, ::-1]),
+                                   np.ascontiguousarray(masks[:, ::-1]), errors=True)
+    assert np.array_equal(If, I) and np.array_equal(Sf, S)
+
+
+def test_dynamic_masks_reject_unsupported(pilatus):
+    """Per-frame masks take DIO_WITH_ERRORS only, on 1D plans, with masks of
+    the frames' shape."""
+    from dioptrin_amd import _native as N
+    d = pilatus
+    plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"])
+    frames = d["frames"][:1]
+    with pytest.raises(ValueError):
+        plan.integrate_masked(frames, np.zeros((2,) + d["w"]["shape"], np.uint8))
+    I = np.empty((1, plan.nbins))
+    m = np.zeros_like(frames, dtype=np.uint8)
+    rc = N.lib().dio_integrate_masked(plan._h, frames.ctypes.data, N.DTYPES["float32"], 1,
+                                      m.ctypes.data, N.WITH_AZIMUTHAL_STD, I.ctypes.data, None)
+    assert rc == N.DIO_ERR_UNSUPPORTED
+    g = GEOMETRIES_2D[0]
+    plan2 = GpuPlan(g[1], g[2], n_radial=g[3], n_azimuthal=g[4])
+    with pytest.raises(Exception):
+        plan2.integrate_masked(np.zeros((1,) + g[2], np.float32),
+                               np.zeros((1,) + g[2], np.uint8))
