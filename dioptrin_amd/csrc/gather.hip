@@ -148,7 +148,8 @@ __global__ __launch_bounds__(kBlock) void k_std(const uint32_t* __restrict__ row
 
 // prepared.rs:1011-1121 (row, strict order per lane) with a per-frame mask:
 // one wavefront per row and NF frames; entry k's index, weight and
-// normalization are read once for the NF frames. FP64 sums of FP64 weights;
+// normalization are read once for the NF frames, and its NF mask bytes and
+// signals are loaded together before the sums. FP64 sums of FP64 weights;
 // lane partial sums combined by the fixed tree of wave_sum.
 template <class P, int NF>
 __global__ __launch_bounds__(kBlock) void k_masked(const uint32_t* __restrict__ row_ptr,
@@ -178,13 +179,21 @@ __global__ __launch_bounds__(kBlock) void k_masked(const uint32_t* __restrict__ 
     // Frames and masks hold the rows in input order (detector order flipped
     // for bottom-up input).
     const uint64_t jf = flip ? (uint64_t)(dim1 - 1 - j / dim2) * dim2 + j % dim2 : j;
+    // All loads of the entry first, then the sums: a masked entry is
+    // skipped (its signal is never read into a sum, NaN or not).
+    bool on[NF];
+    double xs[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-      if (f >= nf || mk[(uint64_t)f * npix + jf]) continue;
-      const double x = (double)fr[(uint64_t)f * npix + jf];
-      sx[f] = sx[f] + w * x;
+      on[f] = f < nf && mk[(uint64_t)f * npix + jf] == 0;
+      xs[f] = f < nf ? (double)fr[(uint64_t)f * npix + jf] : 0.0;
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      if (!on[f]) continue;
+      sx[f] = sx[f] + w * xs[f];
       sn[f] = sn[f] + w * n;
-      if (errors) sv[f] = sv[f] + w * w * fmax(x, 0.0);
+      if (errors) sv[f] = sv[f] + w * w * fmax(xs[f], 0.0);
     }
   }
 #pragma unroll
@@ -230,7 +239,7 @@ void masked_frames(const StdTable& t, const void* frames, int dtype, const uint8
                    uint64_t n_frames, bool errors, double* intensity, double* sigma,
                    hipStream_t s) {
   if (n_frames == 0) return;
-  constexpr int NF = 4;  // frames per wavefront
+  constexpr int NF = 8;  // frames per wavefront
   dim3 grid((t.nbins + kWavesPerBlock - 1) / kWavesPerBlock, (uint32_t)((n_frames + NF - 1) / NF));
   switch (dtype) {
 #define DIO_MASKED(T)                                                                         \

@@ -694,8 +694,8 @@ def test_dynamic_mask_equal_to_fixed_mask(pilatus):
 
 
 def test_dynamic_masks_batch_single_device_and_flip_bitwise(pilatus):
-    """Frame k's bits do not depend on the batch it comes in (4 frames per
-    wavefront, 11 frames = 2 full groups and a ragged one), the device entry
+    """Frame k's bits do not depend on the batch it comes in (8 frames per
+    wavefront, 11 frames = a full group and a ragged one), the device entry
     point gives the host entry point's bits, and flipped-row frames and masks
     on a flip-order plan give the detector-order bits."""
     from dioptrin_amd.device import DeviceArray, synchronize
