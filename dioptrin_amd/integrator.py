@@ -21,6 +21,7 @@ from typing import Optional, Sequence, Union
 import numpy as np
 
 from . import _native as N
+from .poni import load_poni  # noqa: F401  (part of the package API)
 
 _NP_DTYPES = {np.dtype(np.uint16): "uint16", np.dtype(np.uint32): "uint32",
               np.dtype(np.int32): "int32", np.dtype(np.float32): "float32",
@@ -52,32 +53,6 @@ class BatchResult:
 
     def is_err(self) -> bool:
         return self.error is not None
-
-
-def load_poni(path: str) -> dict:
-    """Reads the PONI v1/v2 keys used by the geometry (poni.rs:79-135)."""
-    keys = {"pixelsize1": "pixel1", "pixelsize2": "pixel2", "distance": "distance",
-            "poni1": "poni1", "poni2": "poni2", "rot1": "rot1", "rot2": "rot2", "rot3": "rot3",
-            "wavelength": "wavelength"}
-    out = {k: 0.0 for k in keys.values()}
-    import json
-    with open(path) as f:
-        for line in f:
-            line = line.strip()
-            if not line or line.startswith("#") or ":" not in line:
-                continue
-            k, v = line.split(":", 1)
-            k, v = k.strip().lower(), v.strip()
-            if k == "detector_config":
-                cfg = json.loads(v)
-                for name in ("pixel1", "pixel2"):
-                    if name in cfg:
-                        out[name] = float(cfg[name])
-            elif k in keys:
-                out[keys[k]] = float(v)
-    if out["pixel1"] == 0.0 or out["pixel2"] == 0.0:
-        raise ValueError(f"{path}: missing pixel sizes (pixel1/pixel2) in PONI file")
-    return out
 
 
 class _Context:
