@@ -381,6 +381,37 @@ int dio_integrate_device(dio_plan* p, const void* d_frames, int dtype, uint64_t 
   });
 }
 
+// Dynamic masks. With a plan mask the execution table is compacted, so the
+// CSR kernel runs on the unmasked table; otherwise the tile kernel's raw sums
+// of the zeroed signal and of the zeroed normalization give numerator,
+// variance and the per-frame denominator, in chunks of 64 frames.
+static void run_masked(dio_plan* p, const void* frames, int dtype, uint64_t n_frames,
+                       const uint8_t* masks, bool var, double* I, double* S, hipStream_t s) {
+  if (p->b.has_exec) {
+    masked_frames(p->st, frames, dtype, masks, n_frames, var, I, S, s);
+    return;
+  }
+  const uint64_t npix = p->gt.npix, esz = dtype_bytes(dtype);
+  const uint32_t nb = p->st.nbins;
+  const bool f64 = p->desc.precision == DIO_F64;
+  const int ndt = f64 ? DIO_F64T : DIO_F32T;
+  const uint64_t chunk = std::min<uint64_t>(n_frames, 64);
+  if (p->dyn_x.n < chunk * npix * esz) p->dyn_x.alloc(chunk * npix * esz);
+  if (p->dyn_n.n < chunk * npix * (f64 ? 8 : 4)) p->dyn_n.alloc(chunk * npix * (f64 ? 8 : 4));
+  if (p->dyn_d.n < chunk * nb) p->dyn_d.alloc(chunk * nb);
+  const char* src = static_cast<const char*>(frames);
+  for (uint64_t f0 = 0; f0 < n_frames; f0 += chunk) {
+    const uint64_t nf = std::min<uint64_t>(chunk, n_frames - f0);
+    masked_prepare(p->st, src + f0 * npix * esz, dtype, masks + f0 * npix, nf, f64, p->dyn_x.p,
+                   p->dyn_n.p, s);
+    tiled_gather(p->tt, p->sc, p->dyn_x.p, dtype, nf, var, p->desc.precision, p->tile, p->lanes,
+                 I + f0 * nb, var ? S + f0 * nb : nullptr, s, nullptr, true);
+    tiled_gather(p->tt, p->sc, p->dyn_n.p, ndt, nf, false, p->desc.precision, p->tile, p->lanes,
+                 p->dyn_d.p, nullptr, s, nullptr, true);
+    dio::masked_finish(I + f0 * nb, var ? S + f0 * nb : nullptr, p->dyn_d.p, nf * nb, var, s);
+  }
+}
+
 static void check_masked(dio_plan* p, uint32_t flags) {
   require((flags & ~(uint32_t)DIO_WITH_ERRORS) == 0, DIO_ERR_UNSUPPORTED,
           "per-frame masks take DIO_WITH_ERRORS only");
@@ -414,8 +445,7 @@ int dio_integrate_masked(dio_plan* p, const void* frames, int dtype, uint64_t n_
                              p->stream));
       DIO_HIP(hipMemcpyAsync(p->mstage.p, masks + f0 * npix, nf * npix, hipMemcpyHostToDevice,
                              p->stream));
-      masked_frames(p->st, p->stage.p, dtype, p->mstage.p, nf, var, p->out_i.p, p->out_s.p,
-                    p->stream);
+      run_masked(p, p->stage.p, dtype, nf, p->mstage.p, var, p->out_i.p, p->out_s.p, p->stream);
       DIO_HIP(hipMemcpyAsync(intensity + f0 * nb, p->out_i.p, nf * nb * 8, hipMemcpyDeviceToHost,
                              p->stream));
       if (var)
@@ -440,7 +470,7 @@ int dio_integrate_masked_device(dio_plan* p, const void* d_frames, int dtype, ui
     std::lock_guard<std::mutex> lk(p->mu);
     DIO_HIP(hipSetDevice(p->device));
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
-    masked_frames(p->st, d_frames, dtype, d_masks, n_frames, var, d_intensity, d_sigma, s);
+    run_masked(p, d_frames, dtype, n_frames, d_masks, var, d_intensity, d_sigma, s);
   });
 }
 

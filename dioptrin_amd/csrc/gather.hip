@@ -1,6 +1,7 @@
 // Azimuthal-spread kernel (gfx950): one wavefront per bin row of the
 // unmasked CSR table and 4 frames, FP64 arithmetic on float32 weights. The path is a
 // sparse, bandwidth-bound reduction, so it uses the vector ALUs, not MFMA.
+#include <algorithm>
 #include <cfloat>
 
 #include "gather.hpp"
@@ -209,6 +210,34 @@ __global__ __launch_bounds__(kBlock) void k_masked(const uint32_t* __restrict__ 
   }
 }
 
+// One frame per blockIdx.y: masked pixels' signal and normalization set to 0
+// (a masked pixel then adds exactly 0 to every sum of the tile kernel).
+template <class T, class N>
+__global__ __launch_bounds__(256) void k_masked_prepare(const T* __restrict__ frames,
+                                                        const uint8_t* __restrict__ masks,
+                                                        const double* __restrict__ norm,
+                                                        uint64_t npix, uint32_t dim1,
+                                                        uint32_t dim2, int flip, T* xo, N* no) {
+  const uint64_t base = (uint64_t)blockIdx.y * npix;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const bool m = masks[base + p] != 0;
+    const uint64_t pd = flip ? (uint64_t)(dim1 - 1 - p / dim2) * dim2 + p % dim2 : p;
+    xo[base + p] = m ? T(0) : frames[base + p];
+    no[base + p] = m ? N(0) : (N)norm[pd];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_masked_finish(double* I, double* S, const double* D,
+                                                       uint64_t n, int errors) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const double d = D[i];
+    I[i] = d == 0.0 ? 0.0 : I[i] / d;
+    if (errors) S[i] = d == 0.0 ? 0.0 : sqrt(fmax(S[i] / (d * d), 0.0));
+  }
+}
+
 }  // namespace
 
 void azimuthal_std_frames(const StdTable& t, const void* frames, int dtype, uint64_t n_frames,
@@ -256,6 +285,44 @@ void masked_frames(const StdTable& t, const void* frames, int dtype, const uint8
 #undef DIO_MASKED
     default: fail(DIO_ERR_INVALID, "unknown frame dtype");
   }
+  DIO_HIP(hipGetLastError());
+}
+
+void masked_prepare(const StdTable& t, const void* frames, int dtype, const uint8_t* masks,
+                    uint64_t n_frames, bool norm_f64, void* xo, void* no, hipStream_t s) {
+  if (n_frames == 0) return;
+  if (n_frames > 65535) fail(DIO_ERR_INVALID, "at most 65535 frames per prepare");
+  dim3 grid((uint32_t)std::min<uint64_t>((t.npix + 255) / 256, 1024), (uint32_t)n_frames);
+  const int flip = t.flip ? 1 : 0;
+  switch (dtype) {
+#define DIO_PREP(T)                                                                          \
+  if (norm_f64)                                                                              \
+    k_masked_prepare<T, double><<<grid, 256, 0, s>>>(static_cast<const T*>(frames), masks,   \
+                                                     t.norm, t.npix, t.dim1, t.dim2, flip,    \
+                                                     static_cast<T*>(xo),                     \
+                                                     static_cast<double*>(no));               \
+  else                                                                                       \
+    k_masked_prepare<T, float><<<grid, 256, 0, s>>>(static_cast<const T*>(frames), masks,    \
+                                                    t.norm, t.npix, t.dim1, t.dim2, flip,     \
+                                                    static_cast<T*>(xo),                      \
+                                                    static_cast<float*>(no));                 \
+  break;
+    case DIO_U16: DIO_PREP(uint16_t)
+    case DIO_U32: DIO_PREP(uint32_t)
+    case DIO_I32: DIO_PREP(int32_t)
+    case DIO_F32T: DIO_PREP(float)
+    case DIO_F64T: DIO_PREP(double)
+#undef DIO_PREP
+    default: fail(DIO_ERR_INVALID, "unknown frame dtype");
+  }
+  DIO_HIP(hipGetLastError());
+}
+
+void masked_finish(double* intensity, double* sigma, const double* den, uint64_t n, bool errors,
+                   hipStream_t s) {
+  if (n == 0) return;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+  k_masked_finish<<<blocks, 256, 0, s>>>(intensity, sigma, den, n, errors ? 1 : 0);
   DIO_HIP(hipGetLastError());
 }
 

@@ -45,4 +45,15 @@ void masked_frames(const StdTable& t, const void* frames, int dtype, const uint8
                    uint64_t n_frames, bool errors, double* intensity, double* sigma,
                    hipStream_t s);
 
+// Dynamic masks on a plan whose execution table is the unmasked table: the
+// masked frames are turned into two frames each for the tile kernel's raw
+// sums, the signal with masked pixels set to 0 (xo, the frames' dtype) and
+// the pixel normalization with masked pixels set to 0 (no, float64 when
+// norm_f64, else float32), both in input order; masked_finish then divides:
+// I = S_x / S_n, sigma = sqrt(max(S_v / S_n^2, 0)), 0 where S_n is 0.
+void masked_prepare(const StdTable& t, const void* frames, int dtype, const uint8_t* masks,
+                    uint64_t n_frames, bool norm_f64, void* xo, void* no, hipStream_t s);
+void masked_finish(double* intensity, double* sigma, const double* den, uint64_t n, bool errors,
+                   hipStream_t s);
+
 }  // namespace dio

@@ -30,6 +30,10 @@ struct dio_plan {
   std::mutex mu;
   dio::DevBuf<uint8_t> stage;
   dio::DevBuf<uint8_t> mstage;  // per-frame masks of dio_integrate_masked
+  // Dynamic masks on the tile kernel: zeroed signal and normalization frames,
+  // per-frame denominators of one chunk.
+  dio::DevBuf<uint8_t> dyn_x, dyn_n;
+  dio::DevBuf<double> dyn_d;
   dio::DevBuf<double> out_i, out_s, out_d;
   ~dio_plan() {
     if (stream) (void)hipStreamDestroy(stream);

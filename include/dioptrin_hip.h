@@ -168,7 +168,10 @@ int dio_integrate_device(dio_plan *plan, const void *d_frames, int dtype, uint64
  * table; the plan's own mask is not applied. Per bin: intensity =
  * sum w*x / sum w*norm, sigma = sqrt(max(sum w^2*max(x,0) / (sum w*norm)^2,
  * 0)), both 0 where the denominator is 0. flags: 0 or DIO_WITH_ERRORS;
- * 1D plans only. The host variant is synchronous; the device variant is
+ * 1D plans only. A plan without a mask computes at the plan's precision
+ * (tile kernel); a plan with a mask in FP64 (CSR kernel). The device
+ * variant uses plan-owned staging: calls on one plan that overlap on
+ * different streams race. The host variant is synchronous; the device variant is
  * enqueued on `stream` (NULL = the plan's stream). */
 int dio_integrate_masked(dio_plan *plan, const void *frames, int dtype, uint64_t n_frames,
                          const uint8_t *masks, uint32_t flags, double *intensity,
