@@ -56,6 +56,30 @@ def host_frame_rates(plan, frames_dev, n, errors):
     return out
 
 
+def dynamic_mask_rate(plan, frames_dev, n, errors):
+    """frames/s of plan.integrate_masked_device on n device frames, each with
+    its own mask (5 % of the pixels, a different set per frame); best of
+    three runs. The plan has no mask of its own, so this is the tile-kernel
+    route."""
+    from dioptrin_amd.device import DeviceArray, synchronize
+    shape = frames_dev.shape[1:]
+    base = (np.random.default_rng(3).random(shape[0] * shape[1]) < 0.05).astype(np.uint8)
+    masks = DeviceArray.from_numpy(np.stack([np.roll(base, 977 * k).reshape(shape)
+                                             for k in range(n)]))
+    oi = DeviceArray((n, plan.nbins), np.float64)
+    os_ = DeviceArray((n, plan.nbins), np.float64) if errors else None
+    fr = frames_dev[:n]
+    plan.integrate_masked_device(fr, masks, oi, os_, errors=errors)
+    synchronize()
+    best = 0.0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        plan.integrate_masked_device(fr, masks, oi, os_, errors=errors)
+        synchronize()
+        best = max(best, n / (time.perf_counter() - t0))
+    return round(best, 1)
+
+
 def cpu_baseline(frames_dev, poni, w, seconds=12.0):
     """Restated reference CPU batch1d (CSC, 4-frame tiles, all host cores)."""
     import oracle as O
@@ -288,6 +312,8 @@ def main():
     # rate of the batched path, from pageable and from page-locked buffers.
     host_fps = host_frame_rates(plan, frames, min(nfr, 64), args.errors)
     log(f"[bench] host frames: {host_fps}")
+    dyn_fps = dynamic_mask_rate(plan, frames, min(nfr, 64), args.errors)
+    log(f"[bench] per-frame masks: {dyn_fps} frames/s")
 
     fl = max(f for _, f in launches)  # frames of a full launch = frames per table read
     full = [ms for ms, f in launches if f == fl]
@@ -345,6 +371,7 @@ def main():
         "hbm_pct_peak": round(100 * bytes_frame * fps / 1e9 / HBM_PEAK_GBS, 1),
         "per_frame_fps": round(per_frame_fps, 1),
         "host_frames_fps_pcie": host_fps,
+        "dynamic_masks_fps": dyn_fps,
         "batched_vs_per_frame": round(fps / per_frame_fps, 2),
         "lut_build_ms_gpu": round(info["build_ms"], 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
