@@ -52,8 +52,10 @@ size_t dtype_bytes(int dtype) {
 // Configuration checks before any device work (cuda.rs:1001-1057).
 void check_desc(const dio_plan_desc& d) {
   require(d.dim1 > 0 && d.dim2 > 0, DIO_ERR_INVALID, "image shape has a zero side");
-  require((uint64_t)d.dim1 * d.dim2 <= 0xffffffffull, DIO_ERR_INVALID,
-          "image exceeds 2^32 pixels");
+  // The tiled gather addresses a frame with 32-bit byte offsets (up to 8
+  // bytes per pixel), so an image holds fewer than 2^29 pixels.
+  require((uint64_t)d.dim1 * d.dim2 < (1ull << 29), DIO_ERR_INVALID,
+          "image must hold fewer than 2^29 pixels");
   require(d.ndim == 1 || d.ndim == 2, DIO_ERR_INVALID, "ndim must be 1 or 2");
   require(d.n_radial > 0 && d.n_radial <= 0x7fffffffu, DIO_ERR_INVALID,
           "n_points must be between 1 and 2^31-1");

@@ -138,12 +138,6 @@ template <int R> struct SegRegs {
   uint32_t lo[R], hi[R];
 };
 
-// Table groups a lane takes per step of a segment (more groups hide more LDS
-// latency and hold more frame values in registers).
-template <int G, int MODE> struct SegGroups {
-  static constexpr int P = 1;
-};
-
 // One segment's partials of this lane's G frames (and their variance terms).
 template <int G, int MODE, bool VAR> struct SegVals {
   using PT = typename TAcc<MODE>::P;
@@ -196,60 +190,76 @@ __device__ __forceinline__ SegVals<G, MODE, VAR> seg_eval(const LT* base, const 
     acc[j].init();
     if constexpr (VAR) var[j].init();
   }
-  // P groups per iteration: their table words are read together, then all
-  // their frame values, so an iteration waits for two LDS round trips for P
-  // groups. Groups past the segment's end read the zero slot with weight 0:
-  // such a term adds exactly +0 to every accumulator, whose value never is -0
-  // (it starts at +0, and a sum is -0 only if both addends are), so the
-  // result is bit-identical to adding the segment's terms alone, in order.
-  constexpr int P = SegGroups<G, MODE>::P;
+  // Software pipeline over the segment's groups: the table words (pixel
+  // indices and weights) of group g + 1 are read while group g's frame values
+  // are in flight and its products accumulate, so a group waits for one LDS
+  // round trip (its frame values), not two. The loop is unrolled twice over
+  // two register sets (A, B), so the words move into place without copies.
+  // The read after the segment's last group repeats that group (in the
+  // chunk, unused). The products are added in the same order as a plain
+  // loop over the groups, so the bits do not depend on the pipeline.
   const f64x2* twd = reinterpret_cast<const f64x2*>(twb);
   const f32x4* twf = reinterpret_cast<const f32x4*>(twb);
-  for (uint32_t g = ga; g < gb; g += P) {
-    uint32_t l[P][4];
-    float wf[P][4];
-    double wd[P][4];
-#pragma unroll
-    for (int p = 0; p < P; p++) {
-      const bool in = g + p < gb;
-      const uint32_t gp = in ? g + p : g;
-      const u32x2 lc = tl[gp];
-      l[p][0] = in ? (lc.x & 0xffffu) : kPadSlot;
-      l[p][1] = in ? (lc.x >> 16) : kPadSlot;
-      l[p][2] = in ? (lc.y & 0xffffu) : kPadSlot;
-      l[p][3] = in ? (lc.y >> 16) : kPadSlot;
-      if constexpr (MODE == 2) {
-        const f64x2 x0 = twd[2 * gp], x1 = twd[2 * gp + 1];
-        wd[p][0] = in ? x0.x : 0.0;
-        wd[p][1] = in ? x0.y : 0.0;
-        wd[p][2] = in ? x1.x : 0.0;
-        wd[p][3] = in ? x1.y : 0.0;
-        wf[p][0] = wf[p][1] = wf[p][2] = wf[p][3] = 0.f;
-      } else {
-        const f32x4 w4 = twf[gp];
-        wf[p][0] = in ? w4.x : 0.f;
-        wf[p][1] = in ? w4.y : 0.f;
-        wf[p][2] = in ? w4.z : 0.f;
-        wf[p][3] = in ? w4.w : 0.f;
-        wd[p][0] = wd[p][1] = wd[p][2] = wd[p][3] = 0.0;
-      }
+  struct Words {
+    u32x2 l;
+    f32x4 wf;
+    f64x2 wd0, wd1;
+  };
+  auto read_words = [&](uint32_t g, Words& w) {
+    w.l = tl[g];
+    if constexpr (MODE == 2) {
+      w.wd0 = twd[2 * g];
+      w.wd1 = twd[2 * g + 1];
+    } else {
+      w.wf = twf[g];
     }
-    LT x[P][4][G];
+  };
+  // Reads group g's frame values (indices in cur), then the next group's
+  // words into nxt, then adds group g's products.
+  auto step = [&](uint32_t g, const Words& cur, Words& nxt) {
+    const uint32_t l[4] = {cur.l.x & 0xffffu, cur.l.x >> 16, cur.l.y & 0xffffu,
+                           cur.l.y >> 16};
+    LT x[4][G];
 #pragma unroll
-    for (int p = 0; p < P; p++)
+    for (int e = 0; e < 4; e++)
 #pragma unroll
-      for (int e = 0; e < 4; e++)
+      for (int j = 0; j < G; j++) x[e][j] = base[j * kLdsStride + l[e]];
+    read_words(g + 1 < gb ? g + 1 : g, nxt);
+    // Keep the reads above the products: the scheduler would otherwise sink
+    // the next group's reads to just before their use.
+    __builtin_amdgcn_sched_barrier(0);
+    float wf[4];
+    double wd[4];
+    if constexpr (MODE == 2) {
+      wd[0] = cur.wd0.x;
+      wd[1] = cur.wd0.y;
+      wd[2] = cur.wd1.x;
+      wd[3] = cur.wd1.y;
+      wf[0] = wf[1] = wf[2] = wf[3] = 0.f;
+    } else {
+      wf[0] = cur.wf.x;
+      wf[1] = cur.wf.y;
+      wf[2] = cur.wf.z;
+      wf[3] = cur.wf.w;
+      wd[0] = wd[1] = wd[2] = wd[3] = 0.0;
+    }
 #pragma unroll
-        for (int j = 0; j < G; j++) x[p][e][j] = base[j * kLdsStride + l[p][e]];
+    for (int e = 0; e < 4; e++)
 #pragma unroll
-    for (int p = 0; p < P; p++)
-#pragma unroll
-      for (int e = 0; e < 4; e++)
-#pragma unroll
-        for (int j = 0; j < G; j++) {
-          acc[j].add(wf[p][e], wd[p][e], x[p][e][j]);
-          if constexpr (VAR) var[j].add_var(wf[p][e], wd[p][e], x[p][e][j]);
-        }
+      for (int j = 0; j < G; j++) {
+        acc[j].add(wf[e], wd[e], x[e][j]);
+        if constexpr (VAR) var[j].add_var(wf[e], wd[e], x[e][j]);
+      }
+  };
+  if (ga < gb) {
+    Words wa, wb;
+    read_words(ga, wa);
+    for (uint32_t g = ga;;) {
+      step(g, wa, wb);
+      if (++g >= gb) break;
+      step(g, wb, wa);
+      if (++g >= gb) break;
+    }
   }
   SegVals<G, MODE, VAR> o;
 #pragma unroll

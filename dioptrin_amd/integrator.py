@@ -238,6 +238,36 @@ class GpuPlan:
                              f"{self.shape}")
         return np.ascontiguousarray(arr)
 
+    def _check_device(self, frames, out_i, out_s=None, out_d=None, errors=False,
+                      azimuthal_std=False, masks=None, outputs=True) -> str:
+        """Host-side checks of device buffers before the C ABI trusts their
+        sizes: frames (n, dim1, dim2) of a supported dtype, outputs float64
+        (n, nbins), sigma present with errors and the spread with
+        azimuthal_std, masks uint8 of the frames' shape. Returns the dtype
+        name."""
+        dt = np.dtype(frames.dtype)
+        if dt not in _NP_DTYPES:
+            raise TypeError(f"unsupported dtype {dt}; expected float64, float32, uint16, "
+                            f"uint32 or int32")
+        if len(frames.shape) != 3 or tuple(frames.shape[1:]) != self.shape:
+            raise ValueError(f"frames have shape {tuple(frames.shape)}, but the plan expects "
+                             f"(n,) + {self.shape}")
+        n = frames.shape[0]
+        want = (n, self.nbins)
+        for name, buf, needed in (("intensity", out_i, outputs), ("sigma", out_s, errors),
+                                  ("azimuthal std", out_d, azimuthal_std)):
+            if buf is None:
+                if needed:
+                    raise ValueError(f"{name} output buffer required")
+                continue
+            if np.dtype(buf.dtype) != np.float64 or tuple(buf.shape) != want:
+                raise ValueError(f"{name} output must be float64 of shape {want}, got "
+                                 f"{np.dtype(buf.dtype)} {tuple(buf.shape)}")
+        if masks is not None and (tuple(masks.shape) != tuple(frames.shape)
+                                  or np.dtype(masks.dtype) != np.uint8):
+            raise ValueError("masks must be uint8 of the frames' shape")
+        return _NP_DTYPES[dt]
+
     def integrate_stack(self, stack: np.ndarray, errors=False, azimuthal_std=False):
         """Raw arrays for a (n, dim1, dim2) stack: intensity, sigma, std."""
         stack = self._stack(stack)
@@ -275,10 +305,9 @@ class GpuPlan:
         """Device variant of integrate_masked: frames (n, dim1, dim2) and masks
         (n, dim1, dim2) uint8 DeviceArrays, (n, nbins) float64 outputs;
         enqueued on `stream` (None = the plan's stream)."""
-        if tuple(masks.shape) != tuple(frames.shape) or np.dtype(masks.dtype) != np.uint8:
-            raise ValueError("masks must be uint8 of the frames' shape")
+        dt = self._check_device(frames, out_i, out_s, None, errors, False, masks)
         N.check(N.lib().dio_integrate_masked_device(
-            self._h, frames.ptr, N.DTYPES[_NP_DTYPES[frames.dtype]], frames.shape[0], masks.ptr,
+            self._h, frames.ptr, N.DTYPES[dt], frames.shape[0], masks.ptr,
             N.WITH_ERRORS if errors else 0, out_i.ptr, None if out_s is None else out_s.ptr,
             stream))
 
@@ -309,7 +338,7 @@ class GpuPlan:
         """Enqueues integration of device frames (DeviceArray (n, dim1, dim2))
         into device outputs ((n, nbins) float64 DeviceArrays) on `stream`
         (a hipStream_t handle; None = the plan's stream)."""
-        dt = _NP_DTYPES[frames.dtype]
+        dt = self._check_device(frames, out_i, out_s, out_d, errors, azimuthal_std)
         flags = (N.WITH_ERRORS if errors else 0) | (N.WITH_AZIMUTHAL_STD if azimuthal_std else 0)
         N.check(N.lib().dio_integrate_device(
             self._h, frames.ptr, N.DTYPES[dt], frames.shape[0], flags, out_i.ptr,
@@ -328,7 +357,7 @@ class GpuPlan:
 
     def time_gather(self, frames, errors=False, reps=20) -> float:
         """Average ms of one launch of the segment kernel over one tile."""
-        dt = _NP_DTYPES[frames.dtype]
+        dt = self._check_device(frames, None, outputs=False)
         ms = C.c_double()
         N.check(N.lib().dio_plan_time_gather(self._h, frames.ptr, N.DTYPES[dt],
                                              frames.shape[0], N.WITH_ERRORS if errors else 0,
@@ -340,7 +369,7 @@ class GpuPlan:
         device-resident integration of the batch, timed with HIP events on
         the plan's stream (a launch covers one pass, or two passes sharing
         the table)."""
-        dt = _NP_DTYPES[frames.dtype]
+        dt = self._check_device(frames, out_i, out_s, None, errors)
         cap = frames.shape[0]  # at least one frame per launch
         buf = (C.c_float * cap)()
         nfr = (C.c_uint32 * cap)()
@@ -360,7 +389,7 @@ class GpuGraph:
     """hipGraph of one device-resident integration, replayed by launch()."""
 
     def __init__(self, plan: GpuPlan, frames, out_i, out_s, out_d, errors, azimuthal_std):
-        dt = _NP_DTYPES[frames.dtype]
+        dt = plan._check_device(frames, out_i, out_s, out_d, errors, azimuthal_std)
         flags = (N.WITH_ERRORS if errors else 0) | (N.WITH_AZIMUTHAL_STD if azimuthal_std else 0)
         h = C.c_void_p()
         N.check(N.lib().dio_graph_create(

@@ -128,6 +128,26 @@ def frame_pilatus(seed: int, shape=(1043, 981), poni=None) -> np.ndarray:
     return rng.poisson(rate).astype(np.float32)
 
 
+def frame_eiger4m(seed: int) -> np.ndarray:
+    """uint32 frame of eiger4m_ss2d: background 100 + rings modulated by
+    1 + 0.3 cos(2 chi), 10 % of ring pixels x U[1, 20] (spotty), Poisson
+    sampled with `seed` (the ring amplitudes and spots are fixed)."""
+    name = "eiger4m_ss2d"
+    w = WORKLOADS[name]
+    rate = ring_rate(w["shape"], workload_poni(name), seed=1234, azimuthal_modulation=0.3,
+                     spotty=True)
+    return np.random.default_rng(seed).poisson(rate).astype(np.uint32)
+
+
+def frame_tilted(seed: int) -> np.ndarray:
+    """float32 frame of pilatus1m_tilted: background 100 + rings of the tilted
+    geometry, Poisson sampled with `seed`."""
+    name = "pilatus1m_tilted"
+    w = WORKLOADS[name]
+    rate = ring_rate(w["shape"], workload_poni(name), seed=1234)
+    return np.random.default_rng(seed).poisson(rate).astype(np.float32)
+
+
 def frames_device(name: str, n: int, seed: int, device: int = 0):
     """n uint32 frames generated on the device (Poisson of the ring pattern
     plus a per-frame background U[10, 500], per-frame seeded streams)."""

@@ -87,3 +87,48 @@ def test_input_order_descriptor():
     assert d.input_order == N.ORDERS["detector"]
     with pytest.raises(ValueError):
         plan_args(poni, (8, 9), input_order="sideways")
+
+
+def test_image_of_2_29_pixels_is_rejected_before_the_device():
+    # The tiled gather addresses frames with 32-bit byte offsets: images of
+    # 2^29 pixels or more are refused with DIO_ERR_INVALID at plan creation.
+    from dioptrin_amd import _native as N
+    L = N.lib()
+    p = N.Poni(1e-4, 1e-4, 0.1, 0.0, 0.0, 0.0, 0.0, 0.0, 1e-10)
+    d = N.PlanDesc()
+    d.dim1, d.dim2, d.ndim, d.n_radial = 1 << 14, 1 << 15, 1, 10
+    rc = L.dio_plan_create(None, ctypes.byref(p), ctypes.byref(d), ctypes.byref(ctypes.c_void_p()))
+    assert rc == N.DIO_ERR_INVALID
+    assert b"2^29" in L.dio_last_error()
+
+
+class _Buf:
+    def __init__(self, shape, dtype):
+        self.shape, self.dtype, self.ptr = tuple(shape), dtype, 0
+
+
+def test_device_buffers_are_checked_on_the_host():
+    """GpuPlan validates device frames, outputs and masks before the C ABI,
+    which trusts their sizes."""
+    import numpy as np
+    from dioptrin_amd.integrator import GpuPlan
+    plan = GpuPlan.__new__(GpuPlan)
+    plan.shape, plan.nbins = (8, 9), 5
+    fr = _Buf((3, 8, 9), np.uint32)
+    ok = _Buf((3, 5), np.float64)
+    assert plan._check_device(fr, ok) == "uint32"
+    assert plan._check_device(fr, ok, ok, errors=True) == "uint32"
+    bad = [
+        dict(frames=_Buf((3, 8, 10), np.uint32), out_i=ok),       # frame shape
+        dict(frames=_Buf((3, 8, 9), np.int64), out_i=ok),         # dtype
+        dict(frames=fr, out_i=_Buf((2, 5), np.float64)),          # too few rows
+        dict(frames=fr, out_i=_Buf((3, 5), np.float32)),          # output dtype
+        dict(frames=fr, out_i=ok, errors=True),                   # sigma missing
+        dict(frames=fr, out_i=ok, out_s=_Buf((3, 4), np.float64), errors=True),
+        dict(frames=fr, out_i=ok, azimuthal_std=True),            # std missing
+        dict(frames=fr, out_i=ok, masks=_Buf((3, 8, 9), np.int8)),
+        dict(frames=fr, out_i=None),                              # intensity missing
+    ]
+    for kw in bad:
+        with pytest.raises((ValueError, TypeError)):
+            plan._check_device(**kw)
