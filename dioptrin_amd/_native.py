@@ -25,7 +25,8 @@ EXPORTS = [
     "dio_plan_create", "dio_plan_destroy", "dio_plan_info_get", "dio_plan_radial",
     "dio_plan_azimuthal", "dio_plan_export_csr", "dio_plan_denominators", "dio_plan_normalization",
     "dio_plan_counts",
-    "dio_integrate", "dio_integrate_device", "dio_integrate_masked",
+    "dio_integrate", "dio_integrate_device", "dio_integrate_device_noalloc",
+    "dio_integrate_masked",
     "dio_integrate_masked_device", "dio_plan_autotune", "dio_plan_set_tile",
     "dio_graph_create", "dio_graph_launch", "dio_graph_destroy", "dio_plan_time_gather",
     "dio_plan_time_passes", "dio_device_alloc", "dio_device_free", "dio_memcpy",
@@ -125,6 +126,7 @@ def lib() -> C.CDLL:
             "dio_plan_counts": (i32, [vp, dp]),
             "dio_integrate": (i32, [vp, vp, i32, u64, u32, dp, dp, dp]),
             "dio_integrate_device": (i32, [vp, vp, i32, u64, u32, dp, dp, dp, vp]),
+            "dio_integrate_device_noalloc": (i32, [vp, vp, i32, u64, u32, dp, dp, dp, vp]),
             "dio_integrate_masked": (i32, [vp, vp, i32, u64, vp, u32, dp, dp]),
             "dio_integrate_masked_device": (i32, [vp, vp, i32, u64, vp, u32, dp, dp, vp]),
             "dio_plan_autotune": (i32, [vp, i32, u64, u32, C.POINTER(u32), C.POINTER(u32)]),

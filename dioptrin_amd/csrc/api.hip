@@ -187,6 +187,7 @@ int dio_open(int device, dio_ctx** out) {
       c->device = device;
       DIO_HIP(hipSetDevice(device));
       DIO_HIP(hipGetDeviceProperties(&c->prop, device));
+      dio::noalloc_init(device);  // device address of the fixed scratch, this device current
       slot = std::move(c);
     }
     slot->refs++;
@@ -380,6 +381,56 @@ int dio_integrate_device(dio_plan* p, const void* d_frames, int dtype, uint64_t 
     DIO_HIP(hipSetDevice(p->device));
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
     run_device(p, p->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std, s);
+  });
+}
+
+int dio_integrate_device_noalloc(dio_plan* p, const void* d_frames, int dtype,
+                                 uint64_t n_frames, uint32_t flags, double* d_intensity,
+                                 double* d_sigma, double* d_std, void* stream) {
+  return guarded([&] {
+    require(p != nullptr, DIO_ERR_INVALID, "null plan");
+    if (n_frames == 0) return;
+    require(d_frames && d_intensity, DIO_ERR_INVALID, "null frames or output");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DIO_HIP(hipSetDevice(p->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    if (flags & DIO_WITH_AZIMUTHAL_STD) {  // the spread kernel uses no scratch
+      run_device(p, p->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std, s);
+      return;
+    }
+    const bool var = flags & DIO_WITH_ERRORS;
+    require(!var || d_sigma != nullptr, DIO_ERR_INVALID, "sigma output missing");
+    (void)dtype_bytes(dtype);
+    size_t cap = 0;
+    void* scratch = dio::noalloc_scratch(p->device, &cap);
+    // The plan's pass shape if one pass of partials fits the fixed scratch,
+    // else the largest smaller one that does (a frame's bits do not depend
+    // on the pass shape).
+    static const uint32_t kShapes[][2] = {{16, 4}, {8, 4}, {4, 4}, {2, 2}, {1, 1}};
+    uint32_t F = p->tile, G = p->lanes;
+    const int prec = p->desc.precision;
+    if (dio::pass_bytes(p->tt, F, var, prec) > cap) {
+      F = 0;
+      for (const auto& sh : kShapes) {
+        if (sh[0] < p->tile && dio::valid_tile(prec, sh[0], sh[1]) &&
+            dio::pass_bytes(p->tt, sh[0], var, prec) <= cap) {
+          F = sh[0];
+          G = sh[1];
+          break;
+        }
+      }
+      if (!F)
+        dio::fail(DIO_ERR_INVALID,
+                  "the segment partials of one frame exceed the fixed scratch of "
+                  "dio_integrate_device_noalloc (64 MiB)");
+    }
+    const uint32_t passes = (uint32_t)std::min<size_t>(
+        dio::passes_per_launch(p->tt, F, dtype, prec), cap / dio::pass_bytes(p->tt, F, var, prec));
+    dio::TileScratch sc;
+    sc.ext = static_cast<double*>(scratch);
+    sc.ext_n = cap / sizeof(double);
+    dio::tiled_gather(p->tt, sc, d_frames, dtype, n_frames, var, prec, F, G, d_intensity,
+                      d_sigma, s, nullptr, (flags & DIO_RAW_SUMS) != 0, std::max(1u, passes));
   });
 }
 

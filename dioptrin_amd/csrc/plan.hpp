@@ -56,6 +56,12 @@ namespace dio {
 void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype, uint64_t n_frames,
                 uint32_t flags, double* I, double* S, double* SD, hipStream_t s,
                 dio::LaunchEvents* launches = nullptr);
+// Fixed scratch of dio_integrate_device_noalloc (noalloc.hip): its device
+// address is taken once per device, with that device current (dio_open);
+// noalloc_scratch returns it and the usable bytes.
+void noalloc_init(int device);
+void* noalloc_scratch(int device, size_t* bytes);
+size_t noalloc_capacity();
 // Plan construction behind dio_plan_create (validates, builds on the device).
 dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& desc);
 }  // namespace dio

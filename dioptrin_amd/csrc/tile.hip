@@ -1446,14 +1446,23 @@ bool double_buffered(const TiledTable& t, uint32_t F, int dtype, int precision) 
   return F == 16 && precision != DIO_F64 && t.db && w32;
 }
 
+size_t pass_bytes(const TiledTable& t, uint32_t F, bool var, int precision) {
+  return (size_t)t.nseg * F * (var ? 2 : 1) * (precision == DIO_F64 ? sizeof(double)
+                                                                     : sizeof(float));
+}
+
+uint32_t passes_per_launch(const TiledTable& t, uint32_t F, int dtype, int precision) {
+  if (precision == DIO_F64) return 1;
+  return double_buffered(t, F, dtype, precision) ? kMaxPassesPerLaunch : 2;
+}
+
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t F, uint32_t G,
                   double* intensity, double* sigma, hipStream_t s, LaunchEvents* le,
-                  bool raw) {
+                  bool raw, uint32_t max_passes) {
   if (!valid_tile(precision, F, G))
     fail(DIO_ERR_INVALID, "invalid frames per pass / frames per lane");
   const size_t esz = dtype_bytes(dtype);
-  sc.ensure(t, F, var);
   const char* fb = static_cast<const char*>(frames);
   const uint64_t npix = (uint64_t)t.dim1 * t.dim2;
   const bool plain = precision == DIO_F32, f64 = precision == DIO_F64;
@@ -1461,12 +1470,20 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   // the float64 ones of one). The double-buffered kernel runs that many
   // passes in one launch over one read of the table; otherwise a launch is a
   // pass. A reduction launch serves up to 32 frames (two 16-frame passes).
+  // The frames' arithmetic does not depend on the passes per launch.
   const bool db = !f64 && double_buffered(t, F, dtype, precision);
-  const uint32_t P = f64 ? 1 : db ? kMaxPassesPerLaunch : 2;
+  uint32_t P = passes_per_launch(t, F, dtype, precision);
+  const size_t pbytes = pass_bytes(t, F, var, precision);
+  if (max_passes) {
+    P = std::min(P, max_passes);
+    if (!sc.ext || sc.ext_n * sizeof(double) < (size_t)P * pbytes)
+      fail(DIO_ERR_INVALID, "external tile scratch is too small");
+  } else {
+    sc.ensure(t, F, var);
+  }
   const uint32_t FR = F * P;
   const uint32_t RF = std::min(FR, 32u);  // frames per reduction workgroup
   const uint64_t BS = (uint64_t)t.nseg * F * (var ? 2 : 1);  // elements per pass block
-  const size_t pbytes = BS * (f64 ? sizeof(double) : sizeof(float));
   auto timed = [&](uint32_t frames_in_launch, auto&& launch) {
     const bool rec = le && le->n < le->cap;
     if (rec) DIO_HIP(hipEventRecord(le->ev[2 * le->n], s));

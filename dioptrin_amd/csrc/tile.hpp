@@ -98,10 +98,21 @@ struct LaunchEvents {
 // double-buffered kernel, which takes two passes per launch.
 bool double_buffered(const TiledTable& t, uint32_t frames_per_pass, int dtype, int precision);
 
+// Bytes of segment partials one pass of frames_per_pass frames writes.
+size_t pass_bytes(const TiledTable& t, uint32_t frames_per_pass, bool var, int precision);
+
+// Passes per launch of the gather (several share one read of the table in the
+// double-buffered kernel; partials of that many passes are held at once).
+uint32_t passes_per_launch(const TiledTable& t, uint32_t frames_per_pass, int dtype,
+                           int precision);
+
+// max_passes > 0 caps the passes per launch and uses sc's external memory
+// only (it must hold max_passes pass blocks): the gather then allocates
+// nothing.
 void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                   uint64_t n_frames, bool var, int precision, uint32_t frames_per_pass,
                   uint32_t frames_per_lane, double* intensity, double* sigma, hipStream_t s,
-                  LaunchEvents* launches = nullptr, bool raw = false);
+                  LaunchEvents* launches = nullptr, bool raw = false, uint32_t max_passes = 0);
 
 // Only the tile kernel for one pass of nf <= frames_per_pass frames (timing).
 void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,

@@ -344,6 +344,19 @@ class GpuPlan:
             self._h, frames.ptr, N.DTYPES[dt], frames.shape[0], flags, out_i.ptr,
             None if out_s is None else out_s.ptr, None if out_d is None else out_d.ptr, stream))
 
+    def integrate_device_noalloc(self, frames, out_i, out_s=None, out_d=None, errors=False,
+                                 azimuthal_std=False, raw=False, stream=None):
+        """integrate_device through dio_integrate_device_noalloc, which
+        allocates no device memory (its partials use a fixed 64 MiB array
+        per device); same results bit for bit. Calls that overlap on one
+        device race."""
+        dt = self._check_device(frames, out_i, out_s, out_d, errors, azimuthal_std)
+        flags = ((N.WITH_ERRORS if errors else 0) | (N.WITH_AZIMUTHAL_STD if azimuthal_std else 0)
+                 | (N.RAW_SUMS if raw else 0))
+        N.check(N.lib().dio_integrate_device_noalloc(
+            self._h, frames.ptr, N.DTYPES[dt], frames.shape[0], flags, out_i.ptr,
+            None if out_s is None else out_s.ptr, None if out_d is None else out_d.ptr, stream))
+
     def autotune(self, dtype="uint32", n_frames=64, errors=False):
         """Picks and stores (frames per pass, frames per lane); returns them."""
         t, g = C.c_uint32(), C.c_uint32()

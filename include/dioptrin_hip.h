@@ -160,6 +160,20 @@ int dio_integrate_device(dio_plan *plan, const void *d_frames, int dtype, uint64
                          uint32_t flags, double *d_intensity, double *d_sigma, double *d_std,
                          void *stream);
 
+/* dio_integrate_device that allocates nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, flags and results (bit for bit) as
+ * dio_integrate_device; the tile kernel's segment partials go to a fixed
+ * 64 MiB device array that each device holds from dio_open until the
+ * process ends. Passes are shortened to fit it; a plan whose partials of a
+ * single frame exceed it is refused with DIO_ERR_INVALID. Every call on one
+ * device shares that array: calls that overlap on one device (any plan,
+ * any stream) race, so the caller serialises them. */
+int dio_integrate_device_noalloc(dio_plan *plan, const void *d_frames, int dtype,
+                                 uint64_t n_frames, uint32_t flags, double *d_intensity,
+                                 double *d_sigma, double *d_std, void *stream);
+
 /* Dynamic (per-frame) masks: Dynamic1dPlan::integrate(data, Some(mask),
  * errors, false) (src/integration/dynamic.rs:143-186; row sums
  * src/lut/prepared.rs:1011-1121, normalize_bin 985-996). Each frame comes
