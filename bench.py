@@ -80,10 +80,29 @@ def dynamic_mask_rate(plan, frames_dev, n, errors):
     return round(best, 1)
 
 
+def host_cpu():
+    """nproc, the model name from /proc/cpuinfo and the cores this job may
+    use (OMP_NUM_THREADS when set, as the GPU host sets it to the job's share
+    of its cores; else every core)."""
+    n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    share = int(os.environ.get("OMP_NUM_THREADS") or n)
+    return n, model, max(1, min(n, share))
+
+
 def cpu_baseline(frames_dev, poni, w, seconds=12.0):
-    """Restated reference CPU batch1d (CSC, 4-frame tiles, all host cores)."""
+    """Restated reference CPU batch1d (CSC, 4-frame tiles) on the cores this
+    job may use, and the strict single-thread FP64 oracle gather beside it."""
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    nproc, model, threads = host_cpu()
     t0 = time.time()
     lut = O.build_lut1d(O.Poni(**poni), *w["shape"], w["n_radial"])
     build_s = time.time() - t0
@@ -98,9 +117,19 @@ def cpu_baseline(frames_dev, poni, w, seconds=12.0):
         el = time.time() - t0
         if el >= seconds:
             break
+    # Strict oracle: sequential FP64 CSR rows, one thread, a few frames.
+    den = O.denominators(lut, norm)
+    sx = [host[k].astype(np.float64) for k in range(4)]
+    ns, t1 = 0, time.time()
+    while ns < 4 or time.time() - t1 < 2.0:
+        O.gather(lut, den, sx[ns % 4], False)
+        ns += 1
+    strict = ns / (time.time() - t1)
     return {"value": done / el, "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": f"{done} frames ({nf} distinct frames x {done // nf} passes) of the "
                       f"same workload, {el:.1f} s wall",
+            "nproc": nproc, "cpu_model": model,
+            "strict_oracle_1thread_fps": strict,
             "lut_build_ms": build_s * 1e3}
 
 
@@ -356,9 +385,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f32_compensated": "f32x2 (u32 frames, f32 weights)",
-                  "f32_wide": "f64 (u32 frames, f32 weights)", "f32": "f32",
-                  "f64": "f64"}[args.precision],
+        "dtype": {"f32_compensated": "f32 compensated (f32 weights and signal, f32 partials)",
+                  "f32_wide": "f32 (f32 weights and signal, f64 sums of segments of <= 48 "
+                              "entries, f32 partials, f64 bin reduction)",
+                  "f32": "f32", "f64": "f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"{WORKLOAD}: {nfr} x {w['shape'][0]}x{w['shape'][1]} uint32, "
                                f"{bins} bins pixel_split, device-resident frames",

@@ -22,6 +22,7 @@ import numpy as np
 
 from . import _native as N
 from .poni import load_poni  # noqa: F401  (part of the package API)
+from .tiff import read_tiff
 
 _NP_DTYPES = {np.dtype(np.uint16): "uint16", np.dtype(np.uint32): "uint32",
               np.dtype(np.int32): "int32", np.dtype(np.float32): "float32",
@@ -542,8 +543,14 @@ class Integrator:
 
     def batch1d(self, images, n_points: int, errors: bool = False, num_workers=None,
                 num_threads=None, *, azimuthal_std: bool = False) -> list:
-        """batch1d over arrays: per-frame results in input order; a frame that
-        cannot be integrated reports its error without failing the others."""
+        """batch1d (dioptrin.pyi:845-904, python/integrator.rs:1867-2210):
+        a list of 2D arrays or of TIFF paths (or one 3D stack) -> one
+        BatchResult per image, in input order, named "array[i]" or by path.
+        Arrays that are empty, not C-contiguous or of another shape than the
+        mask raise ValueError, and unsupported dtypes TypeError, for the
+        whole call; a file that cannot be read or does not match the mask
+        reports its error in its own BatchResult. num_workers/num_threads
+        size the reference's CPU pools and are accepted for compatibility."""
         if n_points <= 0:
             raise ValueError("n_points must be positive")
         return self._batch(images, lambda shape: self._plan(shape, n_points), errors,
@@ -551,25 +558,68 @@ class Integrator:
 
     def batch2d(self, images, n_radial: int, n_azimuthal: int, errors: bool = False,
                 azimuthal_range=None, num_workers=None, num_threads=None) -> list:
+        """batch2d (python/integrator.rs:2423): batch1d's inputs and error
+        rules, 2D results."""
+        if n_radial <= 0 or n_azimuthal <= 0:
+            raise ValueError("n_radial and n_azimuthal must be positive")
         return self._batch(images, lambda shape: self._plan(shape, n_radial, n_azimuthal,
                                                            azimuthal_range), errors, False)
 
+    # Sample types the device reads directly, and exact widenings of the
+    # others a TIFF may hold (the reference converts every sample to f64).
+    _WIDEN = {np.dtype(np.uint8): np.uint16, np.dtype(np.int8): np.int32,
+              np.dtype(np.int16): np.int32, np.dtype(np.uint64): np.float64,
+              np.dtype(np.int64): np.float64}
+
     def _batch(self, images, plan_for, errors, azimuthal_std):
-        images = list(images) if not (isinstance(images, np.ndarray) and images.ndim == 3) \
-            else [images[i] for i in range(images.shape[0])]
-        out: list = [None] * len(images)
+        if isinstance(images, (str, bytes, bytearray, os.PathLike)):
+            raise TypeError("images must be a list of paths or arrays; wrap one path in a list")
+        if isinstance(images, np.ndarray):
+            if images.ndim != 3:
+                raise TypeError("a single array must be a 3D stack (n_frames, dim1, dim2)")
+            items = [images[i] for i in range(images.shape[0])]
+        else:
+            items = list(images)
+        is_path = [isinstance(x, (str, os.PathLike)) for x in items]
+        if any(is_path) and not all(is_path):
+            raise TypeError("images must be all paths or all arrays")
+        out: list = [None] * len(items)
+        frames = []  # (index, name, array)
+        for i, im in enumerate(items):
+            if is_path[i]:
+                name = os.fspath(im)
+                try:
+                    arr = read_tiff(name)
+                except (OSError, ValueError) as e:
+                    out[i] = BatchResult(name, None, str(e))
+                    continue
+                arr = arr.astype(self._WIDEN.get(arr.dtype, arr.dtype), copy=False)
+                if self.mask is not None and arr.shape != self.mask.shape:
+                    out[i] = BatchResult(name, None, f"image shape {arr.shape} does not match "
+                                                     f"mask shape {self.mask.shape}")
+                    continue
+            else:
+                if not isinstance(im, np.ndarray) or im.ndim != 2:
+                    raise TypeError(f"images[{i}] must be a 2D numpy array")
+                if im.dtype not in _NP_DTYPES:
+                    raise TypeError(f"images[{i}] has unsupported dtype {im.dtype}; expected "
+                                    f"float64, float32, uint16, uint32 or int32")
+                if im.size == 0:
+                    raise ValueError(f"images[{i}] is empty")
+                if not im.flags["C_CONTIGUOUS"]:
+                    raise ValueError(f"Array {i} must be contiguous")
+                if self.mask is not None and im.shape != self.mask.shape:
+                    raise ValueError(f"images[{i}] shape {im.shape} does not match mask "
+                                     f"shape {self.mask.shape}")
+                name, arr = f"array[{i}]", im
+            frames.append((i, name, arr))
         groups: dict = {}
-        for i, im in enumerate(images):
-            im = np.asarray(im)
-            groups.setdefault((im.shape, im.dtype.str), []).append(i)
-        for (shape, _), idx in groups.items():
-            try:
-                plan = plan_for(shape)
-                res = plan.batch1d([images[i] for i in idx], errors=errors,
-                                   azimuthal_std=azimuthal_std)
-                for i, r in zip(idx, res):
-                    out[i] = BatchResult(f"array[{i}]", r, None)
-            except (ValueError, TypeError) as e:
-                for i in idx:
-                    out[i] = BatchResult(f"array[{i}]", None, str(e))
+        for i, name, arr in frames:
+            groups.setdefault((arr.shape, arr.dtype.str), []).append((i, name, arr))
+        for (shape, _), members in groups.items():
+            plan = plan_for(shape)
+            res = plan.batch1d([a for _, _, a in members], errors=errors,
+                               azimuthal_std=azimuthal_std)
+            for (i, name, _), r in zip(members, res):
+                out[i] = BatchResult(name, r, None)
         return out
