@@ -394,3 +394,43 @@ def pyfai_split1d(poly, r0, r1, nbins):
     n = lib().or_pyfai_split1d(_d(p), C.c_double(r0), C.c_double(r1), C.c_size_t(nbins),
                                b.ctypes.data_as(C.c_void_p), _d(w), C.c_size_t(64))
     return list(zip(b[:n].tolist(), w[:n].tolist()))
+
+
+def pyfai_split1d_sector(poly, r0, r1, nbins, a0, a1, sector=True, sign=1.0):
+    """split1d (pyfai_1d.rs:259-354) on one polygon with an azimuthal range
+    [a0, a1], sector exclusion and corner sign."""
+    p = np.ascontiguousarray(poly, np.float64).ravel()
+    b = np.empty(64, np.uint64)
+    w = np.empty(64)
+    L = lib()
+    L.or_pyfai_split1d_sector.restype = C.c_int
+    n = L.or_pyfai_split1d_sector(_d(p), C.c_double(r0), C.c_double(r1), C.c_size_t(nbins),
+                                  C.c_double(a0), C.c_double(a1), C.c_int(int(sector)),
+                                  C.c_double(sign), b.ctypes.data_as(C.c_void_p), _d(w),
+                                  C.c_size_t(64))
+    return list(zip(b[:n].tolist(), w[:n].tolist()))
+
+
+def wrap_tau(x: float) -> float:
+    """pyfai_1d.rs:180-186: x.rem_euclid(TAU), values in [0, TAU) unchanged."""
+    L = lib()
+    L.or_wrap_tau.restype = C.c_double
+    L.or_wrap_tau.argtypes = [C.c_double]
+    return L.or_wrap_tau(x)
+
+
+GRID_RADIAL_END, GRID_INDEX_ROUNDING, GRID_RADIAL_OUTSIDE, GRID_AZIMUTHAL_OUTSIDE = 1, 2, 4, 8
+
+
+def grid2d_bin(nr, na, radial_range, azimuthal_range, flags, radial, az):
+    """IntegrationGrid2d::bin (grid.rs:176-221): the bin index, None for no
+    bin; raises ValueError for an invalid grid."""
+    L = lib()
+    L.or_grid2d_bin.restype = C.c_longlong
+    b = L.or_grid2d_bin(C.c_size_t(nr), C.c_size_t(na), C.c_double(radial_range[0]),
+                        C.c_double(radial_range[1]), C.c_double(azimuthal_range[0]),
+                        C.c_double(azimuthal_range[1]), C.c_int(flags), C.c_double(radial),
+                        C.c_double(az))
+    if b == -2:
+        raise ValueError("invalid grid")
+    return None if b < 0 else int(b)

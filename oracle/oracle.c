@@ -1035,16 +1035,17 @@ static void pyfai_range(const double *rc, size_t nc, double *lo, double *hi) {
   *hi = mx * (1.0 + (double)FLT_EPSILON);
 }
 
-/* pyfai_1d.rs:259-354 (split1d) with sector = false and sign = +1 (no
- * detector: corner_sign, :131-154). Appends the pixel's entries. */
-static void split1d(double poly[4][2], double r0, double r1, size_t bins, size_t pixel,
-                    bucket *bk) {
-  double a0 = -OR_PI, a1 = OR_PI;
+/* pyfai_1d.rs:259-354 (split1d): the azimuthal range [a0, a1] of the grid
+ * sets the seam offset; with `sector`, a pixel whose corners all lie below
+ * a0 or all above a1 is dropped; `sign` is corner_sign (:131-154; +1 with no
+ * detector). Appends the pixel's entries. */
+static void split1d(double poly[4][2], double r0, double r1, size_t bins, double a0, double a1,
+                    int sector, double sign, size_t pixel, bucket *bk) {
   double offset = (a0 >= -OR_PI && a1 <= OR_PI) ? 0.0
                   : (a0 >= 0.0 && a1 <= OR_TAU) ? OR_PI : a0 + OR_PI;
   for (int i = 0; i < 4; i++) poly[i][1] = wrap_tau(poly[i][1] - offset + OR_PI) - OR_PI;
   double *a = poly[0], *b = poly[1], *c = poly[2], *d = poly[3];
-  double area = 1.0 * (0.5 * (c[0] - a[0]) * (d[1] - b[1]) - (c[1] - a[1]) * (d[0] - b[0]));
+  double area = sign * (0.5 * (c[0] - a[0]) * (d[1] - b[1]) - (c[1] - a[1]) * (d[0] - b[0]));
   if (area > 0.0) {
     for (int i = 0; i < 4; i++)
       if (poly[i][1] < 0.0) poly[i][1] += OR_TAU;
@@ -1057,6 +1058,14 @@ static void split1d(double poly[4][2], double r0, double r1, size_t bins, size_t
       for (int i = 0; i < 4; i++) poly[i][1] -= OR_TAU;
   }
   for (int i = 0; i < 4; i++) poly[i][1] += offset;
+  if (sector) {
+    int below = 1, above = 1;
+    for (int i = 0; i < 4; i++) {
+      below = below && poly[i][1] < a0;
+      above = above && poly[i][1] > a1;
+    }
+    if (below || above) return;
+  }
   double dr = (r1 - r0) / (double)bins;
   for (int i = 0; i < 4; i++) poly[i][0] = (poly[i][0] - r0) / dr;
   double lo = INFINITY, hi = -INFINITY;
@@ -1136,7 +1145,7 @@ int or_build_lut1d_pyfai(const or_poni *p, size_t dim1, size_t dim2, size_t n_po
     }
     if (singular((const double(*)[2])poly)) sample_pixel(p, &t, dim2, pix, unit, n_points, r0, r1,
                                                           !has_range, bk);
-    else split1d(poly, r0, r1, n_points, pix, bk);
+    else split1d(poly, r0, r1, n_points, -OR_PI, OR_PI, 0, 1.0, pix, bk);
   }
   for (size_t k = 0; k < n_points; k++) entries += bk[k].len;
   buckets_to_lut(bk, n_points, 1, r0, (r1 - r0) / (double)n_points, out);
@@ -1450,17 +1459,20 @@ void or_azimuthal_std(const or_lut *lut, const uint8_t *mask, const double *norm
   }
 }
 
+double or_wrap_tau(double x) { return wrap_tau(x); }
+
 /* Exposes split1d (pyfai_1d.rs:259-354) on one polygon for known-answer
  * checks; returns the entry count written to bins/weights (capacity cap). */
-int or_pyfai_split1d(const double *poly8, double r0, double r1, size_t nbins, uint64_t *bins,
-                     double *weights, size_t cap) {
+int or_pyfai_split1d_sector(const double *poly8, double r0, double r1, size_t nbins, double a0,
+                            double a1, int sector, double sign, uint64_t *bins, double *weights,
+                            size_t cap) {
   double poly[4][2];
   for (int i = 0; i < 4; i++) {
     poly[i][0] = poly8[2 * i];
     poly[i][1] = poly8[2 * i + 1];
   }
   bucket *bk = (bucket *)calloc(nbins, sizeof(bucket));
-  split1d(poly, r0, r1, nbins, 0, bk);
+  split1d(poly, r0, r1, nbins, a0, a1, sector, sign, 0, bk);
   size_t n = 0;
   for (size_t b = 0; b < nbins; b++) {
     for (size_t e = 0; e < bk[b].len; e++)
@@ -1474,4 +1486,25 @@ int or_pyfai_split1d(const double *poly8, double r0, double r1, size_t nbins, ui
   }
   free(bk);
   return (int)n;
+}
+
+int or_pyfai_split1d(const double *poly8, double r0, double r1, size_t nbins, uint64_t *bins,
+                     double *weights, size_t cap) {
+  return or_pyfai_split1d_sector(poly8, r0, r1, nbins, -OR_PI, OR_PI, 0, 1.0, bins, weights,
+                                 cap);
+}
+
+/* grid.rs:26-222 IntegrationGrid2d::new + bin for known-answer checks: flags
+ * bit 0 clamped_radial_end, 1 clamped_index_rounding, 2 clamped_radial_outside,
+ * 3 clamped_azimuthal_outside. Returns the bin, -1 for none, -2 for an
+ * invalid grid. */
+long long or_grid2d_bin(size_t nr, size_t na, double r0, double r1, double a0, double a1,
+                        int flags, double radial, double az) {
+  grid2d g;
+  if (grid_new(&g, nr, na, r0, r1, a0, a1)) return -2;
+  g.clamp_radial_end = (flags & 1) != 0;
+  g.clamp_index_rounding = (flags & 2) != 0;
+  g.clamp_radial_outside = (flags & 4) != 0;
+  g.clamp_azimuthal_outside = (flags & 8) != 0;
+  return grid_bin(&g, radial, az);
 }

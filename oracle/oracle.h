@@ -24,15 +24,19 @@
  *   azimuthal std   src/lut/azimuthal.rs:48-118
  * Each function's comment cites the lines it follows.
  *
- * Pinning (tests/test_oracle_golden.py): geometry by pyFAI vectors and closed
- * forms; bin rules, pixel split, supersampling, 2D grids, gather and std by
- * the reference's own known answers, brute force and invariants; pyFAI 1D by
- * pyFAI golden weights. Partially pinned: the per-bin distribution of the
- * pyFAI 2D split (totals and 1D agreement only). Parity unpinned: the
- * ulp-level bits of the reference's builders on large geometries, which
- * depend on Rust's platform libm and fold order; no reference test pins them
- * below 1e-10 / 1e-6, so bit-exactness is claimed against this oracle with
- * the host glibc.
+ * Pinning (tests/test_oracle_golden.py, tests/test_oracle_pins.py,
+ * tests/test_cpu_baseline.py): geometry by pyFAI vectors and closed forms;
+ * bin rules, pixel split, supersampling, 2D grids, gather and std by the
+ * reference's own known answers, brute force and invariants; pyFAI 1D by
+ * pyFAI golden weights and the reference's sector, renormalisation, seam and
+ * wrap_tau cases; 2D placement along the azimuth by brute force per pixel
+ * (exact polygon-rectangle areas for pixel split, pyFAI's raster rule for
+ * pyfai_full, subpixel counts for supersampling) and the reference's sector
+ * and index-rounding cases; the CSC baseline against the strict gather.
+ * Parity unpinned: the ulp-level bits of the reference's builders on large
+ * geometries, which depend on Rust's platform libm and fold order; no
+ * reference test pins them below 1e-10 / 1e-6, so bit-exactness is claimed
+ * against this oracle with the host glibc.
  */
 #ifndef DIOPTRIN_ORACLE_H
 #define DIOPTRIN_ORACLE_H
@@ -105,6 +109,17 @@ int or_build_lut1d_pyfai(const or_poni *p, size_t dim1, size_t dim2, size_t n_po
 int or_build_lut2d_pyfai(const or_poni *p, size_t dim1, size_t dim2, int unit, size_t n_radial,
                          size_t n_azimuthal, int has_rr, double rlo, double rhi, int has_ar,
                          double alo, double ahi, or_lut *out, double *edges);
+/* pyfai_1d.rs:259-354 split1d on one polygon with the grid's azimuthal range
+ * [a0, a1], sector exclusion and corner sign; pyfai_1d.rs:180-186 wrap_tau. */
+int or_pyfai_split1d_sector(const double *poly8, double r0, double r1, size_t nbins, double a0,
+                            double a1, int sector, double sign, uint64_t *bins, double *weights,
+                            size_t cap);
+/* grid.rs:26-222 IntegrationGrid2d bin with clamp flags (bit 0 radial end,
+ * 1 index rounding, 2 radial outside, 3 azimuthal outside); -1 none, -2
+ * invalid grid. */
+long long or_grid2d_bin(size_t nr, size_t na, double r0, double r1, double a0, double a1,
+                        int flags, double radial, double az);
+double or_wrap_tau(double x);
 int or_pyfai_split1d(const double *poly8, double r0, double r1, size_t nbins, uint64_t *bins,
                      double *weights, size_t cap);
 void or_lut_free(or_lut *lut);
