@@ -78,8 +78,6 @@ void check_desc(const dio_plan_desc& d) {
   if (d.has_polarization)
     require(d.polarization_factor >= -1.0 && d.polarization_factor <= 1.0, DIO_ERR_INVALID,
             "polarization factor must lie in [-1, 1]");
-  require(d.unit < DIO_UNIT_Q_A, DIO_ERR_UNSUPPORTED,
-          "q units are not available for device-built tables");
 }
 
 void ensure_out(dio::DevBuf<double>& b, size_t n) {
@@ -112,6 +110,11 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
 namespace dio {
 dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& desc) {
   check_desc(desc);
+  // RadialConverter::new (integration.rs:154-169) asserts a positive
+  // wavelength for q units.
+  if (desc.unit == DIO_UNIT_Q_A || desc.unit == DIO_UNIT_Q_NM)
+    require(poni.wavelength > 0.0, DIO_ERR_INVALID,
+            "wavelength must be positive for q units");
     auto t0 = std::chrono::steady_clock::now();
     DIO_HIP(hipSetDevice(ctx->device));
     auto p = std::make_unique<dio_plan>();

@@ -57,9 +57,25 @@ __global__ void k_patch(const uint32_t* idx, const double* tv, const double* cv,
   chi[idx[i]] = cv[i];
 }
 
-__global__ void k_to_unit(Geo g, const double* tth, double* out, uint64_t n) {
+// 2theta in the plan's radial unit; q values whose certified sine is
+// uncertain are listed for the host (out may alias tth).
+__global__ void k_to_unit(Geo g, const double* tth, double* out, uint64_t n, uint32_t* flist,
+                          uint32_t* fcount) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = tth_to_unit(g, tth[i]);
+  if (i >= n) return;
+  const UnitVal u = unit_cert(g, tth[i]);
+  out[i] = u.value;
+  if (u.uncertain) flist[atomicAdd(fcount, 1u)] = (uint32_t)i;
+}
+
+__global__ void k_gather_f64(const double* v, const uint32_t* idx, uint32_t n, double* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v[idx[i]];
+}
+
+__global__ void k_scatter_f64(const uint32_t* idx, const double* val, uint32_t n, double* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[idx[i]] = val[i];
 }
 
 // Block-level min/max partials (src/math/mod.rs:8-32 semantics for finite
@@ -309,12 +325,12 @@ __global__ void k_sing_samples(Geo g, const uint32_t* sing, uint32_t nsing, doub
 
 // pyfai_1d.rs:228-251 binning of the samples on the 1D grid (n radial bins,
 // one azimuthal bin over (-pi, pi)).
-__global__ void k_sing_bin(Geo g, SampleGrid G, uint32_t nsing, const double* stth,
+__global__ void k_sing_bin(SampleGrid G, uint32_t nsing, const double* srad,
                            const double* schi, uint32_t* hits) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (uint64_t)nsing * kSamplesPerPixel) return;
   uint32_t slot = (uint32_t)(i / kSamplesPerPixel);
-  double r = tth_to_unit(g, stth[i]);
+  double r = srad[i];
   double a0 = -kPiD, a1 = kPiD;
   double a = a0 + rem_euclid(schi[i] - a0, kTauD);
   if ((r < G.r0 && !G.clamp_low) || r >= G.r1 || a >= a1) return;
@@ -588,12 +604,12 @@ __global__ void k_sing_samples2d(Geo g, const uint32_t* sing, uint32_t nsing, do
   if (a.uncertain) flist[atomicAdd(fcount, 1u)] = (uint32_t)i;
 }
 
-__global__ void k_sing_bin2d(Geo g, Grid2 G, uint32_t nsing, const double* stth,
+__global__ void k_sing_bin2d(Grid2 G, uint32_t nsing, const double* srad,
                              const double* schi, uint32_t* hits) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (uint64_t)nsing * kSamplesPerPixel) return;
   uint32_t slot = (uint32_t)(i / kSamplesPerPixel);
-  long long b = grid_bin(G, tth_to_unit(g, stth[i]), schi[i]);
+  long long b = grid_bin(G, srad[i], schi[i]);
   if (b >= 0) atomicAdd(&hits[(uint64_t)slot * G.nr * G.na + (uint64_t)b], 1u);
 }
 
@@ -622,8 +638,8 @@ struct SsBin {
   int inv;  // 1: rf + (i + 0.5) * inv_ss coordinates; 0: (i + 0.5) / n_ss
 };
 
-__host__ __device__ inline long long ss_bin(const SsBin& B, double tth, double chi) {
-  double r = tth_to_unit(B.g, tth);
+// r: the subpixel's radial value in the plan's unit.
+__host__ __device__ inline long long ss_bin(const SsBin& B, double r, double chi) {
   if (B.twod) return grid_bin(B.G, r, chi);
   if (!B.clamped1 && !(r >= B.min1 && r <= B.max1)) return -1;
   unsigned long long u = sat_usize((r - B.min1) / B.step1);
@@ -660,14 +676,32 @@ __global__ void k_ss_bins(SsBin B, uint64_t nsub, uint32_t* bins, uint32_t* flis
   double d1, d2;
   ss_point(B, t, d1, d2);
   Angles a = angles_cert(B.g, d1, d2);
-  long long b = ss_bin(B, a.tth, a.chi);
+  const UnitVal u = unit_cert(B.g, a.tth);
+  long long b = ss_bin(B, u.value, a.chi);
   bool ambiguous = false;
-  if (a.uncertain) {
+  if (a.uncertain || u.uncertain) {
+    // The host's values are within one double of the certified ones: the
+    // subpixel is ambiguous when a neighbouring 2theta, chi or sine (q
+    // units) changes its bin.
     double tc[3] = {a.tth, next_up(a.tth), next_down(a.tth)};
     double cc[3] = {a.chi, next_up(a.chi), next_down(a.chi)};
     int nt = (a.uncertain & 1) ? 3 : 1, nc = (B.twod && (a.uncertain & 2)) ? 3 : 1;
-    for (int x = 0; x < nt; x++)
-      for (int y = 0; y < nc; y++) ambiguous |= ss_bin(B, tc[x], cc[y]) != b;
+    for (int x = 0; x < nt && !ambiguous; x++) {
+      const UnitVal ux = x == 0 ? u : unit_cert(B.g, tc[x]);
+      double rc[3] = {ux.value, 0.0, 0.0};
+      int nr = 1;
+      if (ux.uncertain) {
+        if (ux.sin == 0.0) {  // outside the certified range: the host decides
+          ambiguous = true;
+          break;
+        }
+        rc[1] = B.g.q_scale * next_up(ux.sin);
+        rc[2] = B.g.q_scale * next_down(ux.sin);
+        nr = 3;
+      }
+      for (int k = 0; k < nr; k++)
+        for (int y = 0; y < nc; y++) ambiguous |= ss_bin(B, rc[k], cc[y]) != b;
+    }
   }
   bins[t] = b < 0 ? 0xffffffffu : (uint32_t)b;
   if (ambiguous) {
@@ -1146,6 +1180,39 @@ void host_patch(const Geo& g, const std::vector<uint32_t>& idx,
   for (auto& th : pool) th.join();
 }
 
+// 2theta values in the plan's radial unit, q values that the certified
+// sine leaves uncertain recomputed with the host's glibc (tth and out are
+// distinct buffers). Returns the number recomputed.
+uint64_t unit_array(const Geo& g, const double* tth, double* out, uint64_t n, hipStream_t s) {
+  if (n == 0) return 0;
+  DevBuf<uint32_t> flist(g.unit >= 2 ? n : 1), fcount(1);
+  DIO_HIP(hipMemsetAsync(fcount.p, 0, sizeof(uint32_t), s));
+  k_to_unit<<<blocks_for(n, kThreads), kThreads, 0, s>>>(g, tth, out, n, flist.p, fcount.p);
+  DIO_HIP(hipGetLastError());
+  const uint32_t nf = read1(fcount.p, s);
+  if (nf == 0) return 0;
+  DevBuf<double> dv(nf);
+  k_gather_f64<<<blocks_for(nf, kThreads), kThreads, 0, s>>>(tth, flist.p, nf, dv.p);
+  std::vector<uint32_t> idx(nf);
+  std::vector<double> v(nf);
+  DIO_HIP(hipMemcpyAsync(idx.data(), flist.p, nf * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipMemcpyAsync(v.data(), dv.p, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipStreamSynchronize(s));
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (nf < 4096) nt = 1;
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; t++)
+    pool.emplace_back([&, t]() {
+      for (size_t i = t; i < nf; i += nt) v[i] = unit_host(g, v[i]);
+    });
+  for (auto& th : pool) th.join();
+  DIO_HIP(hipMemcpyAsync(dv.p, v.data(), nf * sizeof(double), hipMemcpyHostToDevice, s));
+  k_scatter_f64<<<blocks_for(nf, kThreads), kThreads, 0, s>>>(flist.p, dv.p, nf, out);
+  DIO_HIP(hipGetLastError());
+  DIO_HIP(hipStreamSynchronize(s));
+  return nf;
+}
+
 // Angles of a rows x cols point grid at d = index + off, certified and
 // patched. Returns the number of patched points.
 uint64_t grid_angles(const Geo& g, uint32_t rows, uint32_t cols, double off, double* tth,
@@ -1319,13 +1386,21 @@ void singular_prepare(const Geo& g, const SampleGrid& G, const Grid2* G2, Singul
     DIO_HIP(hipGetLastError());
     S.patched = nf;
   }
+  // Sample radial values in the plan's unit.
+  const double* srad = st.p;
+  DevBuf<double> sr;
+  if (g.unit != 0) {
+    sr.alloc(total);
+    S.patched += unit_array(g, st.p, sr.p, total, s);
+    srad = sr.p;
+  }
   S.hits.alloc((uint64_t)ns * nbins);
   DIO_HIP(hipMemsetAsync(S.hits.p, 0, S.hits.bytes(), s));
   if (G2)
-    k_sing_bin2d<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, *G2, ns, st.p, sc.p,
+    k_sing_bin2d<<<blocks_for(total, kThreads), kThreads, 0, s>>>(*G2, ns, srad, sc.p,
                                                                  S.hits.p);
   else
-    k_sing_bin<<<blocks_for(total, kThreads), kThreads, 0, s>>>(g, G, ns, st.p, sc.p, S.hits.p);
+    k_sing_bin<<<blocks_for(total, kThreads), kThreads, 0, s>>>(G, ns, srad, sc.p, S.hits.p);
   k_sing_count<<<ns, kThreads, 0, s>>>(S.hits.p, nbins, ns, S.dpix.p, cnt);
   DIO_HIP(hipGetLastError());
   DIO_HIP(hipStreamSynchronize(s));
@@ -1341,7 +1416,7 @@ void build_1d(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, 
   const double* rad = ttp.p;
   if (g.unit != 0) {
     radp.alloc(npix);
-    k_to_unit<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(g, ttp.p, radp.p, npix);
+    out.uncertain += unit_array(g, ttp.p, radp.p, npix, s);
     rad = radp.p;
   }
   DevBuf<uint32_t> cnt(npix + 1), off(npix + 1);
@@ -1382,7 +1457,7 @@ void build_1d(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, 
     const double* rc = tc.p;
     if (g.unit != 0) {
       rcb.alloc(nc);
-      k_to_unit<<<blocks_for(nc, kThreads), kThreads, 0, s>>>(g, tc.p, rcb.p, nc);
+      out.uncertain += unit_array(g, tc.p, rcb.p, nc, s);
       rc = rcb.p;
     }
     double min, max;
@@ -1513,7 +1588,7 @@ void ss_ranges(const Geo& g, uint32_t nss, const double* rad, const double* chi,
   const double* hrad = ht.p;
   if (g.unit != 0) {
     hr.alloc(h1.size());
-    k_to_unit<<<blocks_for(h1.size(), kThreads), kThreads, 0, s>>>(g, ht.p, hr.p, h1.size());
+    patched += unit_array(g, ht.p, hr.p, h1.size(), s);
     hrad = hr.p;
   }
   double a, b, c, d;
@@ -1552,7 +1627,7 @@ uint64_t ss_entries(const SsBin& B, uint64_t npix, hipStream_t s, Scratch& sc,
       double d1, d2;
       ss_point(B, idx[i], d1, d2);
       AngleArgs a = angle_args(B.g, d1, d2);
-      long long b = ss_bin(B, ::atan2(a.ty, a.tx), ::atan2(a.cy, a.cx));
+      long long b = ss_bin(B, unit_host(B.g, ::atan2(a.ty, a.tx)), ::atan2(a.cy, a.cx));
       val[i] = b < 0 ? 0xffffffffu : (uint32_t)b;
     }
     DevBuf<uint32_t> di(nf), dv(nf);
@@ -1620,7 +1695,7 @@ void build_2d(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, 
   const double* rad = ttp.p;
   if (g.unit != 0) {
     radp.alloc(npix);
-    k_to_unit<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(g, ttp.p, radp.p, npix);
+    out.uncertain += unit_array(g, ttp.p, radp.p, npix, s);
     rad = radp.p;
   }
   bool ps = d.method == DIO_PIXEL_SPLIT;
@@ -1636,7 +1711,7 @@ void build_2d(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, 
     rc = tc.p;
     if (g.unit != 0) {
       rcb.alloc(nc);
-      k_to_unit<<<blocks_for(nc, kThreads), kThreads, 0, s>>>(g, tc.p, rcb.p, nc);
+      out.uncertain += unit_array(g, tc.p, rcb.p, nc, s);
       rc = rcb.p;
     }
   }
@@ -1740,7 +1815,7 @@ void build_ss(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& sc, 
   const double* rad = ttp.p;
   if (g.unit != 0) {
     radp.alloc(npix);
-    k_to_unit<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(g, ttp.p, radp.p, npix);
+    out.uncertain += unit_array(g, ttp.p, radp.p, npix, s);
     rad = radp.p;
   }
   SsBin B{};
@@ -1830,8 +1905,8 @@ void build_pyfai(const Geo& g, const dio_plan_desc& d, hipStream_t s, Scratch& s
   if (g.unit != 0) {
     radp.alloc(npix);
     rcb.alloc(nc);
-    k_to_unit<<<blocks_for(npix, kThreads), kThreads, 0, s>>>(g, ttp.p, radp.p, npix);
-    k_to_unit<<<blocks_for(nc, kThreads), kThreads, 0, s>>>(g, tc.p, rcb.p, nc);
+    out.uncertain += unit_array(g, ttp.p, radp.p, npix, s);
+    out.uncertain += unit_array(g, tc.p, rcb.p, nc, s);
     rad = radp.p;
     rc = rcb.p;
   }
@@ -1951,7 +2026,10 @@ Geo make_geo(const dio_poni& p, const dio_plan_desc& d) {
   g.c3 = ::cos(p.rot3);
   g.wavelength = p.wavelength;
   g.unit = d.unit;
+  // integration.rs:154-169 (RadialConverter::new): 4 pi / (lambda * scale).
   g.q_scale = 0.0;
+  if (d.unit == DIO_UNIT_Q_A || d.unit == DIO_UNIT_Q_NM)
+    g.q_scale = 4.0 * kPiD / (p.wavelength * (d.unit == DIO_UNIT_Q_A ? 1e10 : 1e9));
   g.dim1 = d.dim1;
   g.dim2 = d.dim2;
   return g;

@@ -7,6 +7,7 @@
 #include <cmath>
 
 #include "atan2_cert.hpp"
+#include "sin_cert.hpp"
 
 namespace dio {
 
@@ -73,11 +74,28 @@ __host__ __device__ inline Angles angles_cert(const Geo& g, double d1, double d2
   return {t.value, c.value, (t.uncertain ? 1 : 0) | (c.uncertain ? 2 : 0)};
 }
 
-// integration.rs:171-180 (RadialConverter::convert) for the angle units;
-// q units go through sin and are converted on the host (see the builder).
-__host__ __device__ inline double tth_to_unit(const Geo& g, double tth) {
+// integration.rs:171-180 (RadialConverter::convert): 2theta in the radial
+// unit: radians as they are, degrees as f64::to_degrees, q as
+// q_scale * sin(2theta * 0.5). For q the device takes the certified sine
+// (sin_cert.hpp): `uncertain` marks values the host recomputes with glibc
+// (unit_host), `sin` is the sine whose neighbouring doubles bound glibc's.
+struct UnitVal {
+  double value;
+  double sin;
+  bool uncertain;
+};
+__host__ __device__ inline UnitVal unit_cert(const Geo& g, double tth) {
+  if (g.unit == 0) return {tth, 0.0, false};
+  if (g.unit == 1) return {tth * (180.0 / kPiD), 0.0, false};
+  const SinResult r = sin_cert(tth * 0.5);
+  return {g.q_scale * r.value, r.value, r.uncertain};
+}
+
+// The same conversion on the host with glibc, the reference's libm.
+inline double unit_host(const Geo& g, double tth) {
+  if (g.unit == 0) return tth;
   if (g.unit == 1) return tth * (180.0 / kPiD);
-  return tth;
+  return g.q_scale * ::sin(tth * 0.5);
 }
 
 // Rust `f64 as isize` / `as usize`: saturating, NaN -> 0.

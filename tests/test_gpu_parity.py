@@ -782,3 +782,96 @@ def test_sector_tables_bit_exact(method, rng):
     else:
         lut, _ = O.build_lut2d(P, *shape, nr, na, method, "2th_deg", None, rng)
     assert_same_table(plan, lut)
+
+
+# ---------------------------------------------------------------- q units
+
+Q_CASES = [
+    # name, poni, shape, method, unit, nr, na, radial_range
+    ("ps_qA_tilted", dict(pixel1=1e-4, pixel2=1.2e-4, distance=0.05, poni1=5e-4, poni2=7e-4,
+                          rot1=0.02, rot2=-0.01, wavelength=1e-10), (23, 29), "pixel_split",
+     "q_A^-1", 40, None, None),
+    ("ps_qnm_beam", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                         poni2=36 * 172e-6, wavelength=0.7e-10), (64, 80), "pixel_split",
+     "q_nm^-1", 50, None, None),
+    ("counting_qA", dict(pixel1=1.7e-4, pixel2=2.1e-4, distance=0.14, poni1=-0.003,
+                         poni2=-0.002, rot1=0.007, rot2=-0.009, wavelength=1e-10), (23, 29),
+     "counting", "q_A^-1", 30, None, None),
+    ("ss_qnm", dict(pixel1=1e-4, pixel2=1.2e-4, distance=0.05, poni1=5e-4, poni2=7e-4,
+                    rot1=0.02, rot2=-0.01, wavelength=1e-10), (11, 13), "supersampled",
+     "q_nm^-1", 12, None, None),
+    ("pf_qA_beam", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                        poni2=36 * 172e-6, rot1=0.02, rot2=-0.01, rot3=0.05, wavelength=1e-10),
+     (64, 80), "pyfai_full", "q_A^-1", 40, None, None),
+    ("ps2d_qA_beam", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                          poni2=36 * 172e-6, wavelength=1e-10), (64, 80), "pixel_split",
+     "q_A^-1", 32, 36, None),
+    ("counting2d_qnm", dict(pixel1=1e-4, pixel2=1e-4, distance=0.05, poni1=4.5e-4,
+                            poni2=5.5e-4, wavelength=1e-10), (9, 11), "counting", "q_nm^-1",
+     10, 8, None),
+    ("ss2d_qA", dict(pixel1=1e-4, pixel2=1.2e-4, distance=0.05, poni1=5e-4, poni2=7e-4,
+                     rot1=0.02, rot2=-0.01, wavelength=1e-10), (11, 13), "supersampled",
+     "q_A^-1", 12, 16, None),
+    ("pf2d_qnm", dict(pixel1=172e-6, pixel2=172e-6, distance=0.05, poni1=24 * 172e-6,
+                      poni2=36 * 172e-6, rot1=0.02, rot2=-0.01, rot3=0.05, wavelength=1e-10),
+     (64, 80), "pyfai_full", "q_nm^-1", 32, 36, None),
+    ("ps_qA_range", dict(pixel1=1e-4, pixel2=1e-4, distance=0.05, poni1=4.5e-4, poni2=5.5e-4,
+                         wavelength=1e-10), (9, 11), "pixel_split", "q_A^-1", 20, None,
+     (0.05, 1.5)),
+]
+
+
+def oracle_q_table(poni, shape, method, unit, nr, na, rr):
+    P = O.Poni(**poni)
+    if na is None:
+        if method == "supersampled":
+            return O.build_lut1d_ss(P, *shape, nr, 3, unit, rr)
+        if method == "pyfai_full":
+            return O.build_lut1d_pyfai(P, *shape, nr, unit, rr)
+        return O.build_lut1d(P, *shape, nr, method, unit, rr)
+    if method == "supersampled":
+        return O.build_lut2d_ss(P, *shape, nr, na, 3, unit, rr)[0]
+    if method == "pyfai_full":
+        return O.build_lut2d_pyfai(P, *shape, nr, na, unit, rr)[0]
+    return O.build_lut2d(P, *shape, nr, na, method, unit, rr)[0]
+
+
+@pytest.mark.parametrize("g", Q_CASES, ids=[g[0] for g in Q_CASES])
+def test_q_unit_tables_bit_exact(g):
+    """Tables in q (RadialConverter::convert, integration.rs:171-180: q =
+    4 pi / (lambda 1e10 or 1e9) sin(2theta / 2)), every method, 1D and 2D:
+    the device's certified sine, with glibc on the host where it is
+    uncertain, gives the oracle's tables bit for bit."""
+    name, poni, shape, method, unit, nr, na, rr = g
+    print(f"case {name}", flush=True)
+    plan = GpuPlan(poni, shape, method=method, n_ss=3 if method == "supersampled" else None,
+                   unit=unit, n_radial=nr, n_azimuthal=na, radial_range=rr)
+    lut = oracle_q_table(poni, shape, method, unit, nr, na, rr)
+    assert_same_table(plan, lut)
+    if na is None:
+        assert np.array_equal(plan.radial, lut.bins())
+
+
+def test_q_unit_full_size_table_and_gather():
+    """The batch1d detector (3269 x 3110) at 2000 bins in q (1/nm): the
+    table bit-exact and a frame's intensity within the gather tolerance."""
+    w = synth.WORKLOADS["eiger9m_batch1d"]
+    poni = synth.workload_poni("eiger9m_batch1d")
+    print("case eiger9m_batch1d q_nm^-1", flush=True)
+    plan = GpuPlan(poni, w["shape"], n_radial=w["n_radial"], unit="q_nm^-1",
+                   precision="f32_wide")
+    P = O.Poni(**poni)
+    lut = O.build_lut1d(P, *w["shape"], w["n_radial"], "pixel_split", "q_nm^-1")
+    assert_same_table(plan, lut)
+    frame = synth.frames_device("eiger9m_batch1d", 1, seed=3).numpy()[0]
+    I, _, _ = plan.integrate_stack(frame[None])
+    den = O.denominators(lut, O.normalization(P, *w["shape"], None))
+    i, _ = O.gather(lut, den, frame.astype(np.float64), False)
+    assert np.all(np.abs(I[0] - i) <= 2e-6 * np.abs(i))
+
+
+def test_q_units_need_a_wavelength():
+    poni = dict(pixel1=1e-4, pixel2=1e-4, distance=0.05, poni1=4.5e-4, poni2=5.5e-4,
+                wavelength=0.0)
+    with pytest.raises(ValueError, match="wavelength"):
+        GpuPlan(poni, (9, 11), unit="q_A^-1", n_radial=10)
