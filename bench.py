@@ -140,8 +140,10 @@ def serve_bench(rates, n_requests, seed=7):
     """Continuous-batching server over the serving mix: Poisson arrivals,
     the three detector shapes, 1-64 device-resident uint32 frames per
     request, 60 % 1D (1000-3000 bins) / 40 % 2D (500 x 360), pixel
-    splitting. Tables are built once (warm cache) before the timed load; the
-    same requests are also run through the plain per-frame path."""
+    splitting. Tables are built once (warm cache) before the timed loads; an
+    infinite rate submits every request at once (the saturating load); a
+    cold-start load on a fresh server builds its plans during the load. The
+    first rate's requests are also run through the plain per-frame path."""
     from dioptrin_amd import GpuPlan, synth
     from dioptrin_amd.device import synchronize
     from dioptrin_amd.serve import LoadMix, Server, make_requests, run_load
@@ -190,7 +192,7 @@ def serve_bench(rates, n_requests, seed=7):
             res = run_load(srv, reqs, rate=rate, seed=seed)
             frames = res["frames_total"]
             out["rates"].append({
-                "offered_req_s": rate,
+                "offered_req_s": rate if np.isfinite(rate) else "saturating: all at once",
                 "achieved_req_s": round(len(reqs) / res["wall_s"], 1),
                 "frames": frames,
                 "frames_per_s": round(res["frames_per_s_wall"], 1),
@@ -199,6 +201,21 @@ def serve_bench(rates, n_requests, seed=7):
                 "peak_inflight": res["peak_inflight"],
                 "plan_builds_during_load": res["plan_builds"] - base["plan_builds"]})
             log(f"[bench] serve {rate} req/s: {out['rates'][-1]}")
+        st = srv.stats()
+        out["device_bytes"] = {"plans": st["plan_bytes"], "slot_scratch": st["scratch_bytes"],
+                               "peak_total": st["peak_bytes"],
+                               "plans_resident": st["plans_resident"]}
+    # Cold start: a fresh server (empty plan cache) under the first rate's
+    # load; plans are built on the server's builder thread while requests of
+    # already-built plans keep running.
+    with Server(n_slots=4, plan_cache_entries=64, hbm_budget_bytes=32 << 30) as srv:
+        res = run_load(srv, all_reqs[0], rate=rates[0], seed=seed)
+        out["cold_start"] = {
+            "offered_req_s": rates[0], "plan_builds_during_load": res["plan_builds"],
+            "frames_per_s": round(res["frames_per_s_wall"], 1),
+            "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+            "max_ms": round(res["max_ms"], 3)}
+        log(f"[bench] serve cold start: {out['cold_start']}")
     # The same requests of the first rate through the plain per-frame path:
     # one plan per request shape (built beforehand), one call and one
     # synchronisation per frame.
@@ -370,7 +387,7 @@ def main():
 
     serve = None
     if not args.no_serve:
-        serve = serve_bench([50.0, 200.0, 400.0], args.serve_requests)
+        serve = serve_bench([50.0, 200.0, 400.0, float("inf")], args.serve_requests)
 
     bytes_frame = algorithmic_bytes_per_frame(npix, info["nnz_masked"], bins, fl,
                                               outputs=1 + int(args.errors))

@@ -93,7 +93,9 @@ class ServerStats(C.Structure):
         ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64), ("plan_builds", C.c_uint64),
         ("cache_hits", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
         ("mean_ms", C.c_double), ("max_ms", C.c_double), ("span_s", C.c_double),
-        ("frames_per_s", C.c_double)]
+        ("frames_per_s", C.c_double), ("plan_bytes", C.c_uint64),
+        ("scratch_bytes", C.c_uint64), ("request_bytes", C.c_uint64),
+        ("plans_resident", C.c_uint32), ("builds_pending", C.c_uint32)]
 
 
 _lib = None

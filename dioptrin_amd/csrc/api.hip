@@ -108,6 +108,16 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
 }  // namespace dio
 
 namespace dio {
+size_t plan_device_bytes(const dio_plan* p) {
+  auto tb = [](const dio::Table& t) {
+    return t.row_ptr.bytes() + t.col.bytes() + t.w64.bytes() + t.w32.bytes();
+  };
+  return tb(p->b.full) + (p->b.has_exec ? tb(p->b.exec) : 0) + p->b.norm.bytes() +
+         p->b.den.bytes() + p->b.mask.bytes() + p->tt.bytes() + p->sc.part.bytes() +
+         p->stage.bytes() + p->mstage.bytes() + p->dyn_x.bytes() + p->dyn_n.bytes() +
+         p->dyn_d.bytes() + p->out_i.bytes() + p->out_s.bytes() + p->out_d.bytes();
+}
+
 dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& desc) {
   check_desc(desc);
   // RadialConverter::new (integration.rs:154-169) asserts a positive
@@ -115,53 +125,53 @@ dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& des
   if (desc.unit == DIO_UNIT_Q_A || desc.unit == DIO_UNIT_Q_NM)
     require(poni.wavelength > 0.0, DIO_ERR_INVALID,
             "wavelength must be positive for q units");
-    auto t0 = std::chrono::steady_clock::now();
-    DIO_HIP(hipSetDevice(ctx->device));
-    auto p = std::make_unique<dio_plan>();
-    p->ctx = ctx;
-    p->device = ctx->device;
-    p->desc = desc;
-    p->desc.mask = nullptr;
-    DIO_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-    p->geo = dio::make_geo(poni, desc);
-    dio::build_tables(p->geo, desc, p->stream, p->b);
-    const dio::Table& t = p->b.exec_table();
-    p->gt.col = t.col.p;
-    p->gt.w32 = t.w32.p;
-    p->gt.w64 = t.w64.p;
-    p->gt.den = p->b.den.p;
-    p->gt.nbins = t.nbins;
-    p->gt.npix = (uint64_t)desc.dim1 * desc.dim2;
-    p->gt.nnz = t.nnz;
-    dio::build_tiled(p->tt, desc.dim1, desc.dim2, t.nbins, t.nnz, t.row_ptr.p, t.col.p,
-                     t.w32.p, t.w64.p, desc.precision == DIO_F64, p->stream);
-    p->tt.den = p->b.den.p;
-    p->tt.flip = desc.input_order == DIO_ORDER_FLIP_VERTICAL;
-    p->tt.cus = (uint32_t)std::max(1, ctx->prop.multiProcessorCount);
-    // DIOPTRIN_TILE_WORKGROUPS caps the gather's persistent grid (tuning, and
-    // tests of the many-tiles-per-workgroup path on small detectors).
-    if (const char* e = std::getenv("DIOPTRIN_TILE_WORKGROUPS")) {
-      const long v = std::atol(e);
-      if (v > 0) p->tt.max_wgs = (uint32_t)std::min<long>(v, 1L << 20);
-    }
-    // DIOPTRIN_TILE_DB=0 selects the single-buffered kernel for 16-frame passes
-    // (comparison runs; results are the same bits either way).
-    if (const char* e = std::getenv("DIOPTRIN_TILE_DB")) p->tt.db = std::atol(e) != 0;
-    if (desc.precision == DIO_F64) p->tile = std::min(p->tile, dio::max_frames_for(DIO_F64));
-    p->st.row_ptr = p->b.full.row_ptr.p;
-    p->st.col = p->b.full.col.p;
-    p->st.w32 = p->b.full.w32.p;
-    p->st.w64 = p->b.full.w64.p;
-    p->st.weighted = p->b.weighted ? 1 : 0;
-    p->st.mask = p->b.mask.p;
-    p->st.norm = p->b.norm.p;
-    p->st.nbins = p->b.full.nbins;
-    p->st.npix = p->gt.npix;
-    p->st.dim1 = desc.dim1;
-    p->st.dim2 = desc.dim2;
-    p->st.flip = desc.input_order == DIO_ORDER_FLIP_VERTICAL;
-    p->build_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  auto t0 = std::chrono::steady_clock::now();
+  DIO_HIP(hipSetDevice(ctx->device));
+  auto p = std::make_unique<dio_plan>();
+  p->ctx = ctx;
+  p->device = ctx->device;
+  p->desc = desc;
+  p->desc.mask = nullptr;
+  DIO_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  p->geo = dio::make_geo(poni, desc);
+  dio::build_tables(p->geo, desc, p->stream, p->b);
+  const dio::Table& t = p->b.exec_table();
+  p->gt.col = t.col.p;
+  p->gt.w32 = t.w32.p;
+  p->gt.w64 = t.w64.p;
+  p->gt.den = p->b.den.p;
+  p->gt.nbins = t.nbins;
+  p->gt.npix = (uint64_t)desc.dim1 * desc.dim2;
+  p->gt.nnz = t.nnz;
+  dio::build_tiled(p->tt, desc.dim1, desc.dim2, t.nbins, t.nnz, t.row_ptr.p, t.col.p,
+                   t.w32.p, t.w64.p, desc.precision == DIO_F64, p->stream);
+  p->tt.den = p->b.den.p;
+  p->tt.flip = desc.input_order == DIO_ORDER_FLIP_VERTICAL;
+  p->tt.cus = (uint32_t)std::max(1, ctx->prop.multiProcessorCount);
+  // DIOPTRIN_TILE_WORKGROUPS caps the gather's persistent grid (tuning, and
+  // tests of the many-tiles-per-workgroup path on small detectors).
+  if (const char* e = std::getenv("DIOPTRIN_TILE_WORKGROUPS")) {
+    const long v = std::atol(e);
+    if (v > 0) p->tt.max_wgs = (uint32_t)std::min<long>(v, 1L << 20);
+  }
+  // DIOPTRIN_TILE_DB=0 selects the single-buffered kernel for 16-frame passes
+  // (comparison runs; results are the same bits either way).
+  if (const char* e = std::getenv("DIOPTRIN_TILE_DB")) p->tt.db = std::atol(e) != 0;
+  if (desc.precision == DIO_F64) p->tile = std::min(p->tile, dio::max_frames_for(DIO_F64));
+  p->st.row_ptr = p->b.full.row_ptr.p;
+  p->st.col = p->b.full.col.p;
+  p->st.w32 = p->b.full.w32.p;
+  p->st.w64 = p->b.full.w64.p;
+  p->st.weighted = p->b.weighted ? 1 : 0;
+  p->st.mask = p->b.mask.p;
+  p->st.norm = p->b.norm.p;
+  p->st.nbins = p->b.full.nbins;
+  p->st.npix = p->gt.npix;
+  p->st.dim1 = desc.dim1;
+  p->st.dim2 = desc.dim2;
+  p->st.flip = desc.input_order == DIO_ORDER_FLIP_VERTICAL;
+  p->build_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return p.release();
 }
 }  // namespace dio
@@ -258,12 +268,7 @@ int dio_plan_info_get(dio_plan* p, dio_plan_info* out) {
     out->build_ms = p->build_ms;
     out->hist_min = p->b.hist_min;
     out->hist_step = p->b.hist_step;
-    auto tb = [](const dio::Table& t) {
-      return t.row_ptr.bytes() + t.col.bytes() + t.w64.bytes() + t.w32.bytes();
-    };
-    out->device_bytes = tb(p->b.full) + (p->b.has_exec ? tb(p->b.exec) : 0) +
-                        p->b.norm.bytes() + p->b.den.bytes() + p->b.mask.bytes() +
-                        p->tt.bytes() + p->sc.part.bytes();
+    out->device_bytes = dio::plan_device_bytes(p);
   });
 }
 

@@ -62,6 +62,9 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype, uin
 void noalloc_init(int device);
 void* noalloc_scratch(int device, size_t* bytes);
 size_t noalloc_capacity();
+// Device bytes a plan holds (tables, corrections, tile layout, scratch and
+// staging buffers it keeps).
+size_t plan_device_bytes(const dio_plan* p);
 // Plan construction behind dio_plan_create (validates, builds on the device).
 dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& desc);
 }  // namespace dio

@@ -257,29 +257,38 @@ int dio_synthetic_poisson_u32(dio_ctx *ctx, const float *d_rate, uint64_t npix,
  * reference's one-entry table caches, src/integration/cache.rs:42-96, to an
  * LRU of device-resident plans, and its one-stream-per-plan execution,
  * src/integration/cuda/device.rs:1354-1584, to a pool of stream slots).
- * Requests are admitted in arrival order when a slot is free and their
- * device footprint (frames + outputs) fits hbm_budget_bytes; a request larger
- * than the budget runs alone. Zeroed fields take the defaults shown. */
+ * Plans missing from the cache are built on a builder thread; a request
+ * waits only for its own plan. Requests whose plan is ready are admitted in
+ * arrival order when a slot is free and the server's device bytes (resident
+ * plans + slot scratch + frames and outputs of requests in flight) fit
+ * hbm_budget_bytes, evicting idle plans for room; a request that does not
+ * fit with nothing in flight runs alone. Zeroed fields take the defaults
+ * shown. */
 typedef struct {
   uint32_t n_slots;            /* concurrent requests (streams), default 4, max 64 */
   uint32_t plan_cache_entries; /* device-resident plans kept, default 32 */
-  uint64_t hbm_budget_bytes;   /* request frames + outputs in flight, default 32 GiB */
+  uint64_t hbm_budget_bytes;   /* plans + scratch + requests in flight, default 32 GiB */
 } dio_server_config;
 
 typedef struct {
   uint64_t completed, failed, frames, queued;
   uint32_t inflight, peak_inflight;
-  uint64_t bytes_in_use, peak_bytes; /* admitted request footprints */
+  uint64_t bytes_in_use, peak_bytes; /* plan + scratch + request bytes, now and at peak */
   uint64_t plan_builds, cache_hits;
   double p50_ms, p99_ms, mean_ms, max_ms; /* submit-to-result latency */
   double span_s, frames_per_s;             /* first submit to last completion */
+  uint64_t plan_bytes;    /* resident plans */
+  uint64_t scratch_bytes; /* gather partials of the slots */
+  uint64_t request_bytes; /* frames + outputs of requests in flight */
+  uint32_t plans_resident, builds_pending;
 } dio_server_stats;
 
 int dio_server_create(dio_ctx *ctx, const dio_server_config *cfg, dio_server **out);
 /* Queues one request; frames (n_frames x dim1 x dim2 of dtype, in host
  * memory, or device memory with DIO_DEVICE_FRAMES) and the host outputs
- * (n_frames x nbins doubles each) are borrowed until poll reports it done. The plan is identified by (poni, desc, mask contents) like
- * dio_plan_create's arguments (Integrator::integrate1d / integrate2d,
+ * (n_frames x nbins doubles each) are borrowed until poll reports it done.
+ * The plan is identified by every field of poni and desc that shapes it and
+ * the full mask contents, like dio_plan_create's arguments (Integrator::integrate1d / integrate2d,
  * src/integration/methods.rs:94-100, 461). */
 int dio_server_submit(dio_server *srv, const dio_poni *poni, const dio_plan_desc *desc,
                       const void *frames, int dtype, uint64_t n_frames, uint32_t flags,

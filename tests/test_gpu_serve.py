@@ -58,7 +58,7 @@ def mixed_requests(seed=1, n=24):
 def alone(req):
     plan = GpuPlan(req.poni, req.frames.shape[1:], method=req.method, n_radial=req.n_radial,
                    n_azimuthal=req.n_azimuthal, polarization_factor=req.polarization_factor,
-                   precision=req.precision)
+                   mask=req.mask, precision=req.precision)
     return plan.integrate_stack(req.frames, errors=req.errors, azimuthal_std=req.azimuthal_std)
 
 
@@ -103,20 +103,113 @@ def test_mix_matches_oracle():
             assert np.all(np.abs(resp.intensity[f] - i) <= 2e-6 * mag + 1e-300)
 
 
-def test_hbm_admission_limits_footprint():
+def footprint(r):
+    return r.frames.nbytes + r.frames.shape[0] * r.n_radial * (r.n_azimuthal or 1) * 8 * \
+        (1 + r.errors + r.azimuthal_std)
+
+
+def test_hbm_admission_counts_plans_and_scratch():
     reqs = mixed_requests(seed=2, n=12)
-    # Budget below two typical requests: admission must serialise most of them.
-    foot = [r.frames.nbytes + r.frames.shape[0] * r.n_radial * (r.n_azimuthal or 1) * 8 *
-            (1 + r.errors + r.azimuthal_std) for r in reqs]
-    budget = int(max(foot) * 1.2)
+    # Resident plans and slot scratch of this mix, on a server with room for
+    # everything.
+    with Server(n_slots=4) as srv:
+        for t in [srv.submit(r) for r in reqs]:
+            srv.result(t)
+        st0 = srv.stats()
+    assert st0["request_bytes"] == 0 and st0["builds_pending"] == 0
+    assert st0["bytes_in_use"] == st0["plan_bytes"] + st0["scratch_bytes"]
+    assert st0["plans_resident"] == st0["plan_builds"] > 0 and st0["scratch_bytes"] > 0
+    # Room for every plan, for the slots' scratch (a slot's scratch stays
+    # below twice the largest need, and st0 holds at least that need once)
+    # and for 1.2 of the largest request: admission must serialise most
+    # requests.
+    budget = st0["plan_bytes"] + 8 * st0["scratch_bytes"] + int(max(map(footprint, reqs)) * 1.2)
     with Server(n_slots=4, hbm_budget_bytes=budget) as srv:
+        # Plans first (one frame each, one at a time), so the load below
+        # runs against resident plans.
+        for r in reqs:
+            srv.result(srv.submit(Request(**{**r.__dict__, "frames": r.frames[:1]})))
+        assert srv.stats()["plan_bytes"] == st0["plan_bytes"]
         resps = [srv.result(t) for t in [srv.submit(r) for r in reqs]]
         st = srv.stats()
-    assert st["completed"] == len(reqs)
+    assert st["completed"] == 2 * len(reqs) and st["failed"] == 0
     assert st["peak_bytes"] <= budget
-    assert st["bytes_in_use"] == 0 and st["inflight"] == 0
+    assert st["request_bytes"] == 0 and st["inflight"] == 0
+    assert st["bytes_in_use"] == st["plan_bytes"] + st["scratch_bytes"]
     for r, resp in zip(reqs[:4], resps[:4]):
         assert same(resp.intensity, alone(r)[0])
+
+
+def test_idle_plans_evicted_for_room():
+    # A detector large enough that its plans outweigh a request.
+    shape = (512, 512)
+    poni = dict(pixel1=172e-6, pixel2=172e-6, distance=0.1, poni1=250 * 172e-6,
+                poni2=260 * 172e-6, wavelength=1e-10)
+    fr = np.random.default_rng(3).poisson(50.0, size=(1, *shape)).astype(np.uint32)
+    bins = [30, 31, 32, 30]
+    # The slot scratch this sequence leaves, on a server with room for all.
+    with Server(n_slots=2) as srv:
+        for b in bins:
+            srv.result(srv.submit(Request(poni, fr, n_radial=b)))
+        scratch = srv.stats()["scratch_bytes"]
+    sizes = [GpuPlan(poni, shape, n_radial=b).info["device_bytes"] for b in (30, 31, 32)]
+    # Room for the largest plan, that scratch and one request, never for two
+    # plans.
+    budget = max(sizes) * 21 // 20 + scratch + footprint(Request(poni, fr, n_radial=32))
+    print(f"plans {sizes} B, scratch {scratch} B, budget {budget} B", flush=True)
+    assert budget < 2 * min(sizes) + scratch
+    with Server(n_slots=2, plan_cache_entries=8, hbm_budget_bytes=budget) as srv:
+        resps = [srv.result(srv.submit(Request(poni, fr, n_radial=b))) for b in bins]
+        st = srv.stats()
+    assert st["plan_builds"] == 4 and st["cache_hits"] == 0
+    assert st["plans_resident"] == 1
+    assert st["bytes_in_use"] <= budget
+    for b, resp in zip(bins, resps):
+        assert same(resp.intensity, alone(Request(poni, fr, n_radial=b))[0])
+
+
+def test_plan_key_compares_full_mask():
+    poni, shape = GEOS["a"]
+    fr = np.random.default_rng(11).poisson(80.0, size=(2, *shape)).astype(np.uint32)
+    m1 = np.zeros(shape, np.uint8)
+    m2 = m1.copy()
+    m2[30, 40] = 1  # one pixel apart
+    m3 = m1.copy()  # same contents, another array
+    with Server(n_slots=2) as srv:
+        r1 = srv.result(srv.submit(Request(poni, fr, n_radial=40, mask=m1)))
+        r2 = srv.result(srv.submit(Request(poni, fr, n_radial=40, mask=m2)))
+        r3 = srv.result(srv.submit(Request(poni, fr, n_radial=40, mask=m3)))
+        st = srv.stats()
+    assert st["plan_builds"] == 2 and st["cache_hits"] == 1
+    assert same(r1.intensity, r3.intensity)
+    assert not same(r1.intensity, r2.intensity)
+    assert same(r1.intensity, alone(Request(poni, fr, n_radial=40, mask=m1))[0])
+    assert same(r2.intensity, alone(Request(poni, fr, n_radial=40, mask=m2))[0])
+
+
+def test_plan_key_ignores_switched_off_fields(monkeypatch):
+    import dioptrin_amd.serve as S
+    real = S.plan_args
+
+    def leftovers(*a, **k):
+        p, d, m = real(*a, **k)
+        # Values in fields that the descriptor's own settings switch off.
+        d.radial_min, d.radial_max = 7.0, -3.0      # has_radial_range = 0
+        d.azimuthal_min, d.azimuthal_max = 1.0, 2.0  # 1D plan
+        d.n_azimuthal = 99                           # 1D plan
+        d.polarization_factor = 0.25                 # has_polarization = 0
+        d.n_ss = 9                                   # not supersampled
+        return p, d, m
+
+    poni, shape = GEOS["b"]
+    fr = np.random.default_rng(12).poisson(80.0, size=(2, *shape)).astype(np.uint32)
+    with Server(n_slots=2) as srv:
+        a = srv.result(srv.submit(Request(poni, fr, n_radial=40)))
+        monkeypatch.setattr(S, "plan_args", leftovers)
+        b = srv.result(srv.submit(Request(poni, fr, n_radial=40)))
+        st = srv.stats()
+    assert st["plan_builds"] == 1 and st["cache_hits"] == 1
+    assert same(a.intensity, b.intensity)
 
 
 def test_plan_cache_reuse_and_eviction():
@@ -142,7 +235,8 @@ def test_bad_request_fails_alone():
     fr = np.random.default_rng(4).poisson(50.0, size=(1, *shape)).astype(np.uint32)
     with Server(n_slots=2) as srv:
         good1 = srv.submit(Request(poni, fr, n_radial=40))
-        bad = srv.submit(Request(poni, fr, n_radial=40, unit="q_nm^-1"))
+        # q units need a positive wavelength (integration.rs:154-169).
+        bad = srv.submit(Request(dict(poni, wavelength=0.0), fr, n_radial=40, unit="q_nm^-1"))
         std2d = srv.submit(Request(poni, fr, n_radial=20, n_azimuthal=8, azimuthal_std=True))
         good2 = srv.submit(Request(poni, fr, n_radial=41))
         assert srv.result(good1).intensity.shape == (1, 40)
