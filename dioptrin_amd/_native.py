@@ -27,7 +27,8 @@ EXPORTS = [
     "dio_plan_counts",
     "dio_integrate", "dio_integrate_device", "dio_integrate_device_noalloc",
     "dio_integrate_masked",
-    "dio_integrate_masked_device", "dio_plan_autotune", "dio_plan_set_tile",
+    "dio_integrate_masked_device", "dio_integrate_variances", "dio_multi_integrate",
+    "dio_plan_autotune", "dio_plan_set_tile",
     "dio_graph_create", "dio_graph_launch", "dio_graph_destroy", "dio_plan_time_gather",
     "dio_plan_time_passes", "dio_device_alloc", "dio_device_free", "dio_memcpy",
     "dio_host_alloc", "dio_host_free", "dio_synchronize",
@@ -98,6 +99,11 @@ class ServerStats(C.Structure):
         ("plans_resident", C.c_uint32), ("builds_pending", C.c_uint32)]
 
 
+class RawSums(C.Structure):
+    _fields_ = [("sum_signal", C.c_void_p), ("sum_norm", C.c_void_p),
+                ("sum_variance", C.c_void_p), ("count", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -131,6 +137,9 @@ def lib() -> C.CDLL:
             "dio_integrate_device_noalloc": (i32, [vp, vp, i32, u64, u32, dp, dp, dp, vp]),
             "dio_integrate_masked": (i32, [vp, vp, i32, u64, vp, u32, dp, dp]),
             "dio_integrate_masked_device": (i32, [vp, vp, i32, u64, vp, u32, dp, dp, vp]),
+            "dio_integrate_variances": (i32, [vp, vp, i32, vp, i32, u64, u32, dp, dp]),
+            "dio_multi_integrate": (i32, [u32, vp, vp, vp, vp, vp, vp, u64, u32, dp, dp,
+                                          C.POINTER(RawSums)]),
             "dio_plan_autotune": (i32, [vp, i32, u64, u32, C.POINTER(u32), C.POINTER(u32)]),
             "dio_plan_set_tile": (i32, [vp, u32, u32]),
             "dio_graph_create": (i32, [vp, vp, i32, u64, u32, dp, dp, dp, C.POINTER(vp)]),

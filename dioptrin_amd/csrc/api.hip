@@ -18,36 +18,9 @@ thread_local std::string g_last_error;
 std::mutex g_ctx_mu;
 std::map<int, std::unique_ptr<dio_ctx>> g_ctx;
 
-template <class Fn>
-int guarded(Fn&& fn) {
-  try {
-    fn();
-    g_last_error.clear();
-    return DIO_OK;
-  } catch (const dio::Error& e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host out of memory";
-    return DIO_ERR_DEVICE;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return DIO_ERR_DEVICE;
-  }
-}
-
-void require(bool ok, int code, const char* msg) {
-  if (!ok) dio::fail(code, msg);
-}
-
-size_t dtype_bytes(int dtype) {
-  switch (dtype) {
-    case DIO_U16: return 2;
-    case DIO_U32: case DIO_I32: case DIO_F32T: return 4;
-    case DIO_F64T: return 8;
-    default: dio::fail(DIO_ERR_INVALID, "unknown frame dtype");
-  }
-}
+using dio::dtype_bytes;
+using dio::guarded;
+using dio::require;
 
 // Configuration checks before any device work (cuda.rs:1001-1057).
 void check_desc(const dio_plan_desc& d) {

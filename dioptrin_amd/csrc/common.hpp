@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -69,6 +70,40 @@ struct DevBuf {
 
 inline unsigned blocks_for(size_t n, unsigned threads) {
   return static_cast<unsigned>((n + threads - 1) / threads);
+}
+
+inline void require(bool ok, int code, const char* msg) {
+  if (!ok) fail(code, msg);
+}
+
+// Bytes per element of a frame dtype (DIO_U16 .. DIO_F64T).
+inline size_t dtype_bytes(int dtype) {
+  switch (dtype) {
+    case DIO_U16: return 2;
+    case DIO_U32: case DIO_I32: case DIO_F32T: return 4;
+    case DIO_F64T: return 8;
+    default: fail(DIO_ERR_INVALID, "unknown frame dtype");
+  }
+}
+
+// Runs fn at the C boundary: exceptions become a status code and the
+// thread-local text of dio_last_error(), which success clears.
+template <class Fn>
+int guarded(Fn&& fn) {
+  try {
+    fn();
+    dio_set_last_error("");
+    return DIO_OK;
+  } catch (const Error& e) {
+    dio_set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    dio_set_last_error("host out of memory");
+    return DIO_ERR_DEVICE;
+  } catch (const std::exception& e) {
+    dio_set_last_error(e.what());
+    return DIO_ERR_DEVICE;
+  }
 }
 
 }  // namespace dio

@@ -1288,16 +1288,6 @@ void launch_final(const TiledTable& t, const void* part, uint32_t F, uint64_t BS
                                                           raw ? 1 : 0, I, S);
 }
 
-size_t dtype_bytes(int dtype) {
-  switch (dtype) {
-    case DIO_U16: return 2;
-    case DIO_U32: case DIO_I32: case DIO_F32T: return 4;
-    case DIO_F64T: return 8;
-    default: fail(DIO_ERR_INVALID, "unknown frame dtype");
-  }
-}
-
-
 }  // namespace
 
 uint32_t max_frames_for(int precision) { return precision == DIO_F64 ? 8 : kMaxFrames; }

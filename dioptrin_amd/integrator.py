@@ -21,6 +21,7 @@ from typing import Optional, Sequence, Union
 import numpy as np
 
 from . import _native as N
+from .grid import IntegrationGrid2d, IntermediateResult1d, IntermediateResult2d
 from .poni import load_poni  # noqa: F401  (part of the package API)
 from .tiff import read_tiff
 
@@ -40,6 +41,7 @@ class IntegrationResult:
     azimuthal: Optional[np.ndarray] = None
     n_radial: Optional[int] = None
     n_azimuthal: Optional[int] = None
+    raw: Optional[dict] = None  # merged sums before division (MultiIntegrator, raw=True)
 
 
 @dataclass
@@ -218,6 +220,31 @@ class GpuPlan:
                 None if var is None else var.ctypes.data, None))
         return {"bins": self.radial, "sum_signal": sig, "sum_norm": self.denominators(),
                 "sum_variance": var, "count": self.counts()}
+
+    def integrate_with_variances(self, stack, variances, raw=False):
+        """Intensity and sigma from supplied pixel variances
+        (dio_integrate_variances; Prepared1d::integrate_with_variances,
+        lut/prepared.rs:737-766): per bin sum w*x / den and
+        sqrt(sum w^2*v) / den, or with raw=True the sums sum w*x and
+        sum w^2*v. variances: one finite, nonnegative plane per frame."""
+        stack = self._stack(stack)
+        var = np.asarray(variances)
+        if var.ndim == 2:
+            var = var[None]
+        var = np.ascontiguousarray(var)
+        if var.shape != stack.shape:
+            raise ValueError(f"variances have shape {var.shape}, frames {stack.shape}")
+        if var.dtype not in _NP_DTYPES:
+            raise TypeError(f"unsupported variance dtype {var.dtype}")
+        n = stack.shape[0]
+        I = np.empty((n, self.nbins))
+        S = np.empty((n, self.nbins))
+        if n:
+            N.check(N.lib().dio_integrate_variances(
+                self._h, stack.ctypes.data, N.DTYPES[_NP_DTYPES[stack.dtype]], var.ctypes.data,
+                N.DTYPES[_NP_DTYPES[var.dtype]], n, N.RAW_SUMS if raw else 0, I.ctypes.data,
+                S.ctypes.data))
+        return I, S
 
     # -- integration
     def _stack(self, images) -> np.ndarray:
@@ -491,19 +518,91 @@ class Integrator:
     def get_unit(self) -> str:
         return self.unit
 
-    def _plan(self, shape, n_radial, n_azimuthal=None, azimuthal_range=None, precision=None):
-        key = (tuple(shape), n_radial, n_azimuthal, azimuthal_range, precision or self.precision)
+    def _plan(self, shape, n_radial, n_azimuthal=None, azimuthal_range=None, precision=None,
+              *, radial_range=None, method=None):
+        rr = radial_range if radial_range is not None else self.radial_range
+        method = method or self.method
+        key = (tuple(shape), n_radial, n_azimuthal, azimuthal_range, precision or self.precision,
+               rr, method)
         plan = self._plans.get(key)
         if plan is None:
             if self.mask is not None and self.mask.shape != tuple(shape):
                 raise ValueError(f"image shape {tuple(shape)} does not match mask shape "
                                  f"{self.mask.shape}")
-            plan = GpuPlan(self.poni, shape, self.method, self.n_ss, self.unit, n_radial,
-                           n_azimuthal, self.radial_range, azimuthal_range,
+            plan = GpuPlan(self.poni, shape, method, self.n_ss, self.unit, n_radial,
+                           n_azimuthal, rr, azimuthal_range,
                            self.polarization_factor, self.mask, precision or self.precision,
                            self.device)
             self._plans = {key: plan}  # single entry, like the reference caches
         return plan
+
+    def grid_plan(self, shape, grid: IntegrationGrid2d, precision=None) -> "GpuPlan":
+        """The plan of a shared grid (Integrator::shared_grid_key,
+        grid.rs:406-428): the grid's edges, clamped index rounding and, for
+        pixel splitting, the grid's splitting convention (pyFAI's full
+        splitting for pyfai_full integrators whatever the grid says,
+        grid.rs:366-371)."""
+        method = self.method
+        if method == "pyfai_full" or (method == "pixel_split" and
+                                      grid.pixel_split_mode == "pyfai_histogram"):
+            method = "pyfai_full"
+        return self._plan(tuple(shape), grid.n_radial, grid.n_azimuthal, grid.azimuthal_range,
+                          precision, radial_range=grid.radial_range, method=method)
+
+    @staticmethod
+    def _image2d(image) -> np.ndarray:
+        a = np.asarray(image)
+        if a.ndim != 2:
+            raise ValueError(f"image must be 2D, got {a.ndim} dimensions")
+        if a.size == 0:
+            raise ValueError("image is empty")
+        return a
+
+    # -- sums before division (multi-detector merging)
+    def integrate1d_raw(self, image, n_points: int) -> IntermediateResult1d:
+        """integrate1d_raw (integration/methods.rs:318-345): sum_signal,
+        sum_norm (the denominators) and count per bin, without dividing."""
+        return self._raw1d(image, n_points, False)
+
+    def integrate1d_raw_with_variance(self, image, n_points: int) -> IntermediateResult1d:
+        """integrate1d_raw_with_variance: adds sum_variance =
+        sum w^2 * max(x, 0) (Poisson)."""
+        return self._raw1d(image, n_points, True)
+
+    def _raw1d(self, image, n_points, variance):
+        if n_points <= 0:
+            raise ValueError("n_points must be positive")
+        image = self._image2d(image)
+        r = self._plan(image.shape, n_points).integrate_raw(image[None], errors=variance)
+        return IntermediateResult1d(r["bins"], r["sum_signal"][0], r["sum_norm"],
+                                    None if r["sum_variance"] is None else r["sum_variance"][0],
+                                    r["count"])
+
+    def integrate2d_raw_on_grid(self, image, grid: IntegrationGrid2d,
+                                errors: bool = False) -> IntermediateResult2d:
+        """integrate2d_raw_on_grid_from_slice (grid.rs:327-341, 358-392): raw
+        sums on explicit shared edges; errors adds the Poisson sum_variance."""
+        image = self._image2d(image)
+        plan = self.grid_plan(image.shape, grid)
+        r = plan.integrate_raw(image[None], errors=errors)
+        return IntermediateResult2d(r["sum_signal"][0], r["sum_norm"],
+                                    None if r["sum_variance"] is None else r["sum_variance"][0],
+                                    r["count"], grid.n_radial, grid.n_azimuthal)
+
+    def integrate2d_raw_on_grid_with_variances(self, image, variance,
+                                               grid: IntegrationGrid2d) -> IntermediateResult2d:
+        """integrate2d_raw_on_grid_with_variances_from_slice (grid.rs:343-356):
+        sum_variance = sum w^2 * v of the supplied (finite, nonnegative)
+        variances, which must match the image."""
+        image = self._image2d(image)
+        variance = np.asarray(variance)
+        if variance.shape != image.shape:
+            raise ValueError("pixel variances must match the image and be finite and "
+                             "nonnegative")
+        plan = self.grid_plan(image.shape, grid)
+        sig, var = plan.integrate_with_variances(image[None], variance[None], raw=True)
+        return IntermediateResult2d(sig[0], plan.denominators(), var[0], plan.counts(),
+                                    grid.n_radial, grid.n_azimuthal)
 
     def prepare_gpu1d(self, shape, n_points: int, *, precision: str = "f32_compensated",
                       device: Optional[int] = None, input_order: str = "detector") -> GpuPlan:

@@ -194,6 +194,49 @@ int dio_integrate_masked_device(dio_plan *plan, const void *d_frames, int dtype,
                                 uint64_t n_frames, const uint8_t *d_masks, uint32_t flags,
                                 double *d_intensity, double *d_sigma, void *stream);
 
+/* Supplied pixel variances (Prepared1d::integrate_with_variances,
+ * src/lut/prepared.rs:737-766; Dynamic1dPlan::integrate_with_variances,
+ * src/integration/dynamic.rs:128-139; on an explicit-grid 2D plan
+ * Integrator::integrate2d_raw_on_grid_with_variances_from_slice,
+ * src/integration/grid.rs:347-392). variances: n_frames planes of the image's
+ * shape, of dtype vdtype, finite and nonnegative (grid.rs:1102-1107; checked,
+ * DIO_ERR_INVALID otherwise). Per bin: sum_signal = sum w*x and
+ * sum_variance = sum w^2*v over the execution table. flags DIO_RAW_SUMS:
+ * intensity and sigma receive those sums; otherwise intensity =
+ * sum_signal / den and sigma = sqrt(max(sum_variance / den^2, 0)), both 0
+ * where den == 0 (normalize_bin, prepared.rs:985-996). Host buffers,
+ * synchronous. */
+int dio_integrate_variances(dio_plan *plan, const void *frames, int dtype,
+                            const void *variances, int vdtype, uint64_t n_frames,
+                            uint32_t flags, double *intensity, double *sigma);
+
+/* Merged sums before division of dio_multi_integrate (the merge totals,
+ * Multi2dResult::raw, src/multi.rs:265-297); any field may be NULL. */
+typedef struct {
+  double *sum_signal;   /* n_frames x nbins */
+  double *sum_norm;     /* nbins: sum over detectors of den x factor */
+  double *sum_variance; /* n_frames x nbins (with errors or variances) */
+  double *count;        /* nbins */
+} dio_raw_sums;
+
+/* MultiIntegrator (src/multi.rs:198-316 for 2D, 346-510 for 1D): one plan per
+ * detector, all on one device and one bin grid (same nbins and ndim; the
+ * caller builds them with the shared radial range and, in 2D, the same
+ * explicit azimuthal range). frames[d]: n_frames frames of detector d (host,
+ * dtypes[d]); variances (may be NULL): per detector n_frames supplied
+ * variance planes (vdtypes[d]), which imply errors; factors (may be NULL =
+ * all 1): finite positive multipliers of each detector's denominators
+ * (monitor x solid-angle scale, multi.rs:244-264). Per frame and bin the raw
+ * sums of every detector are added on the device in detector order from 0:
+ * S = sum sum_signal, N = sum den x factor, V = sum sum_variance; intensity =
+ * S / N, and with DIO_WITH_ERRORS (or variances) sigma = sqrt(max(V / N^2,
+ * 0)) for 1D plans and sqrt(max(V, 0)) / N for 2D plans, all 0 where
+ * N == 0. raw (may be NULL) receives the totals. Synchronous. */
+int dio_multi_integrate(uint32_t n_det, dio_plan *const *plans, const void *const *frames,
+                        const int *dtypes, const void *const *variances, const int *vdtypes,
+                        const double *factors, uint64_t n_frames, uint32_t flags,
+                        double *intensity, double *sigma, dio_raw_sums *raw);
+
 /* Launch shape of the fused batch kernel (no reference counterpart; the
  * reference launches one frame per kernel, src/integration/cuda/device.rs:
  * 1460-1584): frames per pass and frames per lane, one of the pairs (1, 1),
