@@ -28,6 +28,8 @@ EXPORTS = [
     "dio_integrate", "dio_integrate_device", "dio_integrate_device_noalloc",
     "dio_integrate_masked",
     "dio_integrate_masked_device", "dio_integrate_variances", "dio_multi_integrate",
+    "dio_integrate_bslz4", "dio_bslz4_scan", "dio_bslz4_decode_device", "dio_bslz4_compress",
+    "dio_bslz4_bound",
     "dio_plan_autotune", "dio_plan_set_tile",
     "dio_graph_create", "dio_graph_launch", "dio_graph_destroy", "dio_plan_time_gather",
     "dio_plan_time_passes", "dio_device_alloc", "dio_device_free", "dio_memcpy",
@@ -99,6 +101,11 @@ class ServerStats(C.Structure):
         ("plans_resident", C.c_uint32), ("builds_pending", C.c_uint32)]
 
 
+class Bslz4Block(C.Structure):
+    _fields_ = [("src_off", C.c_uint32), ("comp_len", C.c_uint32), ("dst_off", C.c_uint32),
+                ("uncomp_bytes", C.c_uint32)]
+
+
 class RawSums(C.Structure):
     _fields_ = [("sum_signal", C.c_void_p), ("sum_norm", C.c_void_p),
                 ("sum_variance", C.c_void_p), ("count", C.c_void_p)]
@@ -140,6 +147,13 @@ def lib() -> C.CDLL:
             "dio_integrate_variances": (i32, [vp, vp, i32, vp, i32, u64, u32, dp, dp]),
             "dio_multi_integrate": (i32, [u32, vp, vp, vp, vp, vp, vp, u64, u32, dp, dp,
                                           C.POINTER(RawSums)]),
+            "dio_integrate_bslz4": (i32, [vp, vp, vp, u64, i32, u32, dp, dp]),
+            "dio_bslz4_scan": (i32, [vp, vp, u64, u64, i32, vp, u64, C.POINTER(u64),
+                                     C.POINTER(u32), C.POINTER(u32)]),
+            "dio_bslz4_decode_device": (i32, [vp, vp, u64, vp, u64, u32, u32, i32, vp, u64, vp,
+                                              vp]),
+            "dio_bslz4_compress": (i32, [vp, u64, i32, u32, vp, u64, C.POINTER(u64)]),
+            "dio_bslz4_bound": (u64, [u64, i32, u32]),
             "dio_plan_autotune": (i32, [vp, i32, u64, u32, C.POINTER(u32), C.POINTER(u32)]),
             "dio_plan_set_tile": (i32, [vp, u32, u32]),
             "dio_graph_create": (i32, [vp, vp, i32, u64, u32, dp, dp, dp, C.POINTER(vp)]),

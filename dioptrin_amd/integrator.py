@@ -246,6 +246,30 @@ class GpuPlan:
                 S.ctypes.data))
         return I, S
 
+    # -- Bitshuffle + LZ4 chunks decoded on the device
+    def integrate_bslz4(self, chunks, dtype="uint32", errors=False, raw=False):
+        """(I, S) of raw HDF5 Bitshuffle+LZ4 chunks, one per frame, decoded
+        and integrated on the GPU (Cuda1dPlan::integrate_bslz4,
+        cuda.rs:550-564): the bits of integrate_stack on the decoded frames."""
+        from . import bslz4
+        return bslz4.integrate(self, chunks, dtype, errors, raw)
+
+    def batch1d_bslz4(self, chunks, dtype="uint32", *, errors=False) -> list:
+        """One IntegrationResult per chunk (Cuda1dPlan.batch1d_bslz4)."""
+        I, S = self.integrate_bslz4(chunks, dtype, errors)
+        return [self._result(I[k], None if S is None else S[k], None) for k in range(I.shape[0])]
+
+    batch2d_bslz4 = batch1d_bslz4
+
+    def batch2d_bslz4_stack(self, chunks, dtype="uint32", *, errors=False):
+        """(n_frames, n_azimuthal, n_radial) intensity (and errors) of a 2D
+        plan (Cuda2dPlan.batch2d_bslz4_stack)."""
+        if self.ndim != 2:
+            raise ValueError("batch2d_bslz4_stack needs a 2D plan")
+        I, S = self.integrate_bslz4(chunks, dtype, errors)
+        shape = (I.shape[0], self.n_azimuthal, self.n_radial)
+        return (I.reshape(shape), S.reshape(shape)) if errors else I.reshape(shape)
+
     # -- integration
     def _stack(self, images) -> np.ndarray:
         if isinstance(images, np.ndarray) and images.ndim == 3:

@@ -237,6 +237,52 @@ int dio_multi_integrate(uint32_t n_det, dio_plan *const *plans, const void *cons
                         const double *factors, uint64_t n_frames, uint32_t flags,
                         double *intensity, double *sigma, dio_raw_sums *raw);
 
+/* Bitshuffle + LZ4 chunks (raw HDF5 filter 32008 chunks, one per frame, as
+ * h5py's read_direct_chunk returns them; bitshuffle 0.5.2 / LZ4 block
+ * format). Cuda1dPlan::integrate_bslz4 / Cuda2dPlan::integrate_bslz4(_into)
+ * (src/integration/cuda.rs:550-564, 758-767, 843-862; decode kernel
+ * src/integration/cuda/kernels.cu:302-493; descriptor pre-scan
+ * src/integration/cuda/device.rs:2029-2180). */
+typedef struct {
+  uint32_t src_off;      /* byte offset of the block in the compressed buffer */
+  uint32_t comp_len;     /* compressed bytes; bit 31 set: verbatim tail bytes */
+  uint32_t dst_off;      /* first element of the block in the decoded frames */
+  uint32_t uncomp_bytes; /* decoded bytes of the block */
+} dio_bslz4_block;
+
+/* Decodes n_frames chunks (chunks[f], sizes[f] bytes, host memory) of dtype
+ * DIO_U16, DIO_U32 or DIO_I32 on the device and integrates them with the
+ * plan; flags DIO_WITH_ERRORS, DIO_RAW_SUMS. Results are the bits of
+ * dio_integrate on the decoded frames. A malformed chunk (header, sizes,
+ * block size, truncation: checked on the host; LZ4 stream: checked on the
+ * device) fails with DIO_ERR_INVALID. Synchronous. */
+int dio_integrate_bslz4(dio_plan *plan, const uint8_t *const *chunks, const uint64_t *sizes,
+                        uint64_t n_frames, int dtype, uint32_t flags, double *intensity,
+                        double *sigma);
+/* Block descriptors of n_frames chunks of `pixels` elements each, their
+ * offsets relative to the chunks concatenated in order (the reference's
+ * pre-scan). blocks == NULL: count only. max_comp / max_block (may be NULL)
+ * receive the largest compressed and decoded block sizes. */
+int dio_bslz4_scan(const uint8_t *const *chunks, const uint64_t *sizes, uint64_t n_frames,
+                   uint64_t pixels, int dtype, dio_bslz4_block *blocks, uint64_t cap,
+                   uint64_t *n_blocks, uint32_t *max_comp, uint32_t *max_block);
+/* Device-resident decode (no reference counterpart as an entry point: the
+ * reference decodes inside integrate_bslz4): d_comp holds comp_bytes bytes
+ * of concatenated chunks, d_blocks their descriptors; decoded elements go to
+ * d_frames (n_elems elements of dtype). *d_err becomes nonzero when a block
+ * is corrupt or inconsistent with the buffers (the caller zeroes it first);
+ * nothing is written outside d_frames. Enqueued on `stream`. */
+int dio_bslz4_decode_device(dio_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
+                            const dio_bslz4_block *d_blocks, uint64_t n_blocks, uint32_t max_comp,
+                            uint32_t max_block, int dtype, void *d_frames, uint64_t n_elems,
+                            uint32_t *d_err, void *stream);
+/* Writes one frame as a Bitshuffle + LZ4 chunk (greedy LZ4 block
+ * compressor; block_bytes 0 = 8192). dio_bslz4_bound gives a sufficient
+ * capacity. No reference counterpart (the reference reads chunks only). */
+int dio_bslz4_compress(const void *frame, uint64_t n_elems, int dtype, uint32_t block_bytes,
+                       uint8_t *out, uint64_t cap, uint64_t *out_len);
+uint64_t dio_bslz4_bound(uint64_t n_elems, int dtype, uint32_t block_bytes);
+
 /* Launch shape of the fused batch kernel (no reference counterpart; the
  * reference launches one frame per kernel, src/integration/cuda/device.rs:
  * 1460-1584): frames per pass and frames per lane, one of the pairs (1, 1),

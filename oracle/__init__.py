@@ -434,3 +434,53 @@ def grid2d_bin(nr, na, radial_range, azimuthal_range, flags, radial, az):
     if b == -2:
         raise ValueError("invalid grid")
     return None if b < 0 else int(b)
+
+
+# ------------------------------------------------------------ Bitshuffle + LZ4
+
+def lz4_decode_block(block: bytes, out_cap: int):
+    """Decoded bytes of one LZ4 block, or None when it is malformed."""
+    out = np.empty(max(out_cap, 1), np.uint8)
+    L = lib()
+    L.or_lz4_decode_block.restype = C.c_long
+    n = L.or_lz4_decode_block(C.c_char_p(bytes(block)), C.c_size_t(len(block)),
+                              out.ctypes.data_as(C.c_void_p), C.c_size_t(out_cap))
+    return None if n < 0 else out[:n].tobytes()
+
+
+def bitshuffle(arr: np.ndarray) -> bytes:
+    a = np.ascontiguousarray(arr)
+    out = np.empty(a.nbytes, np.uint8)
+    lib().or_bitshuffle(a.ctypes.data_as(C.c_void_p), C.c_size_t(a.size),
+                        C.c_size_t(a.itemsize), out.ctypes.data_as(C.c_void_p))
+    return out.tobytes()
+
+
+def bitunshuffle(data: bytes, dtype, n: int) -> np.ndarray:
+    out = np.empty(n, dtype)
+    lib().or_bitunshuffle(C.c_char_p(bytes(data)), C.c_size_t(n), C.c_size_t(out.itemsize),
+                          out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def bslz4_decode(chunk: bytes, pixels: int, dtype):
+    """(status, elements): status 0 ok, 1 header, 2 truncated, 3 bad block."""
+    out = np.zeros(pixels, dtype)
+    rc = lib().or_bslz4_decode(C.c_char_p(bytes(chunk)), C.c_size_t(len(chunk)),
+                               C.c_size_t(pixels), C.c_size_t(out.itemsize),
+                               out.ctypes.data_as(C.c_void_p))
+    return rc, out
+
+
+def bslz4_encode(frame: np.ndarray, block_bytes: int = 8192, mode: int = 0) -> bytes:
+    """The reference's test encoders: mode 0 literal-only LZ4, 1 byte runs."""
+    a = np.ascontiguousarray(frame).ravel()
+    cap = 64 + (a.nbytes // block_bytes + 3) * (block_bytes + block_bytes // 255 + 24) + a.nbytes
+    out = np.empty(cap, np.uint8)
+    L = lib()
+    L.or_bslz4_encode.restype = C.c_size_t
+    n = L.or_bslz4_encode(a.ctypes.data_as(C.c_void_p), C.c_size_t(a.size),
+                          C.c_size_t(a.itemsize), C.c_size_t(block_bytes), C.c_int(mode),
+                          out.ctypes.data_as(C.c_void_p), C.c_size_t(cap))
+    assert n > 0
+    return out[:n].tobytes()

@@ -149,6 +149,18 @@ void or_csc_free(void *csc);
 void or_csc_batch_u32(const void *csc, const uint32_t *frames, size_t n_frames, int threads,
                       double *out_i);
 
+/* Bitshuffle + LZ4 chunks (bslz4.c; the codec of HDF5 filter 32008,
+ * bitshuffle 0.5.2): LZ4 block decode (-1 when malformed), bit transposition
+ * both ways, whole-chunk decode (0 ok, 1 header, 2 truncated, 3 bad LZ4
+ * block), and the reference's two test encoders (mode 0 literal-only,
+ * metal/tests.rs:297-357; mode 1 byte runs, metal/bench.rs:150-230). */
+long or_lz4_decode_block(const uint8_t *in, size_t in_len, uint8_t *out, size_t out_cap);
+void or_bitshuffle(const uint8_t *src, size_t n, size_t eb, uint8_t *dst);
+void or_bitunshuffle(const uint8_t *src, size_t n, size_t eb, uint8_t *dst);
+int or_bslz4_decode(const uint8_t *ch, size_t len, size_t pixels, size_t eb, uint8_t *out);
+size_t or_bslz4_encode(const uint8_t *raw, size_t pixels, size_t eb, size_t block_bytes, int mode,
+                       uint8_t *out, size_t out_cap);
+
 #ifdef __cplusplus
 }
 #endif
