@@ -241,6 +241,62 @@ def serve_bench(rates, n_requests, seed=7):
     return out
 
 
+def bslz4_bench(plan, frames, errors, n_chunks=64, distinct=16, reps=10):
+    """Bitshuffle + LZ4 leg: `distinct` of the step's frames written as raw
+    HDF5 chunks (8 KiB blocks, the library's LZ4 encoder, 16 host threads),
+    repeated to n_chunks chunks resident in device memory with their block
+    descriptors. Timed: the device decode alone and decode + integrate of the
+    n_chunks frames; the decoded frames are checked against the originals."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from dioptrin_amd import bslz4
+    from dioptrin_amd.device import DeviceArray, synchronize
+    host = frames[:distinct].numpy()
+    t0 = time.time()
+    with ThreadPoolExecutor(16) as ex:
+        distinct_chunks = list(ex.map(bslz4.compress, host))
+    log(f"[bench] bslz4: {distinct} chunks written in {time.time() - t0:.1f} s")
+    chunks = [distinct_chunks[k % distinct] for k in range(n_chunks)]
+    comp = sum(len(c) for c in chunks)
+    shape = tuple(frames.shape[1:])
+    npix = shape[0] * shape[1]
+    dc = bslz4.DeviceChunks(chunks, shape, "uint32")
+    out = DeviceArray((n_chunks, *shape), np.uint32)
+    dc.decode(out)
+    synchronize()
+    dc.check()
+    if not np.array_equal(out[:distinct].numpy(), host):
+        raise RuntimeError("bslz4: decoded frames differ from the originals")
+    out_i = DeviceArray((n_chunks, plan.nbins), np.float64)
+    out_s = DeviceArray((n_chunks, plan.nbins), np.float64) if errors else None
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dc.decode(out)
+    synchronize()
+    dec_s = (time.perf_counter() - t0) / reps
+    for _ in range(2):
+        dc.decode(out)
+        plan.integrate_device(out, out_i, out_s, errors=errors)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dc.decode(out)
+        plan.integrate_device(out, out_i, out_s, errors=errors)
+    synchronize()
+    both_s = (time.perf_counter() - t0) / reps
+    dc.check()
+    # Algorithmic bytes of a decode: compressed bytes and descriptors read,
+    # decoded frames written.
+    alg = comp + dc.n_blocks * 16 + n_chunks * npix * 4
+    return {"chunks": n_chunks, "distinct_frames": distinct, "block_bytes": 8192,
+            "compression_ratio": round(n_chunks * npix * 4 / comp, 3),
+            "decode_ms": round(dec_s * 1e3, 3), "decode_fps": round(n_chunks / dec_s, 1),
+            "decode_gbs_algorithmic": round(alg / dec_s / 1e9, 1),
+            "decode_frac_hbm_peak": round(alg / dec_s / 1e9 / HBM_PEAK_GBS, 4),
+            "decode_integrate_fps": round(n_chunks / both_s, 1),
+            "frames": "device-resident compressed chunks (uint32 Eiger 9M frames of the step)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +311,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
     ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--no-bslz4", action="store_true",
+                    help="skip the Bitshuffle+LZ4 decode section")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB 5 times "
                          "(reference for the memory-traffic counters) and exit")
@@ -360,6 +418,10 @@ def main():
     log(f"[bench] host frames: {host_fps}")
     dyn_fps = dynamic_mask_rate(plan, frames, min(nfr, 64), args.errors)
     log(f"[bench] per-frame masks: {dyn_fps} frames/s")
+    bslz4_res = None
+    if not args.no_bslz4:
+        bslz4_res = bslz4_bench(plan, frames, args.errors)
+        log(f"[bench] bslz4: {bslz4_res}")
 
     fl = max(f for _, f in launches)  # frames of a full launch = frames per table read
     full = [ms for ms, f in launches if f == fl]
@@ -428,6 +490,7 @@ def main():
         "cpu_baseline": None if cpu is None else {k: (round(v, 2) if isinstance(v, float) else v)
                                                   for k, v in cpu.items()},
         "serve": serve,
+        "bslz4": bslz4_res,
     }
     print(json.dumps(line), flush=True)
 

@@ -10,17 +10,21 @@
 // A decompressed block of n elements (n a multiple of 8) is bit-transposed:
 // bit k of element i sits in byte k * n/8 + i/8, bit i % 8.
 //
-// Device design (wave64): one wavefront per block. The compressed bytes are
-// staged in LDS with coalesced dword loads; every lane parses the LZ4
-// sequence stream from LDS in lockstep, the parse state kept scalar
+// Device design (wave64): one wavefront (and workgroup) per block, decoding
+// in place: the compressed block is staged with coalesced dword loads at the
+// end of an LDS region one block plus LZ4's in-place margin long, and
+// decoded forward from the region's start, so a wave holds ~8.3 KB of LDS
+// and about 19 blocks are in flight per CU to hide the serial LZ4 latency
+// (twice as many as with separate input and output areas). Every lane
+// parses the sequence stream in lockstep, the parse state kept scalar
 // (readfirstlane), and the 64 lanes copy literals and matches. A match byte
 // j equals out[op - offset + (j mod offset)], all of it written before the
 // match starts, so a whole match is copied in one parallel pass whatever its
 // offset. The bit transposition is done per lane, eight elements at a time,
-// as 8x8 bit-matrix transposes of 64-bit words; decoded elements go through
-// LDS so that the global stores are contiguous. Every index is checked
-// against the descriptor and buffer sizes: a corrupt stream sets the error
-// flag and writes nothing outside its block.
+// as 8x8 bit-matrix transposes of 64-bit words, stored straight to the
+// frames. Every index is checked against the descriptor and buffer sizes,
+// and the write position against the read position: a corrupt stream sets
+// the error flag, never reads or writes outside its region and block.
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -77,20 +81,22 @@ __device__ inline uint64_t transpose8(uint64_t x) {
 
 __device__ inline uint32_t sgpr(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// One block per wavefront. lds per wave: cap_c bytes of compressed input
-// (also reused for the decoded elements) and cap_o bytes of LZ4 output.
+// One block per wavefront; cap_o bytes of LDS per wave.
 template <int EB>
 __global__ __launch_bounds__(256) void k_bslz4(const uint8_t* __restrict__ comp, uint64_t comp_bytes,
                                                 const dio_bslz4_block* __restrict__ desc,
-                                                uint64_t nblocks, uint32_t cap_c, uint32_t cap_o,
+                                                uint64_t nblocks, uint32_t cap_o,
                                                 uint8_t* __restrict__ out, uint64_t n_elems,
                                                 uint32_t* __restrict__ err) {
   extern __shared__ __align__(16) uint8_t lds[];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint64_t b = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
   if (b >= nblocks) return;
-  uint8_t* c = lds + (size_t)wave * (cap_c + cap_o);
-  uint8_t* o = c + cap_c;
+  // One region of cap_o bytes per wave: the decoded block from its start,
+  // the compressed block staged at its end (in-place decoding: the margin
+  // cap_o - ub >= len / 256 + 32 keeps every write behind the read position
+  // of a well-formed stream; a malformed one decodes wrongly but in bounds).
+  uint8_t* o = lds + (size_t)wave * cap_o;
   const uint32_t src = sgpr(desc[b].src_off), len_raw = sgpr(desc[b].comp_len);
   const uint32_t dst = sgpr(desc[b].dst_off), ub = sgpr(desc[b].uncomp_bytes);
   auto fail = [&] {
@@ -108,8 +114,18 @@ __global__ __launch_bounds__(256) void k_bslz4(const uint8_t* __restrict__ comp,
   }
   const uint32_t len = len_raw;
   const uint32_t n = ub / EB;  // elements of the block, a multiple of 8
-  if ((uint64_t)src + len > comp_bytes || len + 8 > cap_c || ub > cap_o || ub % (8 * EB) ||
-      (uint64_t)dst + n > n_elems) {
+  const uint32_t L4 = (len + 7) & ~3u;
+  if ((uint64_t)src + len > comp_bytes || ub % (8 * EB) || (uint64_t)dst + n > n_elems ||
+      L4 + 4 > cap_o) {
+    fail();
+    return;
+  }
+  // Staged input: dword aligned, ending at the end of the region; byte k of
+  // the block sits at region offset in_base + k.
+  const uint32_t s0 = (cap_o - L4 - 4) & ~3u;
+  uint8_t* c = o + s0;
+  const uint32_t in_base = s0 + (src & 3u);
+  if ((uint64_t)ub + len / 256 + 32 > (uint64_t)in_base + len) {  // in-place margin
     fail();
     return;
   }
@@ -118,16 +134,27 @@ __global__ __launch_bounds__(256) void k_bslz4(const uint8_t* __restrict__ comp,
   // pass the end of the compressed buffer.
   const uint64_t a0 = src & ~3ull;
   const uint32_t head = src & 3u, nwords = (head + len + 3) / 4;
-  for (uint32_t i = lane; i < nwords; i += 64) {
-    const uint64_t a = a0 + 4ull * i;
-    uint32_t w;
-    if (a + 4 <= comp_bytes) {
-      w = *reinterpret_cast<const uint32_t*>(comp + a);
-    } else {
-      w = 0;
-      for (uint32_t k = 0; k < 4 && a + k < comp_bytes; k++) w |= (uint32_t)comp[a + k] << (8 * k);
+  const uint32_t* comp32 = reinterpret_cast<const uint32_t*>(comp + a0);
+  uint32_t* c32 = reinterpret_cast<uint32_t*>(c);
+  const uint32_t full = a0 + 4ull * nwords <= comp_bytes ? nwords : nwords - 1;
+  for (uint32_t i0 = 0; i0 < full; i0 += 64 * 8) {
+    uint32_t w[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint32_t i = i0 + lane + 64 * u;
+      w[u] = comp32[i < full ? i : 0u];
     }
-    *reinterpret_cast<uint32_t*>(c + 4 * i) = w;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint32_t i = i0 + lane + 64 * u;
+      if (i < full) c32[i] = w[u];
+    }
+  }
+  if (full < nwords && lane == 0) {
+    const uint64_t a = a0 + 4ull * full;
+    uint32_t w = 0;
+    for (uint32_t k = 0; k < 4 && a + k < comp_bytes; k++) w |= (uint32_t)comp[a + k] << (8 * k);
+    c32[full] = w;
   }
   __builtin_amdgcn_wave_barrier();
   const uint8_t* in = c + head;
@@ -147,7 +174,7 @@ __global__ __launch_bounds__(256) void k_bslz4(const uint8_t* __restrict__ comp,
         lit += s;
       } while (s == 255);
     }
-    if (bad || lit > len - ip || lit > ub - op) { bad = true; break; }
+    if (bad || lit > len - ip || lit > ub - op || op > in_base + ip) { bad = true; break; }
     for (uint32_t k = lane; k < lit; k += 64) o[op + k] = in[ip + k];
     ip += lit;
     op += lit;
@@ -166,7 +193,7 @@ __global__ __launch_bounds__(256) void k_bslz4(const uint8_t* __restrict__ comp,
         ml += s;
       } while (s == 255);
     }
-    if (bad || ml > ub - op) { bad = true; break; }
+    if (bad || ml > ub - op || op + ml > in_base + ip) { bad = true; break; }
     // Byte j of the match is o[op - off + j mod off]: all sources precede op.
     uint32_t r = off >= 64 ? lane : lane % off;
     const uint32_t step = off >= 64 ? (off == 64 ? 0u : 64u) : 64u % off;
@@ -187,7 +214,6 @@ __global__ __launch_bounds__(256) void k_bslz4(const uint8_t* __restrict__ comp,
   // bit 8q + p of element 8g + e); after the transpose byte e of y holds
   // bits 8q .. 8q + 7 of element 8g + e.
   const uint32_t plane = n / 8;
-  uint8_t* e8 = c;  // decoded elements, little endian, reusing the input area
   for (uint32_t g = lane; g < plane; g += 64) {
     uint64_t y[EB];
 #pragma unroll
@@ -203,21 +229,10 @@ __global__ __launch_bounds__(256) void k_bslz4(const uint8_t* __restrict__ comp,
 #pragma unroll
       for (int q = 0; q < EB; q++) v |= (uint32_t)(uint8_t)(y[q] >> (8 * e)) << (8 * q);
       if (EB == 4)
-        reinterpret_cast<uint32_t*>(e8)[8 * g + e] = v;
+        reinterpret_cast<uint32_t*>(out)[(uint64_t)dst + 8 * g + e] = v;
       else
-        reinterpret_cast<uint16_t*>(e8)[8 * g + e] = (uint16_t)v;
+        reinterpret_cast<uint16_t*>(out)[(uint64_t)dst + 8 * g + e] = (uint16_t)v;
     }
-  }
-  __builtin_amdgcn_wave_barrier();
-  // Contiguous stores: lane i writes element i, i + 64, ...
-  if (EB == 4) {
-    uint32_t* dst32 = reinterpret_cast<uint32_t*>(out) + dst;
-    const uint32_t* s32 = reinterpret_cast<const uint32_t*>(e8);
-    for (uint32_t i = lane; i < n; i += 64) dst32[i] = s32[i];
-  } else {
-    uint16_t* dst16 = reinterpret_cast<uint16_t*>(out) + dst;
-    const uint16_t* s16 = reinterpret_cast<const uint16_t*>(e8);
-    for (uint32_t i = lane; i < n; i += 64) dst16[i] = s16[i];
   }
 }
 
@@ -291,20 +306,21 @@ void launch_decode(const uint8_t* d_comp, uint64_t comp_bytes, const dio_bslz4_b
           "Bitshuffle/LZ4 block exceeds the decoder's limits");
   // Compressed staging holds the block and its 3-byte alignment head, and
   // later the decoded elements (max_block bytes).
-  const uint32_t cap_c = (std::max(max_comp + 8, max_block) + 15) / 16 * 16;
-  const uint32_t cap_o = (max_block + 15) / 16 * 16;
-  const uint32_t per_wave = cap_c + cap_o;
-  const uint32_t waves = std::max(1u, std::min(4u, (64u << 10) / per_wave));
+  // In-place region: the largest block, the in-place margin of the largest
+  // compressed block and its alignment.
+  const uint32_t cap_o = (std::max(max_block + max_comp / 256 + 32, max_comp) + 16 + 15) / 16 * 16;
+  const uint32_t per_wave = cap_o;
+  const uint32_t waves = 1;  // one wave per workgroup measured fastest (LDS-bound occupancy)
   const uint64_t grid = (nblocks + waves - 1) / waves;
   require(grid < (1ull << 31), DIO_ERR_INVALID, "too many Bitshuffle/LZ4 blocks in one call");
   const size_t shmem = (size_t)waves * per_wave;
   auto* out = static_cast<uint8_t*>(d_frames);
   if (eb == 4)
-    k_bslz4<4><<<(uint32_t)grid, 64 * waves, shmem, s>>>(d_comp, comp_bytes, d_desc, nblocks, cap_c,
-                                                        cap_o, out, n_elems, d_err);
+    k_bslz4<4><<<(uint32_t)grid, 64 * waves, shmem, s>>>(d_comp, comp_bytes, d_desc, nblocks, cap_o,
+                                                        out, n_elems, d_err);
   else
-    k_bslz4<2><<<(uint32_t)grid, 64 * waves, shmem, s>>>(d_comp, comp_bytes, d_desc, nblocks, cap_c,
-                                                        cap_o, out, n_elems, d_err);
+    k_bslz4<2><<<(uint32_t)grid, 64 * waves, shmem, s>>>(d_comp, comp_bytes, d_desc, nblocks, cap_o,
+                                                        out, n_elems, d_err);
   DIO_HIP(hipGetLastError());
 }
 
