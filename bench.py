@@ -80,6 +80,33 @@ def dynamic_mask_rate(plan, frames_dev, n, errors):
     return round(best, 1)
 
 
+def std_bytes_per_frame(npix, nnz, bins, frames_per_pass, px_bytes=4):
+    """Algorithmic bytes per frame of the azimuthal spread on the tiled
+    table: the frame once, the table (u32 column + f32 weight per entry, row
+    pointers) and the float64 pixel normalization once per pass of
+    frames_per_pass frames, and three float64 outputs (I, sigma, std)."""
+    return (px_bytes * npix + (8 * nnz + 4 * (bins + 1) + 8 * npix) / frames_per_pass
+            + 8 * bins * 3)
+
+
+def azimuthal_std_rate(plan, frames_dev, n):
+    """frames/s of plan.integrate_device with errors and the azimuthal std on
+    n device frames (the whole call: spread kernel and bin merge); best of
+    three runs."""
+    from dioptrin_amd.device import DeviceArray, synchronize
+    outs = [DeviceArray((n, plan.nbins), np.float64) for _ in range(3)]
+    fr = frames_dev[:n]
+    plan.integrate_device(fr, *outs, errors=True, azimuthal_std=True)
+    synchronize()
+    best = 0.0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        plan.integrate_device(fr, *outs, errors=True, azimuthal_std=True)
+        synchronize()
+        best = max(best, n / (time.perf_counter() - t0))
+    return round(best, 1)
+
+
 def host_cpu():
     """nproc, the model name from /proc/cpuinfo and the cores this job may
     use (OMP_NUM_THREADS when set, as the GPU host sets it to the job's share
@@ -418,6 +445,9 @@ def main():
     log(f"[bench] host frames: {host_fps}")
     dyn_fps = dynamic_mask_rate(plan, frames, min(nfr, 64), args.errors)
     log(f"[bench] per-frame masks: {dyn_fps} frames/s")
+    std_fps = azimuthal_std_rate(plan, frames, min(nfr, 64))
+    std_bpf = std_bytes_per_frame(npix, info["nnz_masked"], bins, 8)
+    log(f"[bench] azimuthal std: {std_fps} frames/s")
     bslz4_res = None
     if not args.no_bslz4:
         bslz4_res = bslz4_bench(plan, frames, args.errors)
@@ -481,6 +511,11 @@ def main():
         "per_frame_fps": round(per_frame_fps, 1),
         "host_frames_fps_pcie": host_fps,
         "dynamic_masks_fps": dyn_fps,
+        "azimuthal_std": {"fps": std_fps, "outputs": "intensity, sigma, std",
+                          "frames_per_pass": 8,
+                          "bytes_per_frame_algorithmic": int(std_bpf),
+                          "gbs_algorithmic": round(std_bpf * std_fps / 1e9, 1),
+                          "frac_hbm_peak": round(std_bpf * std_fps / 1e9 / HBM_PEAK_GBS, 4)},
         "batched_vs_per_frame": round(fps / per_frame_fps, 2),
         "lut_build_ms_gpu": round(info["build_ms"], 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

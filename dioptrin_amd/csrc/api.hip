@@ -70,7 +70,10 @@ void run_device(dio_plan* p, TileScratch& sc, const void* frames, int dtype,
             "raw sums are not defined with the azimuthal std");
     require(p->desc.ndim == 1, DIO_ERR_UNSUPPORTED, "azimuthal std is defined for 1D plans");
     require(SD != nullptr, DIO_ERR_INVALID, "azimuthal std output missing");
-    azimuthal_std_frames(p->st, frames, dtype, n_frames, var, I, S, SD, s);
+    require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
+    sc.ensure_bytes(std_pass_bytes(p->tt, std_frames_per_pass(), var));
+    azimuthal_std_tiled(p->tt, p->st.norm, sc.ptr(), sc.bytes(), frames, dtype, n_frames, var, I,
+                        S, SD, s);
     return;
   }
   require(!var || S != nullptr, DIO_ERR_INVALID, "sigma output missing");
@@ -375,8 +378,19 @@ int dio_integrate_device_noalloc(dio_plan* p, const void* d_frames, int dtype,
     std::lock_guard<std::mutex> lk(p->mu);
     DIO_HIP(hipSetDevice(p->device));
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
-    if (flags & DIO_WITH_AZIMUTHAL_STD) {  // the spread kernel uses no scratch
-      run_device(p, p->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std, s);
+    if (flags & DIO_WITH_AZIMUTHAL_STD) {
+      // The spread's segment partials in the fixed scratch: passes of as
+      // many frames as it holds (at most the usual pass).
+      size_t cap = 0;
+      void* scratch = dio::noalloc_scratch(p->device, &cap);
+      if (dio::std_pass_bytes(p->tt, 1, flags & DIO_WITH_ERRORS) > cap)
+        dio::fail(DIO_ERR_INVALID,
+                  "the azimuthal-spread partials of one frame exceed the fixed scratch of "
+                  "dio_integrate_device_noalloc (64 MiB)");
+      dio::TileScratch sc;
+      sc.ext = static_cast<double*>(scratch);
+      sc.ext_n = cap / sizeof(double);
+      run_device(p, sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std, s);
       return;
     }
     const bool var = flags & DIO_WITH_ERRORS;
@@ -592,6 +606,9 @@ int dio_graph_create(dio_plan* p, const void* d_frames, int dtype, uint64_t n_fr
     auto g = std::make_unique<dio_graph>();
     g->plan = p;
     g->sc.ensure(p->tt, p->tile, flags & DIO_WITH_ERRORS);
+    if (flags & DIO_WITH_AZIMUTHAL_STD)
+      g->sc.ensure_bytes(dio::std_pass_bytes(p->tt, dio::std_frames_per_pass(),
+                                             flags & DIO_WITH_ERRORS));
     DIO_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     try {
       run_device(p, g->sc, d_frames, dtype, n_frames, flags, d_intensity, d_sigma, d_std,

@@ -1,6 +1,7 @@
-// Azimuthal-spread kernel (gfx950): one wavefront per bin row of the
-// unmasked CSR table and 4 frames, FP64 arithmetic on float32 weights. The path is a
-// sparse, bandwidth-bound reduction, so it uses the vector ALUs, not MFMA.
+// Per-frame-mask kernels (gfx950): the CSR route (one wavefront per bin row
+// of the unmasked table) and the staging and finishing passes of the tile
+// route. The path is a sparse, bandwidth-bound reduction, so it uses the
+// vector ALUs, not MFMA.
 #include <algorithm>
 #include <cfloat>
 
@@ -14,137 +15,12 @@ constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
 
-// --------------------------------------------------------- azimuthal std
+// --------------------------------------------------------- per-frame masks
 
 __device__ inline double wave_sum(double x) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) x = x + __shfl_down(x, off, kWave);
   return __shfl(x, 0, kWave);
-}
-
-// azimuthal.rs:48-118: weighted population spread of corrected pixels on the
-// unmasked table with the mask applied per entry. One wavefront per row and
-// NF frames: each entry's index, weight, mask and normalization are read once
-// for the NF frames, so the three passes over the row read the table once
-// per NF frames. FP64 arithmetic on float32 weights; each frame's arithmetic
-// (lane split, order, reduction tree) is that of a frame alone.
-template <class P, int NF>
-__global__ __launch_bounds__(kBlock) void k_std(const uint32_t* __restrict__ row_ptr,
-                                                const uint32_t* __restrict__ col,
-                                                const float* __restrict__ w32,
-                                                const uint8_t* __restrict__ mask,
-                                                const double* __restrict__ norm,
-                                                const P* __restrict__ frames, uint64_t npix,
-                                                uint64_t n_frames, uint32_t nbins, uint32_t dim1,
-                                                uint32_t dim2, int flip, int errors, double* I,
-                                                double* S, double* SD) {
-  uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  uint32_t r = blockIdx.x * kWavesPerBlock + wave;
-  if (r >= nbins) return;
-  const uint64_t f0 = (uint64_t)blockIdx.y * NF;
-  const int nf = (int)min<uint64_t>(NF, n_frames - f0);
-  const P* fr = frames + f0 * npix;
-  uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
-  // Entry k's frame-independent part; false when masked.
-  auto entry = [&](uint32_t k, uint32_t& j, double& c, double& n) -> bool {
-    j = col[k];
-    if (mask && mask[j]) return false;
-    n = norm[j];
-    c = (double)w32[k];
-    j = flip ? (dim1 - 1 - j / dim2) * dim2 + j % dim2 : j;
-    return true;
-  };
-  auto contrib = [&](double c, double n, double s, double& x, double& w) -> bool {
-    x = s / n;
-    w = c * n;
-    return isfinite(s) && isfinite(n) && n > 0.0 && isfinite(c) && c > 0.0 && isfinite(x) &&
-           isfinite(w) && w > 0.0;
-  };
-  // Origin: the first valid contribution in row order, per frame.
-  bool found[NF];
-  double origin[NF];
-  int left = nf;
-#pragma unroll
-  for (int f = 0; f < NF; f++) {
-    found[f] = f >= nf;
-    origin[f] = 0.0;
-  }
-  for (uint32_t base = k0; base < k1 && left > 0; base += kWave) {
-    const uint32_t k = base + lane;
-    uint32_t j = 0;
-    double c = 0.0, n = 0.0;
-    const bool e = k < k1 && entry(k, j, c, n);
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      if (found[f]) continue;
-      double x = 0.0, w = 0.0;
-      const bool ok = e && contrib(c, n, (double)fr[(uint64_t)f * npix + j], x, w);
-      const unsigned long long m = __ballot(ok);
-      if (m) {
-        const int first = __ffsll((long long)m) - 1;
-        origin[f] = __shfl(ok ? x : 0.0, first, kWave);
-        found[f] = true;
-        left--;
-      }
-    }
-  }
-  double ss[NF], sw[NF], sh[NF], sv[NF];
-#pragma unroll
-  for (int f = 0; f < NF; f++) ss[f] = sw[f] = sh[f] = sv[f] = 0.0;
-  for (uint32_t k = k0 + lane; k < k1; k += kWave) {
-    uint32_t j;
-    double c, n;
-    if (!entry(k, j, c, n)) continue;
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      if (f >= nf) continue;
-      const double s = (double)fr[(uint64_t)f * npix + j];
-      double x, w;
-      if (!contrib(c, n, s, x, w)) continue;
-      ss[f] = ss[f] + c * s;
-      sw[f] = sw[f] + w;
-      sh[f] = sh[f] + w * (x - origin[f]);
-      sv[f] = sv[f] + c * c * fmax(s, 0.0);
-    }
-  }
-  double mean_shifted[NF], m2[NF];
-#pragma unroll
-  for (int f = 0; f < NF; f++) {
-    ss[f] = wave_sum(ss[f]);
-    sw[f] = wave_sum(sw[f]);
-    sh[f] = wave_sum(sh[f]);
-    sv[f] = wave_sum(sv[f]);
-    mean_shifted[f] = sh[f] / sw[f];
-    m2[f] = 0.0;
-  }
-  for (uint32_t k = k0 + lane; k < k1; k += kWave) {
-    uint32_t j;
-    double c, n;
-    if (!entry(k, j, c, n)) continue;
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      if (f >= nf) continue;
-      double x, w;
-      if (!contrib(c, n, (double)fr[(uint64_t)f * npix + j], x, w)) continue;
-      const double delta = (x - origin[f]) - mean_shifted[f];
-      m2[f] = m2[f] + w * delta * delta;
-    }
-  }
-#pragma unroll
-  for (int f = 0; f < NF; f++) {
-    m2[f] = wave_sum(m2[f]);
-    if (f >= nf || lane != 0) continue;
-    const uint64_t ob = (f0 + f) * nbins + r;
-    if (!found[f]) {
-      I[ob] = 0.0;
-      if (errors) S[ob] = 0.0;
-      SD[ob] = NAN;
-      continue;
-    }
-    I[ob] = ss[f] / sw[f];
-    if (errors) S[ob] = sqrt(fmax(sv[f], 0.0)) / sw[f];
-    SD[ob] = sqrt(fmax(m2[f] / sw[f], 0.0));
-  }
 }
 
 // prepared.rs:1011-1121 (row, strict order per lane) with a per-frame mask:
@@ -239,30 +115,6 @@ __global__ __launch_bounds__(256) void k_masked_finish(double* I, double* S, con
 }
 
 }  // namespace
-
-void azimuthal_std_frames(const StdTable& t, const void* frames, int dtype, uint64_t n_frames,
-                          bool errors, double* intensity, double* sigma, double* std_out,
-                          hipStream_t s) {
-  if (n_frames == 0) return;
-  constexpr int NF = 4;  // frames per wavefront
-  dim3 grid((t.nbins + kWavesPerBlock - 1) / kWavesPerBlock, (uint32_t)((n_frames + NF - 1) / NF));
-  switch (dtype) {
-#define DIO_STD(T)                                                                            \
-  k_std<T, NF><<<grid, kBlock, 0, s>>>(t.row_ptr, t.col, t.w32, t.mask, t.norm,              \
-                                       static_cast<const T*>(frames), t.npix, n_frames,        \
-                                       t.nbins, t.dim1, t.dim2, t.flip ? 1 : 0, errors,        \
-                                   intensity, sigma, std_out);                                 \
-  break;
-    case DIO_U16: DIO_STD(uint16_t)
-    case DIO_U32: DIO_STD(uint32_t)
-    case DIO_I32: DIO_STD(int32_t)
-    case DIO_F32T: DIO_STD(float)
-    case DIO_F64T: DIO_STD(double)
-#undef DIO_STD
-    default: fail(DIO_ERR_INVALID, "unknown frame dtype");
-  }
-  DIO_HIP(hipGetLastError());
-}
 
 void masked_frames(const StdTable& t, const void* frames, int dtype, const uint8_t* masks,
                    uint64_t n_frames, bool errors, double* intensity, double* sigma,

@@ -1,6 +1,7 @@
 // Plan table views and the azimuthal-spread kernel on gfx950.
 #pragma once
 #include "common.hpp"
+#include "tile.hpp"
 
 namespace dio {
 
@@ -29,11 +30,19 @@ struct StdTable {
   bool flip = false;  // frames hold the detector's rows bottom-up
 };
 
-// azimuthal.rs:48-118 for n_frames device frames: intensity, sigma (when
-// errors) and the weighted population spread of every bin.
-void azimuthal_std_frames(const StdTable& t, const void* frames, int dtype, uint64_t n_frames,
-                          bool errors, double* intensity, double* sigma, double* std_out,
-                          hipStream_t s);
+// azimuthal.rs:48-118 for n_frames device frames on the detector-tiled
+// table (std_tile.hip): intensity, sigma (when errors) and the weighted
+// population spread of every bin. norm is the plan's pixel normalization
+// (detector order). The segment partials of a pass go to scratch
+// (scratch_bytes): passes hold std_frames_per_pass() frames, fewer when the
+// scratch holds fewer; less than one frame's partials is refused. A frame's
+// result does not depend on the pass.
+size_t std_pass_bytes(const TiledTable& t, uint32_t frames, bool errors);
+uint32_t std_frames_per_pass();
+void azimuthal_std_tiled(const TiledTable& t, const double* norm, void* scratch,
+                         size_t scratch_bytes, const void* frames, int dtype, uint64_t n_frames,
+                         bool errors, double* intensity, double* sigma, double* std_out,
+                         hipStream_t s);
 
 // Dynamic masks (Dynamic1dPlan::integrate, integration/dynamic.rs:143-186;
 // prepared.rs:1011-1121 with a per-frame mask): n_frames device frames, each

@@ -199,6 +199,12 @@ __global__ void k_iota32(uint32_t* v, uint32_t n) {
 
 __global__ void k_set1(uint32_t* p, uint32_t v) { *p = v; }
 
+// rank[perm[k]] = k: the position of each segment in bin order.
+__global__ void k_invert32(const uint32_t* perm, uint32_t n, uint32_t* rank) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) rank[perm[k]] = k;
+}
+
 int bits_for(uint64_t n) {
   int b = 1;
   while (b < 32 && (1ull << b) < n) b++;
@@ -1323,6 +1329,7 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
     if (keep_w64) t.w64.alloc(kGroup);
     t.seg_group.alloc(1);
     t.bin_seg.alloc(1);
+    t.seg_rank.alloc(1);
     t.tile_meta.alloc(4 * (size_t)t.ntiles);
     DIO_HIP(hipMemsetAsync(t.tile_meta.p, 0, 16 * (size_t)t.ntiles, s));
     DIO_HIP(hipMemsetAsync(t.seg_group.p, 0, 4, s));
@@ -1412,6 +1419,8 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
   DIO_HIP(hipMemsetAsync(hist.p, 0, (nbins + 1) * 4, s));
   k_count_u32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(bin_n.p, nseg, hist.p);
   scan_u32(hist.p, t.bin_ptr.p, nbins + 1, s, tmp);
+  t.seg_rank.alloc(nseg);
+  k_invert32<<<blocks_for(nseg, kThreads), kThreads, 0, s>>>(t.bin_seg.p, nseg, t.seg_rank.p);
   DIO_HIP(hipGetLastError());
   std::vector<uint32_t> meta(4 * (size_t)t.ntiles);
   DIO_HIP(hipMemcpyAsync(meta.data(), t.tile_meta.p, meta.size() * 4, hipMemcpyDeviceToHost, s));

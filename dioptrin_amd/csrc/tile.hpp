@@ -44,10 +44,11 @@ struct TiledTable {
   DevBuf<uint32_t> tile_meta;  // 4 * ntiles: first, end segment; first, end group
   DevBuf<uint32_t> bin_seg;    // nseg: segments sorted by (bin, tile)
   DevBuf<uint32_t> bin_ptr;    // nbins + 1
+  DevBuf<uint32_t> seg_rank;   // nseg: position of each segment in bin_seg
   const double* den = nullptr;
   size_t bytes() const {
     return lidx.bytes() + w32.bytes() + w64.bytes() + seg_group.bytes() + tile_seg.bytes() +
-           tile_meta.bytes() + bin_seg.bytes() + bin_ptr.bytes();
+           tile_meta.bytes() + bin_seg.bytes() + bin_ptr.bytes() + seg_rank.bytes();
   }
 };
 
@@ -69,6 +70,13 @@ struct TileScratch {
     return (size_t)t.nseg * frames * (var ? 2 : 1) * (kMaxPassesPerLaunch / 2);
   }
   double* ptr() const { return ext ? ext : part.p; }
+  size_t bytes() const { return (ext ? ext_n : part.n) * sizeof(double); }
+  // At least b bytes of owned scratch (external memory is used as it is).
+  void ensure_bytes(size_t b) {
+    if (ext) return;
+    const size_t n = (b + sizeof(double) - 1) / sizeof(double);
+    if (part.n < n) part.alloc(n);
+  }
   void ensure(const TiledTable& t, uint32_t frames, bool var) {
     const size_t n = need(t, frames, var);
     if (ext) {
