@@ -343,20 +343,50 @@ def test_azimuthal_std_matches_oracle(pilatus):
             assert rel.max() <= 1e-6, (name, rel.max())
 
 
+def test_azimuthal_std_special_values_match_oracle(pilatus):
+    """The reference's contribution filter (azimuthal.rs:60-79) on the
+    device: NaN rows, infinite pixels and negative values are skipped or
+    kept exactly as the oracle does, a frame far from zero (1e6 plus Poisson
+    counts) keeps the spread, and rows left without a valid contribution
+    give (0, 0, NaN)."""
+    d = pilatus
+    shape = d["w"]["shape"]
+    plan = GpuPlan(d["poni"], shape, n_radial=d["w"]["n_radial"],
+                   polarization_factor=0.99, mask=d["mask"])
+    rng = np.random.default_rng(31)
+    frames = rng.poisson(60.0, size=(3,) + shape).astype(np.float64)
+    frames[0, 421:622, 390:591] = np.nan        # the inner rings around the beam: no valid pixel
+    frames[0, 500:520, 100:700] = np.inf
+    frames[1] -= 75.0                           # negative values: kept, sigma uses max(s, 0)
+    frames[1, 700:710, :] = -np.inf
+    frames[2] += 1.0e6
+    I, S, D = plan.integrate_stack(frames, errors=True, azimuthal_std=True)
+    for k in range(3):
+        i, s, std = O.azimuthal_std(d["lut"], d["mask"], d["norm"], frames[k], errors=True)
+        assert np.array_equal(np.isnan(std), np.isnan(D[k])), k
+        empty = np.isnan(std)
+        assert np.all(I[k][empty] == 0) and np.all(S[k][empty] == 0), k
+        ok = ~empty
+        for name, a, b in (("I", I[k], i), ("sigma", S[k], s), ("std", D[k], std)):
+            rel = np.abs(a[ok] - b[ok]) / np.maximum(np.abs(b[ok]), 1e-300)
+            assert rel.max() <= 1e-6, (k, name, rel.max())
+    assert np.isnan(D[0]).sum() > 10  # the NaN rows empty whole bins
+
+
 def test_azimuthal_std_batch_equals_single_frame_bitwise(pilatus):
-    """The spread kernel reads the table once for 4 frames; each frame's
-    results are the bits it gets alone, for batches that fill 4-frame groups
-    and ones that do not (7 = 4 + 3), including frames with NaN and
-    infinite pixels."""
+    """The spread kernel stages 8 frames per pass; each frame's results are
+    the bits it gets alone, for batches that fill a pass and ones that do
+    not, including frames with NaN and infinite pixels."""
     d = pilatus
     plan = GpuPlan(d["poni"], d["w"]["shape"], n_radial=d["w"]["n_radial"],
                    polarization_factor=0.99, mask=d["mask"], precision="f32_wide")
-    frames = np.random.default_rng(25).poisson(90.0, size=(7,) + d["w"]["shape"]).astype(
+    frames = np.random.default_rng(25).poisson(90.0, size=(11,) + d["w"]["shape"]).astype(
         np.float32)
     frames[2, :40, :] = np.nan
     frames[5, 100:110, 200:300] = np.inf
+    frames[9, 300:340, :] = np.nan
     I, S, D = plan.integrate_stack(frames, errors=True, azimuthal_std=True)
-    for k in range(7):
+    for k in range(11):
         i, s, dd = plan.integrate_stack(frames[k:k + 1], errors=True, azimuthal_std=True)
         assert np.array_equal(I[k], i[0]) and np.array_equal(S[k], s[0])
         assert np.array_equal(D[k], dd[0], equal_nan=True)
