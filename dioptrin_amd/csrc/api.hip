@@ -91,7 +91,8 @@ size_t plan_device_bytes(const dio_plan* p) {
   return tb(p->b.full) + (p->b.has_exec ? tb(p->b.exec) : 0) + p->b.norm.bytes() +
          p->b.den.bytes() + p->b.mask.bytes() + p->tt.bytes() + p->sc.part.bytes() +
          p->stage.bytes() + p->mstage.bytes() + p->dyn_x.bytes() + p->dyn_n.bytes() +
-         p->dyn_d.bytes() + p->out_i.bytes() + p->out_s.bytes() + p->out_d.bytes();
+         p->dyn_d.bytes() + p->norm32.bytes() + p->dyn_m.bytes() + p->dyn_r.bytes() +
+         p->out_i.bytes() + p->out_s.bytes() + p->out_d.bytes();
 }
 
 dio_plan* make_plan(dio_ctx* ctx, const dio_poni& poni, const dio_plan_desc& desc) {
@@ -444,6 +445,33 @@ static void run_masked(dio_plan* p, const void* frames, int dtype, uint64_t n_fr
   const bool f64 = p->desc.precision == DIO_F64;
   const int ndt = f64 ? DIO_F64T : DIO_F32T;
   const uint64_t chunk = std::min<uint64_t>(n_frames, 64);
+  // Frames per lane of the masked-row kernel: 2 with the variance sums (4
+  // needs more registers than the kernel holds), else the plan's.
+  const uint32_t lanes = var ? 2 : p->tile == 16 ? p->lanes : 4;
+  if (masked_rows_supported(p->tt, dtype, p->desc.precision, lanes)) {
+    // The masks folded into the tile kernel's staging: 8 frames and their
+    // masked normalization per 16-row pass, no masked copies in HBM.
+    const uint32_t FP = dio::masked_rows_frames_per_pass();
+    const uint64_t npass = (chunk + FP - 1) / FP;
+    if (p->norm32.n < npix) {
+      p->norm32.alloc(npix);
+      dio::norm_to_f32(p->st.norm, npix, p->norm32.p, s);
+    }
+    const uint64_t mstride = dio::mask_bits_stride(npix);
+    if (p->dyn_m.n < npass * mstride) p->dyn_m.alloc(npass * mstride);
+    if (p->dyn_r.n < 2 * npass * 16 * nb) p->dyn_r.alloc(2 * npass * 16 * nb);
+    const char* src = static_cast<const char*>(frames);
+    for (uint64_t f0 = 0; f0 < n_frames; f0 += chunk) {
+      const uint64_t nf = std::min<uint64_t>(chunk, n_frames - f0);
+      dio::mask_bits(masks + f0 * npix, npix, (uint32_t)nf, p->dyn_m.p, s);
+      double* rs = p->dyn_r.p;
+      double* rv = p->dyn_r.p + npass * 16 * nb;
+      tiled_gather_masked(p->tt, p->sc, src + f0 * npix * esz, dtype, p->dyn_m.p, mstride,
+                          p->norm32.p, nf, var, p->desc.precision, lanes, rs, rv, s);
+      dio::masked_rows_finish(rs, rv, nf, nb, I + f0 * nb, var ? S + f0 * nb : nullptr, s);
+    }
+    return;
+  }
   if (p->dyn_x.n < chunk * npix * esz) p->dyn_x.alloc(chunk * npix * esz);
   if (p->dyn_n.n < chunk * npix * (f64 ? 8 : 4)) p->dyn_n.alloc(chunk * npix * (f64 ? 8 : 4));
   if (p->dyn_d.n < chunk * nb) p->dyn_d.alloc(chunk * nb);

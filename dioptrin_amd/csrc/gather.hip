@@ -114,7 +114,107 @@ __global__ __launch_bounds__(256) void k_masked_finish(double* I, double* S, con
   }
 }
 
+// One byte per pixel and 8-frame pass of the masks of n frames: bit i of
+// byte [h][p] (plane h at h * stride) is frame 8 h + i's mask at pixel p
+// (input order). A thread packs 16 consecutive pixels: per frame it reads
+// the 5 aligned words that cover them and takes the 16 bytes out with byte
+// alignment; a group whose words would reach outside the masks, and a
+// frame's last partial group, are read byte by byte.
+__global__ __launch_bounds__(256) void k_mask_bits(const uint8_t* __restrict__ masks,
+                                                   uint64_t npix, uint32_t n, uint64_t stride,
+                                                   uint8_t* out) {
+  const uint32_t h = blockIdx.y;
+  const uint32_t f0 = 8 * h, nf = min(8u, n - f0);
+  const uint64_t total = (uint64_t)n * npix, groups = (npix + 15) / 16;
+  const uintptr_t base = reinterpret_cast<uintptr_t>(masks);
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < groups;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t p0 = 16 * j;
+    uint32_t bits[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) bits[k] = 0;
+    for (uint32_t i = 0; i < nf; i++) {
+      const uint64_t b = (uint64_t)(f0 + i) * npix + p0;
+      const uintptr_t aw = (base + b) & ~(uintptr_t)3;  // the aligned word holding byte b
+      if (p0 + 16 <= npix && aw >= base && aw + 20 <= base + total) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(aw);
+        const uint32_t sh = (uint32_t)((base + b) & 3) * 8;
+        uint32_t v[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) v[k] = w[k];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const uint32_t x = sh ? (v[k] >> sh) | (v[k + 1] << (32 - sh)) : v[k];
+#pragma unroll
+          for (int c = 0; c < 4; c++) bits[4 * k + c] |= ((x >> (8 * c)) & 0xffu ? 1u : 0u) << i;
+        }
+      } else {
+        for (uint32_t k = 0; k < 16 && p0 + k < npix; k++)
+          bits[k] |= (masks[b + k] != 0 ? 1u : 0u) << i;
+      }
+    }
+    uint8_t* o = out + (uint64_t)h * stride + p0;
+    if (p0 + 16 <= stride) {  // planes are padded to 16 bytes: whole 16-byte stores
+      uint4 q;
+      q.x = bits[0] | bits[1] << 8 | bits[2] << 16 | bits[3] << 24;
+      q.y = bits[4] | bits[5] << 8 | bits[6] << 16 | bits[7] << 24;
+      q.z = bits[8] | bits[9] << 8 | bits[10] << 16 | bits[11] << 24;
+      q.w = bits[12] | bits[13] << 8 | bits[14] << 16 | bits[15] << 24;
+      *reinterpret_cast<uint4*>(o) = q;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_norm32(const double* __restrict__ norm, uint64_t n,
+                                                float* out) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x)
+    out[p] = (float)norm[p];
+}
+
+// Frame k of masked-row passes (pass k / 8, row k % 8 holds its sums, row
+// 8 + k % 8 its denominator): the arithmetic of k_masked_finish.
+__global__ __launch_bounds__(256) void k_masked_rows_finish(const double* R, const double* V,
+                                                            uint64_t n_frames, uint32_t nbins,
+                                                            double* I, double* S) {
+  const uint64_t n = n_frames * nbins;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = j / nbins, b = j % nbins, h = k / 8, r = k % 8;
+    const double x = R[(16 * h + r) * nbins + b], d = R[(16 * h + 8 + r) * nbins + b];
+    I[j] = d == 0.0 ? 0.0 : x / d;
+    if (S) S[j] = d == 0.0 ? 0.0 : sqrt(fmax(V[(16 * h + r) * nbins + b] / (d * d), 0.0));
+  }
+}
+
 }  // namespace
+
+uint64_t mask_bits_stride(uint64_t npix) { return (npix + 15) / 16 * 16; }
+
+void mask_bits(const uint8_t* masks, uint64_t npix, uint32_t n_frames, uint8_t* out,
+               hipStream_t s) {
+  if (n_frames == 0) return;
+  const uint64_t groups = (npix + 15) / 16;
+  const dim3 grid((uint32_t)std::min<uint64_t>((groups + 255) / 256, 2048), (n_frames + 7) / 8);
+  k_mask_bits<<<grid, 256, 0, s>>>(masks, npix, n_frames, mask_bits_stride(npix), out);
+  DIO_HIP(hipGetLastError());
+}
+
+void norm_to_f32(const double* norm, uint64_t n, float* out, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+  k_norm32<<<blocks, 256, 0, s>>>(norm, n, out);
+  DIO_HIP(hipGetLastError());
+}
+
+void masked_rows_finish(const double* rows_sum, const double* rows_var, uint64_t n_frames,
+                        uint32_t nbins, double* intensity, double* sigma, hipStream_t s) {
+  const uint64_t n = n_frames * nbins;
+  if (n == 0) return;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+  k_masked_rows_finish<<<blocks, 256, 0, s>>>(rows_sum, rows_var, n_frames, nbins, intensity,
+                                              sigma);
+  DIO_HIP(hipGetLastError());
+}
 
 void masked_frames(const StdTable& t, const void* frames, int dtype, const uint8_t* masks,
                    uint64_t n_frames, bool errors, double* intensity, double* sigma,

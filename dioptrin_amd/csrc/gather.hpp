@@ -65,4 +65,17 @@ void masked_prepare(const StdTable& t, const void* frames, int dtype, const uint
 void masked_finish(double* intensity, double* sigma, const double* den, uint64_t n, bool errors,
                    hipStream_t s);
 
+// Masks folded into the tile kernel's staging (tiled_gather_masked): the
+// masks' bit bytes per 8-frame pass ([pass][npix]), the normalization as
+// float32, and the division of the passes' row sums: I = sum w x / D,
+// sigma = sqrt(max(V / D^2, 0)), 0 where D is 0 (as masked_finish).
+// Planes of mask_bits are mask_bits_stride(npix) bytes apart (a multiple of
+// 16, out must be 16-byte aligned).
+uint64_t mask_bits_stride(uint64_t npix);
+void mask_bits(const uint8_t* masks, uint64_t npix, uint32_t n_frames, uint8_t* out,
+               hipStream_t s);
+void norm_to_f32(const double* norm, uint64_t n, float* out, hipStream_t s);
+void masked_rows_finish(const double* rows_sum, const double* rows_var, uint64_t n_frames,
+                        uint32_t nbins, double* intensity, double* sigma, hipStream_t s);
+
 }  // namespace dio

@@ -34,6 +34,11 @@ struct dio_plan {
   // per-frame denominators of one chunk.
   dio::DevBuf<uint8_t> dyn_x, dyn_n;
   dio::DevBuf<double> dyn_d;
+  // Dynamic masks folded into the tile kernel: the normalization as float32,
+  // one chunk's mask bits and the passes' row sums.
+  dio::DevBuf<float> norm32;
+  dio::DevBuf<uint8_t> dyn_m;
+  dio::DevBuf<double> dyn_r;
   dio::DevBuf<double> out_i, out_s, out_d;
   ~dio_plan() {
     if (stream) (void)hipStreamDestroy(stream);

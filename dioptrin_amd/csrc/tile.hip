@@ -783,6 +783,7 @@ template <int F, int MODE> struct PlanDb {
 
 template <int F> struct DbFrames {
   uint32_t v[F / 2][2];
+  uint32_t mk[2];  // masked rows: the pair's mask bit bytes (pixel 0, pixel 1)
 };
 
 // Arguments of pass h < npass of a launch.
@@ -846,10 +847,103 @@ __device__ __forceinline__ void db_store(const TileArgs& a, const DbFrames<F>& f
   else db_store_as<F, float, LT>(fr, fl);
 }
 
+// Masked rows (see TileArgs::mbits): pass h of a launch holds frames 8 h ..
+// 8 h + 7 (the last, nfr of them) and their masks' bit byte plane h.
+constexpr uint32_t kMaskedFrames = 8;
+
+template <int F>
+__device__ __forceinline__ TileArgs db_pass_masked(const TileArgs& a, uint32_t h) {
+  TileArgs r = a;
+  r.frames = static_cast<const unsigned char*>(a.frames) +
+             (uint64_t)h * kMaskedFrames * a.npix * 4u;
+  r.mbits = a.mbits + (uint64_t)h * a.mbits_stride;
+  r.nfr = h + 1 == a.npass ? a.nfr : kMaskedFrames;
+  r.nf = (uint32_t)F;
+  r.part = static_cast<unsigned char*>(a.part) + h * a.pblock;
+  return r;
+}
+
+// Threads 0-511 load the pair's 8 frames (frames past nfr read the pass's
+// last one), threads 512-1023 the pair's float32 normalization (detector
+// order); both the pair's mask bits. Unconditional loads from valid
+// addresses, as in db_load.
+template <int F, int A>
+__device__ __forceinline__ void db_load_masked(const TileArgs& a, uint32_t t, DbFrames<F>& fr) {
+  static_assert(A == 4 || A == 8, "pair loads are 4 or 8 bytes");
+  static_assert(F == 2 * kMaskedFrames, "8 frames and their normalization");
+  const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
+  const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
+  const uint32_t c = (t % a.ntx) * kTileCols + cp * 2;
+  const uint32_t r = (t / a.ntx) * kTileRows + r0;
+  const uint32_t nvalid = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
+  const uint32_t off = (a.flip && nvalid ? a.dim1 - 1 - r : r) * a.dim2 + c;
+  const uint32_t o0 = nvalid > 0 ? off : 0u, o1 = nvalid > 1 ? off + 1 : 0u;
+  // Kept apart until the store: combining them here would wait for these
+  // loads before the frames' loads are issued.
+  fr.mk[0] = a.mbits[o0];
+  fr.mk[1] = a.mbits[o1];
+  if (half == 0) {
+    const unsigned char* src = static_cast<const unsigned char*>(a.frames);
+#pragma unroll
+    for (uint32_t i = 0; i < kMaskedFrames; i++) {
+      const unsigned char* pf = src + (uint64_t)min(i, a.nfr - 1) * a.npix * 4u;
+      if constexpr (A == 8) {
+        const u32x2 v = load_at<u32x2>(pf, o0 * 4u);
+        fr.v[i][0] = v.x;
+        fr.v[i][1] = v.y;
+      } else {
+        fr.v[i][0] = load_at<uint32_t>(pf, o0 * 4u);
+        fr.v[i][1] = load_at<uint32_t>(pf, o1 * 4u);
+      }
+    }
+  } else {
+    const uint32_t d0 = nvalid > 0 ? r * a.dim2 + c : 0u, d1 = nvalid > 1 ? d0 + 1 : 0u;
+    fr.v[0][0] = __float_as_uint(a.norm32[d0]);
+    fr.v[0][1] = __float_as_uint(a.norm32[d1]);
+  }
+}
+
+// Rows 0-7: the frames with masked pixels set to 0; rows 8-15: the
+// normalization with masked pixels set to 0.
+template <int F, class W, class LT>
+__device__ __forceinline__ void db_store_masked_as(const DbFrames<F>& fr, LT* fl) {
+  const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
+  const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
+  const uint32_t px = r0 * kTileCols + cp * 2;
+  const uint32_t m = fr.mk[0] | (fr.mk[1] << 8);
+  if (half == 0) {
+#pragma unroll
+    for (uint32_t i = 0; i < kMaskedFrames; i++) {
+      Lds2<LT> o;
+      o.v[0] = (m >> i) & 1u ? LT(0) : word_to<W, LT>(fr.v[i][0]);
+      o.v[1] = (m >> (8 + i)) & 1u ? LT(0) : word_to<W, LT>(fr.v[i][1]);
+      *reinterpret_cast<Lds2<LT>*>(fl + i * kLdsStride + px) = o;
+    }
+  } else {
+    const LT n0 = static_cast<LT>(__uint_as_float(fr.v[0][0]));
+    const LT n1 = static_cast<LT>(__uint_as_float(fr.v[0][1]));
+#pragma unroll
+    for (uint32_t i = 0; i < kMaskedFrames; i++) {
+      Lds2<LT> o;
+      o.v[0] = (m >> i) & 1u ? LT(0) : n0;
+      o.v[1] = (m >> (8 + i)) & 1u ? LT(0) : n1;
+      *reinterpret_cast<Lds2<LT>*>(fl + (kMaskedFrames + i) * kLdsStride + px) = o;
+    }
+  }
+}
+
+template <int F, class LT>
+__device__ __forceinline__ void db_store_masked(const TileArgs& a, const DbFrames<F>& fr,
+                                                LT* fl) {
+  if (a.dtype == DIO_U32) db_store_masked_as<F, uint32_t, LT>(fr, fl);
+  else if (a.dtype == DIO_I32) db_store_masked_as<F, int32_t, LT>(fr, fl);
+  else db_store_masked_as<F, float, LT>(fr, fl);
+}
+
 // Loads item (tile tt, pass arguments ap): frames and, for the first pass of
 // a tile, its segment bounds and first table chunk (returns its groups). The
 // later passes of the tile keep the bounds already in sr.
-template <int F, int A, int MODE, uint32_t C, uint32_t SLOTS>
+template <int F, int A, int MODE, uint32_t C, uint32_t SLOTS, bool MASKED = false>
 __device__ __forceinline__ uint32_t db_prefetch(const TileArgs& ap, uint32_t tt, uint4 m,
                                                 bool first, uint32_t slot, DbFrames<F>& fr,
                                                 TableRegs<1, MODE>& tr,
@@ -857,12 +951,13 @@ __device__ __forceinline__ uint32_t db_prefetch(const TileArgs& ap, uint32_t tt,
   if (m.x == m.y) return 0;
   const uint32_t n = min(m.w - m.z, C);
   if (first) seg_bounds<SLOTS>(ap, m, slot, sr);
-  db_load<F, A>(ap, tt, fr);
+  if constexpr (MASKED) db_load_masked<F, A>(ap, tt, fr);
+  else db_load<F, A>(ap, tt, fr);
   if (first) table_load<1, MODE, kDbThreads>(ap, m.z, m.z + n, tr);
   return n;
 }
 
-template <int F, int G, int MODE, bool VAR, int FA>
+template <int F, int G, int MODE, bool VAR, int FA, bool MASKED = false>
 __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_db(
     TileArgs a) {
   static_assert(FA == 4 || FA == 8, "32-bit frames only");
@@ -914,6 +1009,11 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   auto table_buf = [&](const Item& it) {
     return lds_raw + PL::kTable0 + it.tbuf * PL::kTableBytes;
   };
+  // Arguments of pass h of the launch.
+  auto pass_args = [&](uint32_t h) {
+    if constexpr (MASKED) return db_pass_masked<F>(a, h);
+    else return db_pass<F>(a, h);
+  };
   MetaLanes ml;
   ml.load(a, t0, grid, 0);
   // Metadata of an item (zero past the last one).
@@ -939,10 +1039,12 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   Item iA = item(0);
   if (!iA.valid) return;
   uint4 mA = meta(0, iA);
-  uint32_t pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, iA.pass), iA.tile, mA, true,
-                                                   slot, fr, tr, sr);
+  uint32_t pn = db_prefetch<F, FA, MODE, C, SLOTS, MASKED>(pass_args(iA.pass), iA.tile, mA,
+                                                           true, slot, fr, tr, sr);
   if (mA.x != mA.y) {
-    db_store<F, LT>(a, fr, reinterpret_cast<LT*>(frames_buf(0)));
+    LT* fb0 = reinterpret_cast<LT*>(frames_buf(0));
+    if constexpr (MASKED) db_store_masked<F, LT>(a, fr, fb0);
+    else db_store<F, LT>(a, fr, fb0);
     table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(iA)),
                                      table_buf(iA) + (size_t)C * 8);
   }
@@ -950,8 +1052,8 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   Item iB = item(1);
   uint4 mB = meta(1, iB);
   if (iB.valid)
-    pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, iB.pass), iB.tile, mB, iB.first,
-                                            slot, fr, tr, sr);
+    pn = db_prefetch<F, FA, MODE, C, SLOTS, MASKED>(pass_args(iB.pass), iB.tile, mB,
+                                                    iB.first, slot, fr, tr, sr);
   __syncthreads();
 
   for (uint32_t i = 0;; i++) {
@@ -960,7 +1062,9 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
     //    last barrier, and its table chunk if it opens a tile (the other
     //    table buffer, last read by the previous tile).
     if (mB.x != mB.y) {
-      db_store<F, LT>(a, fr, reinterpret_cast<LT*>(frames_buf(i + 1)));
+      LT* fb1 = reinterpret_cast<LT*>(frames_buf(i + 1));
+      if constexpr (MASKED) db_store_masked<F, LT>(a, fr, fb1);
+      else db_store<F, LT>(a, fr, fb1);
       if (iB.first)
         table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(iB)),
                                          table_buf(iB) + (size_t)C * 8);
@@ -971,13 +1075,13 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
     // iteration also waits for the partial stores issued next.
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     // 2. Partials of item i - 1.
-    pending_write<F>(db_pass<F>(a, ph), q, pd);
+    pending_write<F>(pass_args(ph), q, pd);
     // 3. Item i + 2 into the registers.
     const Item iC = item(i + 2);
     const uint4 mC = meta(i + 2, iC);
     if (iC.valid)
-      pn = db_prefetch<F, FA, MODE, C, SLOTS>(db_pass<F>(a, iC.pass), iC.tile, mC, iC.first,
-                                              slot, fr, tr, sr);
+      pn = db_prefetch<F, FA, MODE, C, SLOTS, MASKED>(pass_args(iC.pass), iC.tile, mC,
+                                                      iC.first, slot, fr, tr, sr);
     // 4. Item i from its buffers. A later pass of a tile whose table takes
     //    several chunks reloads the first (the last chunk is in LDS).
     if (mA.x != mA.y) {
@@ -988,7 +1092,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
       // iteration's length: their wavefronts issue first while they compute.
       if (threadIdx.x < kDbThreads / 2) __builtin_amdgcn_s_setprio(2);
       tile_apply<F, G, MODE, VAR, C, 1, SLOTS, kDbThreads>(
-          db_pass<F>(a, iA.pass), mA, cA, slot, q, base, reinterpret_cast<u32x2*>(tb),
+          pass_args(iA.pass), mA, cA, slot, q, base, reinterpret_cast<u32x2*>(tb),
           tb + (size_t)C * 8, &pd, first_in_lds);
       __builtin_amdgcn_s_setprio(0);
     }
@@ -1000,7 +1104,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
     iB = iC;
     mB = mC;
   }
-  pending_write<F>(db_pass<F>(a, ph), q, pd);
+  pending_write<F>(pass_args(ph), q, pd);
 }
 
 // prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =
@@ -1172,7 +1276,7 @@ void launch_tile(const TiledTable& t, const TileArgs& a, hipStream_t s) {
   k_tile<F, G, MODE, VAR, FA><<<grid, PL::kThreads, lds, s>>>(a);
 }
 
-template <int F, int G, int MODE, bool VAR, int FA>
+template <int F, int G, int MODE, bool VAR, int FA, bool MASKED = false>
 void launch_tile_db(const TiledTable& t, const TileArgs& a, hipStream_t s) {
   using PL = PlanDb<F, MODE>;
   constexpr size_t lds = PL::kBytes;
@@ -1181,13 +1285,31 @@ void launch_tile_db(const TiledTable& t, const TileArgs& a, hipStream_t s) {
   DIO_HIP(hipGetDevice(&dev));
   const uint64_t bit = 1ull << (dev & 63);
   if (!(done.load() & bit)) {
-    DIO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_db<F, G, MODE, VAR, FA>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DIO_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_tile_db<F, G, MODE, VAR, FA, MASKED>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     done.fetch_or(bit);
   }
   uint32_t grid = std::min<uint32_t>(t.ntiles, t.cus);
   if (t.max_wgs) grid = std::min(grid, t.max_wgs);
-  k_tile_db<F, G, MODE, VAR, FA><<<grid, kDbThreads, lds, s>>>(a);
+  k_tile_db<F, G, MODE, VAR, FA, MASKED><<<grid, kDbThreads, lds, s>>>(a);
+}
+
+// Masked rows (TileArgs::mbits): 16-row passes, frames-per-lane G, float32
+// weights, with or without the variance sums.
+template <int G, bool VAR>
+void launch_masked(const TiledTable& t, const TileArgs& a, int precision, hipStream_t s) {
+  auto go = [&](auto mode) {
+    constexpr int M = decltype(mode)::value;
+    if (a.aligned8) launch_tile_db<16, G, M, VAR, 8, true>(t, a, s);
+    else launch_tile_db<16, G, M, VAR, 4, true>(t, a, s);
+  };
+  switch (precision) {
+    case DIO_F32: go(std::integral_constant<int, 0>{}); break;
+    case DIO_F32_COMPENSATED: go(std::integral_constant<int, 1>{}); break;
+    case DIO_F32_WIDE: go(std::integral_constant<int, 3>{}); break;
+    default: fail(DIO_ERR_INVALID, "masked rows need float32 weights");
+  }
 }
 
 template <int F, int G, int MODE>
@@ -1525,6 +1647,67 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
         if (var) launch_final<false, true, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
         else launch_final<false, false, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
       }
+    }
+    DIO_HIP(hipGetLastError());
+  }
+}
+
+bool masked_rows_supported(const TiledTable& t, int dtype, int precision, uint32_t G) {
+  return double_buffered(t, 16, dtype, precision) && valid_tile(precision, 16, G);
+}
+
+uint32_t masked_rows_frames_per_pass() { return kMaskedFrames; }
+
+void tiled_gather_masked(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
+                         const uint8_t* mbits, uint64_t mbits_stride, const float* norm32,
+                         uint64_t n_frames, bool var, int precision, uint32_t G,
+                         double* rows_sum, double* rows_var, hipStream_t s) {
+  if (!masked_rows_supported(t, dtype, precision, G))
+    fail(DIO_ERR_INVALID, "masked rows need 32-bit frames and 16-row passes");
+  constexpr uint32_t F = 16;
+  const uint64_t npix = (uint64_t)t.dim1 * t.dim2;
+  const char* fb = static_cast<const char*>(frames);
+  const uint32_t P = kMaxPassesPerLaunch;
+  const size_t pbytes = pass_bytes(t, F, var, precision);
+  sc.ensure(t, F, var);
+  const uint64_t BS = (uint64_t)t.nseg * F * (var ? 2 : 1);  // elements per pass block
+  const bool plain = precision == DIO_F32;
+  for (uint64_t g0 = 0; g0 < n_frames; g0 += (uint64_t)P * kMaskedFrames) {
+    const uint32_t ng = (uint32_t)std::min<uint64_t>((uint64_t)P * kMaskedFrames, n_frames - g0);
+    const uint32_t np = (ng + kMaskedFrames - 1) / kMaskedFrames;
+    const uint64_t pass0 = g0 / kMaskedFrames;
+    const void* fr = fb + g0 * npix * 4;
+    const uint32_t al8 = (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % 8 == 0) ? 1u : 0u;
+    TileArgs a{fr,     dtype,   t.lidx.p, t.w32.p, t.w64.p, t.seg_group.p,
+               t.tile_meta.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
+               npix,   F,       al8,      sc.ptr()};
+    a.flip = t.flip ? 1u : 0u;
+    a.npass = np;
+    a.nf_last = F;
+    a.pblock = pbytes;
+    a.mbits = mbits + pass0 * mbits_stride;
+    a.mbits_stride = mbits_stride;
+    a.norm32 = norm32;
+    a.nfr = ng - (np - 1) * kMaskedFrames;
+    if (t.nseg) {
+      switch (G * 2 + (var ? 1 : 0)) {
+        case 2: launch_masked<1, false>(t, a, precision, s); break;
+        case 3: launch_masked<1, true>(t, a, precision, s); break;
+        case 4: launch_masked<2, false>(t, a, precision, s); break;
+        case 5: launch_masked<2, true>(t, a, precision, s); break;
+        case 8: launch_masked<4, false>(t, a, precision, s); break;
+        case 9: launch_masked<4, true>(t, a, precision, s); break;
+        default: fail(DIO_ERR_INVALID, "unsupported frames per lane");
+      }
+    }
+    double* I = rows_sum + pass0 * F * t.nbins;
+    double* V = var ? rows_var + pass0 * F * t.nbins : nullptr;
+    if (plain) {
+      if (var) launch_final<true, true, float>(t, sc.ptr(), F, BS, np * F, 32, true, I, V, s);
+      else launch_final<true, false, float>(t, sc.ptr(), F, BS, np * F, 32, true, I, V, s);
+    } else {
+      if (var) launch_final<false, true, float>(t, sc.ptr(), F, BS, np * F, 32, true, I, V, s);
+      else launch_final<false, false, float>(t, sc.ptr(), F, BS, np * F, 32, true, I, V, s);
     }
     DIO_HIP(hipGetLastError());
   }

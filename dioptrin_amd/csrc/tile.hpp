@@ -122,6 +122,22 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
                   uint32_t frames_per_lane, double* intensity, double* sigma, hipStream_t s,
                   LaunchEvents* launches = nullptr, bool raw = false, uint32_t max_passes = 0);
 
+// Per-frame masks folded into the double-buffered kernel's staging: passes
+// of 16 rows, rows 0-7 the pass's 8 frames with masked pixels read as 0 and
+// rows 8-15 the plan's float32 normalization (norm32, detector order) with
+// the same pixels set to 0. mbits: one byte per pixel and pass (bit i:
+// frame i of the pass masked), input order, passes mbits_stride bytes apart.
+// rows_sum / rows_var receive, per pass, the 16 rows' raw sums (sum w x and
+// sum w^2 max(x, 0), the latter only with var; the normalization rows' sums
+// are the frames' denominators) as [pass][16][nbins]. Needs 32-bit frames and
+// float32 weights (masked_rows_supported).
+bool masked_rows_supported(const TiledTable& t, int dtype, int precision, uint32_t lanes);
+uint32_t masked_rows_frames_per_pass();
+void tiled_gather_masked(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
+                         const uint8_t* mbits, uint64_t mbits_stride, const float* norm32,
+                         uint64_t n_frames, bool var, int precision, uint32_t lanes,
+                         double* rows_sum, double* rows_var, hipStream_t s);
+
 // Only the tile kernel for one pass of nf <= frames_per_pass frames (timing).
 void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                      uint32_t nf, bool var, int precision, uint32_t frames_per_pass,

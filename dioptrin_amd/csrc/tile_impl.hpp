@@ -117,6 +117,17 @@ struct TileArgs {
   uint32_t npass = 1;
   uint32_t nf_last = 0;
   uint64_t pblock = 0;
+  // Per-frame masks folded into the staging (double-buffered kernel, masked
+  // rows): a pass of 16 rows holds 8 frames (rows 0-7, masked pixels read as
+  // 0) and their masked float32 normalization (rows 8-15). mbits holds one
+  // byte per pixel and pass (bit i: frame i of the pass is masked there),
+  // input order, the passes mbits_stride bytes apart; norm32 the plan's
+  // normalization as float32, detector order; nfr the frames of the last
+  // pass (8 for the others).
+  const uint8_t* mbits = nullptr;
+  const float* norm32 = nullptr;
+  uint64_t mbits_stride = 0;
+  uint32_t nfr = 0;
 };
 
 // Segment (relative to the chunk's first) a slot takes in round r. Segments
