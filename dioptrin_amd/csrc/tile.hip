@@ -783,7 +783,7 @@ template <int F, int MODE> struct PlanDb {
 
 template <int F> struct DbFrames {
   uint32_t v[F / 2][2];
-  uint32_t mk[2];  // masked rows: the pair's mask bit bytes (pixel 0, pixel 1)
+  uint32_t mk0, mk1;  // masked rows: the pair's mask bit bytes (pixel 0, pixel 1)
 };
 
 // Arguments of pass h < npass of a launch.
@@ -865,8 +865,9 @@ __device__ __forceinline__ TileArgs db_pass_masked(const TileArgs& a, uint32_t h
 
 // Threads 0-511 load the pair's 8 frames (frames past nfr read the pass's
 // last one), threads 512-1023 the pair's float32 normalization (detector
-// order); both the pair's mask bits. Unconditional loads from valid
-// addresses, as in db_load.
+// order; the same words for all 8 slots, so both halves run the same loads
+// and fill the same registers); both the pair's mask bits. Unconditional
+// loads from valid addresses, as in db_load.
 template <int F, int A>
 __device__ __forceinline__ void db_load_masked(const TileArgs& a, uint32_t t, DbFrames<F>& fr) {
   static_assert(A == 4 || A == 8, "pair loads are 4 or 8 bytes");
@@ -878,29 +879,28 @@ __device__ __forceinline__ void db_load_masked(const TileArgs& a, uint32_t t, Db
   const uint32_t nvalid = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
   const uint32_t off = (a.flip && nvalid ? a.dim1 - 1 - r : r) * a.dim2 + c;
   const uint32_t o0 = nvalid > 0 ? off : 0u, o1 = nvalid > 1 ? off + 1 : 0u;
-  // Kept apart until the store: combining them here would wait for these
-  // loads before the frames' loads are issued.
-  fr.mk[0] = a.mbits[o0];
-  fr.mk[1] = a.mbits[o1];
-  if (half == 0) {
-    const unsigned char* src = static_cast<const unsigned char*>(a.frames);
+  const uint32_t d0 = nvalid > 0 ? r * a.dim2 + c : 0u, d1 = nvalid > 1 ? d0 + 1 : 0u;
+  const uint32_t p0 = half ? d0 : o0, p1 = half ? d1 : o1;
+  const uint64_t step = half ? 0u : (uint64_t)a.npix * 4u;
+  const unsigned char* src =
+      half ? reinterpret_cast<const unsigned char*>(a.norm32)
+           : static_cast<const unsigned char*>(a.frames);
 #pragma unroll
-    for (uint32_t i = 0; i < kMaskedFrames; i++) {
-      const unsigned char* pf = src + (uint64_t)min(i, a.nfr - 1) * a.npix * 4u;
-      if constexpr (A == 8) {
-        const u32x2 v = load_at<u32x2>(pf, o0 * 4u);
-        fr.v[i][0] = v.x;
-        fr.v[i][1] = v.y;
-      } else {
-        fr.v[i][0] = load_at<uint32_t>(pf, o0 * 4u);
-        fr.v[i][1] = load_at<uint32_t>(pf, o1 * 4u);
-      }
+  for (uint32_t i = 0; i < kMaskedFrames; i++) {
+    const unsigned char* pf = src + (uint64_t)min(i, a.nfr - 1) * step;
+    if constexpr (A == 8) {
+      const u32x2 v = load_at<u32x2>(pf, p0 * 4u);
+      fr.v[i][0] = v.x;
+      fr.v[i][1] = v.y;
+    } else {
+      fr.v[i][0] = load_at<uint32_t>(pf, p0 * 4u);
+      fr.v[i][1] = load_at<uint32_t>(pf, p1 * 4u);
     }
-  } else {
-    const uint32_t d0 = nvalid > 0 ? r * a.dim2 + c : 0u, d1 = nvalid > 1 ? d0 + 1 : 0u;
-    fr.v[0][0] = __float_as_uint(a.norm32[d0]);
-    fr.v[0][1] = __float_as_uint(a.norm32[d1]);
   }
+  // Kept apart until the store: combining them here would wait for these
+  // loads before the next item's loads are issued.
+  fr.mk0 = a.mbits[o0];
+  fr.mk1 = a.mbits[o1];
 }
 
 // Rows 0-7: the frames with masked pixels set to 0; rows 8-15: the
@@ -910,7 +910,7 @@ __device__ __forceinline__ void db_store_masked_as(const DbFrames<F>& fr, LT* fl
   const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
   const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
   const uint32_t px = r0 * kTileCols + cp * 2;
-  const uint32_t m = fr.mk[0] | (fr.mk[1] << 8);
+  const uint32_t m = fr.mk0 | (fr.mk1 << 8);
   if (half == 0) {
 #pragma unroll
     for (uint32_t i = 0; i < kMaskedFrames; i++) {
