@@ -683,7 +683,8 @@ def oracle_dynamic(d, frame, mask, errors=True):
 
 DYN_CASES = [(np.float32, "plan_mask", "f32_wide", 1e-10), (np.uint32, "plan_mask", "f64", 1e-10),
              (np.uint16, None, "f32_wide", 2e-6), (np.uint32, None, "f32", 2e-6),
-             (np.float32, None, "f64", 1e-12)]
+             (np.float32, None, "f64", 1e-12), (np.uint32, None, "f32_compensated", 2e-6),
+             (np.float32, None, "f32_wide", 2e-6), (np.int32, None, "f32_compensated", 2e-6)]
 
 
 @pytest.mark.parametrize("dtype,plan_mask,precision,tol", DYN_CASES,
@@ -718,6 +719,31 @@ def test_dynamic_masks_match_oracle(pilatus, dtype, plan_mask, precision, tol):
         assert np.all(I[k][z] == 0) and np.all(S[k][z] == 0)
         empty += int(z.sum())
     assert empty > 0   # the band masks empty some bins
+
+
+@pytest.mark.parametrize("precision", ["f32_compensated", "f32_wide"])
+def test_dynamic_masks_counting_table(pilatus, precision):
+    """Per-frame masks on a counting (unit-weight) table of an unmasked plan:
+    the tile route's masked staging against the oracle's strict gather on
+    the counting table compacted by each frame's mask."""
+    d = pilatus
+    print(f"case dynamic masks counting {precision}", flush=True)
+    w = d["w"]
+    lut = oracle_lut(d["poni"], w["shape"], w["n_radial"], "counting", "2th_rad", None)
+    n = 9
+    frames = np.stack([synth.frame_pilatus(s) for s in range(n)]).astype(np.uint32)
+    masks = dynamic_masks(d, n, 11)
+    plan = GpuPlan(d["poni"], w["shape"], method="counting", n_radial=w["n_radial"],
+                   polarization_factor=0.99, precision=precision)
+    I, S = plan.integrate_masked(frames, masks, errors=True)
+    for k in range(n):
+        fixed = O.fixed_table(lut, masks[k])
+        den = O.denominators(fixed, d["norm"])
+        i_ref, s_ref = O.gather(fixed, den, frames[k].astype(np.float64), True)
+        assert np.allclose(I[k], i_ref, rtol=2e-6, atol=2e-9 * np.abs(i_ref).max()), k
+        assert np.allclose(S[k], s_ref, rtol=2e-6, atol=2e-9 * np.abs(s_ref).max()), k
+        z = den == 0
+        assert np.all(I[k][z] == 0) and np.all(S[k][z] == 0)
 
 
 def test_dynamic_mask_equal_to_fixed_mask(pilatus):
