@@ -585,10 +585,10 @@ __device__ __forceinline__ void pending_write(const TileArgs& a, uint32_t q,
 // partials of those R segments are left in pd (the caller writes them)
 // instead of being stored here.
 template <int F, int G, int MODE, bool VAR, uint32_t C, int KT, uint32_t SLOTS, unsigned NT,
-          class LT>
+          class FV>
 __device__ __forceinline__ void tile_apply(
     const TileArgs& a, uint4 m, const SegRegs<SegRounds<SLOTS>::R>& cur, uint32_t slot,
-    uint32_t q, const LT* base, u32x2* tl, unsigned char* twb,
+    uint32_t q, const FV& base, u32x2* tl, unsigned char* twb,
     Pending<SegRounds<SLOTS>::R, G, MODE, VAR>* pd = nullptr, bool first_in_lds = true) {
   // Chunk bounds are workgroup-uniform: read them through the scalar cache so
   // that waiting for them never waits on the vector loads in flight.
@@ -687,7 +687,7 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * tile_threads(F) / 256))
   unsigned char* twb = lds_raw + PL::kFrameBytes + (size_t)C * 8;
 
   const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
-  const LT* base = fl + q * G * kLdsStride;
+  const FmView<LT> base{fl + q * G * kLdsStride};
 
   const uint32_t grid = gridDim.x;
   const uint4 none = make_uint4(0u, 0u, 0u, 0u);
@@ -826,18 +826,39 @@ __device__ __forceinline__ void db_load(const TileArgs& a, uint32_t t, DbFrames<
   }
 }
 
+// Stores the pair's values (frames or rows 8 h .. 8 h + 7 of both pixels,
+// v[i][k]: row 8 h + i of pixel k) in the quad-major layout (PmView): two
+// quads per pixel; neighbouring lanes store neighbouring pixel pairs, 32
+// bytes apart, so each store instruction covers every bank.
+__device__ __forceinline__ void pm_store_pair(float* fl, uint32_t px, uint32_t half,
+                                              const float (&v)[8][2]) {
+#pragma unroll
+  for (uint32_t k = 0; k < 2; k++) {
+#pragma unroll
+    for (uint32_t qd = 0; qd < 2; qd++) {
+      const f32x4 o = {v[4 * qd][k], v[4 * qd + 1][k], v[4 * qd + 2][k], v[4 * qd + 3][k]};
+      *reinterpret_cast<f32x4*>(fl + pm_word(px + k, 8u * half + 4u * qd)) = o;
+    }
+  }
+}
+
 template <int F, class W, class LT>
 __device__ __forceinline__ void db_store_as(const DbFrames<F>& fr, LT* fl) {
+  static_assert(F == 16 && std::is_same<LT, float>::value, "pixel-major 16-frame tiles");
   const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
   const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
+  float v[8][2];
 #pragma unroll
   for (int i = 0; i < F / 2; i++) {
-    const uint32_t f = half * (F / 2) + i;
-    Lds2<LT> o;
-    o.v[0] = word_to<W, LT>(fr.v[i][0]);
-    o.v[1] = word_to<W, LT>(fr.v[i][1]);
-    *reinterpret_cast<Lds2<LT>*>(fl + f * kLdsStride + r0 * kTileCols + cp * 2) = o;
+    v[i][0] = word_to<W, LT>(fr.v[i][0]);
+    v[i][1] = word_to<W, LT>(fr.v[i][1]);
   }
+  pm_store_pair(fl, r0 * kTileCols + cp * 2, half, v);
+}
+
+// The quad-major layout's zero pixel (kPadSlot).
+__device__ __forceinline__ void pm_pad(float* fl) {
+  if (threadIdx.x < kPmFrames) fl[pm_word(kPadSlot, threadIdx.x)] = 0.0f;
 }
 
 template <int F, class LT>
@@ -911,25 +932,24 @@ __device__ __forceinline__ void db_store_masked_as(const DbFrames<F>& fr, LT* fl
   const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
   const uint32_t px = r0 * kTileCols + cp * 2;
   const uint32_t m = fr.mk0 | (fr.mk1 << 8);
+  static_assert(F == 16 && std::is_same<LT, float>::value, "pixel-major 16-frame tiles");
+  float v[8][2];
   if (half == 0) {
 #pragma unroll
     for (uint32_t i = 0; i < kMaskedFrames; i++) {
-      Lds2<LT> o;
-      o.v[0] = (m >> i) & 1u ? LT(0) : word_to<W, LT>(fr.v[i][0]);
-      o.v[1] = (m >> (8 + i)) & 1u ? LT(0) : word_to<W, LT>(fr.v[i][1]);
-      *reinterpret_cast<Lds2<LT>*>(fl + i * kLdsStride + px) = o;
+      v[i][0] = (m >> i) & 1u ? 0.0f : word_to<W, LT>(fr.v[i][0]);
+      v[i][1] = (m >> (8 + i)) & 1u ? 0.0f : word_to<W, LT>(fr.v[i][1]);
     }
   } else {
-    const LT n0 = static_cast<LT>(__uint_as_float(fr.v[0][0]));
-    const LT n1 = static_cast<LT>(__uint_as_float(fr.v[0][1]));
+    const float n0 = __uint_as_float(fr.v[0][0]);
+    const float n1 = __uint_as_float(fr.v[0][1]);
 #pragma unroll
     for (uint32_t i = 0; i < kMaskedFrames; i++) {
-      Lds2<LT> o;
-      o.v[0] = (m >> i) & 1u ? LT(0) : n0;
-      o.v[1] = (m >> (8 + i)) & 1u ? LT(0) : n1;
-      *reinterpret_cast<Lds2<LT>*>(fl + (kMaskedFrames + i) * kLdsStride + px) = o;
+      v[i][0] = (m >> i) & 1u ? 0.0f : n0;
+      v[i][1] = (m >> (8 + i)) & 1u ? 0.0f : n1;
     }
   }
+  pm_store_pair(fl, px, half, v);
 }
 
 template <int F, class LT>
@@ -968,6 +988,8 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   constexpr uint32_t L = F / G;
   constexpr uint32_t SLOTS = kDbThreads / L;
   constexpr int R = SegRounds<SLOTS>::R;
+  static_assert((size_t)(kPmFrames / 4) * kPmQuad * sizeof(float) <= PL::kFrameBytes,
+                "a quad-major tile fits its frame buffer");
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const uint32_t slot = threadIdx.x / L, q = threadIdx.x % L;
   // Items: K = ntiles / grid full rounds give the workgroup's tiles
@@ -1026,8 +1048,8 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
     }
     return tile_meta_of(a, it.tile);
   };
-  pad_slots<F>(reinterpret_cast<LT*>(frames_buf(0)));
-  pad_slots<F>(reinterpret_cast<LT*>(frames_buf(1)));
+  pm_pad(reinterpret_cast<float*>(frames_buf(0)));
+  pm_pad(reinterpret_cast<float*>(frames_buf(1)));
 
   DbFrames<F> fr;
   TableRegs<1, MODE> tr;
@@ -1085,7 +1107,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
     // 4. Item i from its buffers. A later pass of a tile whose table takes
     //    several chunks reloads the first (the last chunk is in LDS).
     if (mA.x != mA.y) {
-      const LT* base = reinterpret_cast<const LT*>(frames_buf(i)) + q * G * kLdsStride;
+      const PmView base{reinterpret_cast<const float*>(frames_buf(i)), q * G};
       unsigned char* tb = table_buf(iA);
       const bool first_in_lds = iA.first || mA.w - mA.z <= C;
       // The first slots hold the tile's longest segments, which set the

@@ -187,10 +187,49 @@ __device__ __forceinline__ void seg_write(const TileArgs& a, uint32_t s, uint32_
   }
 }
 
+// A lane's frames in a staged tile. Frame-major (FmView): frame j of the
+// lane at base[j * kLdsStride + pixel]. Quad-major (PmView, 16-frame tiles of
+// float32 values): frames 4k .. 4k + 3 of pixel p at word k * kPmQuad + 4p,
+// so that a lane's 4 frames are one 16-byte read, and the quads a segment's
+// 4 lanes read start at bank 4 (p + k) mod 32 (8 positions, not 2).
+template <class LT> struct FmView {
+  using T = LT;
+  const LT* base;
+  template <int G> __device__ __forceinline__ void load(uint32_t l, T (&x)[G]) const {
+#pragma unroll
+    for (int j = 0; j < G; j++) x[j] = base[j * kLdsStride + l];
+  }
+};
+
+constexpr uint32_t kPmFrames = 16;
+constexpr uint32_t kPmQuad = 4 * (kPadSlot + 1);  // words per quad plane (pixels and zero pixel)
+__device__ __forceinline__ uint32_t pm_word(uint32_t p, uint32_t f) {
+  return (f >> 2) * kPmQuad + 4u * p + (f & 3u);
+}
+
+struct PmView {
+  using T = float;
+  const float* fl;
+  uint32_t f0;  // the lane's first frame
+  template <int G> __device__ __forceinline__ void load(uint32_t l, T (&x)[G]) const {
+    static_assert(G == 1 || G == 2 || G == 4, "frames per lane");
+    const float* p = fl + pm_word(l, f0);
+    if constexpr (G == 4) {
+      const f32x4 v = *reinterpret_cast<const f32x4*>(p);
+      x[0] = v.x, x[1] = v.y, x[2] = v.z, x[3] = v.w;
+    } else if constexpr (G == 2) {
+      const f32x2 v = *reinterpret_cast<const f32x2*>(p);
+      x[0] = v.x, x[1] = v.y;
+    } else {
+      x[0] = *p;
+    }
+  }
+};
+
 // Applies a segment (table groups [ga, gb) of the LDS chunk) to this lane's G
 // frames.
-template <int G, int MODE, bool VAR, class LT>
-__device__ __forceinline__ SegVals<G, MODE, VAR> seg_eval(const LT* base, const u32x2* tl,
+template <int G, int MODE, bool VAR, class FV>
+__device__ __forceinline__ SegVals<G, MODE, VAR> seg_eval(const FV& base, const u32x2* tl,
                                                           const unsigned char* twb, uint32_t ga,
                                                           uint32_t gb) {
   using Acc = typename TAcc<MODE>::T;
@@ -230,11 +269,9 @@ __device__ __forceinline__ SegVals<G, MODE, VAR> seg_eval(const LT* base, const 
   auto step = [&](uint32_t g, const Words& cur, Words& nxt) {
     const uint32_t l[4] = {cur.l.x & 0xffffu, cur.l.x >> 16, cur.l.y & 0xffffu,
                            cur.l.y >> 16};
-    LT x[4][G];
+    typename FV::T x[4][G];
 #pragma unroll
-    for (int e = 0; e < 4; e++)
-#pragma unroll
-      for (int j = 0; j < G; j++) x[e][j] = base[j * kLdsStride + l[e]];
+    for (int e = 0; e < 4; e++) base.template load<G>(l[e], x[e]);
     read_words(g + 1 < gb ? g + 1 : g, nxt);
     // Keep the reads above the products: the scheduler would otherwise sink
     // the next group's reads to just before their use.
@@ -282,8 +319,8 @@ __device__ __forceinline__ SegVals<G, MODE, VAR> seg_eval(const LT* base, const 
 }
 
 // Applies segment s to this lane's G frames and writes its partials.
-template <int F, int G, int MODE, bool VAR, class LT>
-__device__ __forceinline__ void seg_apply(const TileArgs& a, const LT* base, const u32x2* tl,
+template <int F, int G, int MODE, bool VAR, class FV>
+__device__ __forceinline__ void seg_apply(const TileArgs& a, const FV& base, const u32x2* tl,
                                           const unsigned char* twb, uint32_t ga, uint32_t gb,
                                           uint32_t s, uint32_t q) {
   seg_write<F, G, MODE, VAR>(a, s, q, seg_eval<G, MODE, VAR>(base, tl, twb, ga, gb));
