@@ -1131,12 +1131,14 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
 
 // prepared.rs:985-997 (normalize_bin) after adding a bin's partials: I =
 // s / den, sigma = sqrt(max(v / den^2, 0)), 0 where den == 0. One launch
-// reduces the frames of consecutive passes: pass j's partials are the block
-// at part + j * BS, rows of FS frames. A workgroup serves one bin and one
-// group of at most 32 frames (blockIdx.y): 64 segment lanes times Q frame
-// quads of FPT frames (Q = ceil(frames / FPT) rounded up to a power of two,
-// so a wavefront holds whole segment lanes; quads past the group's frames
-// read columns that are not stored, and no quad straddles two passes as FPT
+// reduces the frames of consecutive passes: frame f of segment g is column
+// f % FS of row g of the block at part + (f / FS) * BS (a block per pass
+// with FS = F, or one block whose rows hold every pass of a launch side by
+// side with FS = passes * F). A workgroup serves one bin and one group of at
+// most 32 frames (blockIdx.y): 64 segment lanes times Q frame quads of FPT
+// frames (Q = ceil(frames / FPT) rounded up to a power of two, so a
+// wavefront holds whole segment lanes; quads past the group's frames read
+// columns that are not stored, and no quad straddles two rows as FPT
 // divides FS). Segment lane l adds the bin's partials l, l + 64, l + 128, ...
 // (tile order), FPT frames at a time with one vector load; the 64 lane sums
 // are then combined by a fixed pairwise tree (l with l ^ 1, then l ^ 2, ...
@@ -1172,9 +1174,8 @@ __global__ __launch_bounds__(kFinalLanes * kFinalMaxQuads) void k_tfinal(
     uint32_t ntot, uint32_t RF, int raw, double* __restrict__ I, double* __restrict__ S) {
   using AT = typename std::conditional<PLAIN, float, double>::type;
   // Frame group blockIdx.y: frames [f0, f0 + nf) of the ntot, RF per group
-  // (a multiple of FS, so a group starts on a pass block).
+  // (a multiple or a divisor of FS, so a group's rows lie in whole blocks).
   const uint32_t f0 = blockIdx.y * RF, nf = min(RF, ntot - f0);
-  part += (uint64_t)(f0 / FS) * BS;
   I += (uint64_t)f0 * nbins;
   if constexpr (VAR) S += (uint64_t)f0 * nbins;
   constexpr int NV = VAR ? 2 : 1;
@@ -1190,8 +1191,8 @@ __global__ __launch_bounds__(kFinalLanes * kFinalMaxQuads) void k_tfinal(
     acc[0][k] = 0;
     if constexpr (VAR) acc[NV - 1][k] = 0;
   }
-  const uint32_t c0 = q * FPT;
-  const PT* pr = part + (uint64_t)(c0 / FS) * BS + c0 % FS;
+  const uint32_t fc = f0 + q * FPT;
+  const PT* pr = part + (uint64_t)(fc / FS) * BS + fc % FS;
   const PT* pv = pr + (size_t)nseg * FS;
   // Partials of this lane (i = q0 + l, q0 + l + 64, ...), added in order:
   // U segment indices are read at once, then the U partial rows they name,
@@ -1377,10 +1378,11 @@ void launch_m(const TiledTable& t, const TileArgs& a, bool var, int precision, h
 
 // npass > 1 (double-buffered kernel only): npass passes in one launch, the
 // last of nf frames and the others of F; pass h reads frames h * F onwards
-// and writes its partials at part + h * pblock bytes.
+// and writes its partials at part + h * pblock bytes, in segment rows of prow
+// elements (0: F).
 void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, int dtype,
                  uint32_t nf, bool var, int precision, void* part, hipStream_t s,
-                 uint32_t npass = 1, uint64_t pblock = 0) {
+                 uint32_t npass = 1, uint64_t pblock = 0, uint32_t prow = 0) {
   const uint32_t al8 = (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % 8 == 0) ? 1u : 0u;
   TileArgs a{fr,     dtype,   t.lidx.p, t.w32.p, t.w64.p, t.seg_group.p,
              t.tile_meta.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
@@ -1392,6 +1394,7 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
   a.flip = t.flip ? 1u : 0u;
   a.nf = npass > 1 ? F : nf;
   a.pblock = pblock;
+  a.prow = prow;
   if (npass > 1 && !double_buffered(t, F, dtype, precision))
     fail(DIO_ERR_INVALID, "several passes per launch need the double-buffered kernel");
   if (precision == DIO_F64 && !t.w64.p) fail(DIO_ERR_INVALID, "plan has no float64 weights");
@@ -1421,7 +1424,7 @@ void launch_final(const TiledTable& t, const void* part, uint32_t F, uint64_t BS
   const uint32_t nf = std::min(ntot, RF);
   unsigned quads = 1;
   while (quads * fpt < nf) quads *= 2;
-  if (quads > kFinalMaxQuads || RF % F) fail(DIO_ERR_INVALID, "invalid reduction shape");
+  if (quads > kFinalMaxQuads || (RF % F && F % RF)) fail(DIO_ERR_INVALID, "invalid reduction shape");
   const unsigned threads = kFinalLanes * quads;
   const dim3 grid(t.nbins, (ntot + RF - 1) / RF);
   if (fpt == 4)
@@ -1611,9 +1614,12 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   const bool plain = precision == DIO_F32, f64 = precision == DIO_F64;
   // The scratch holds the float32 partials of kMaxPassesPerLaunch passes (or
   // the float64 ones of one). The double-buffered kernel runs that many
-  // passes in one launch over one read of the table; otherwise a launch is a
-  // pass. A reduction launch serves up to 32 frames (two 16-frame passes).
-  // The frames' arithmetic does not depend on the passes per launch.
+  // passes in one launch over one read of the table, each segment's partials
+  // of all of them in one row (P * F columns, so the reduction reads whole
+  // 128-byte pieces of a row); otherwise a launch is a pass, with a block of
+  // rows per pass. A reduction launch serves up to 32 frames (two 16-frame
+  // passes). The frames' arithmetic does not depend on the passes per launch
+  // or on the layout.
   const bool db = !f64 && double_buffered(t, F, dtype, precision);
   uint32_t P = passes_per_launch(t, F, dtype, precision);
   const size_t pbytes = pass_bytes(t, F, var, precision);
@@ -1643,7 +1649,7 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
       const uint32_t np = (ng + F - 1) / F;
       timed(ng, [&] {
         launch_pass(t, F, G, fb + g0 * npix * esz, dtype, ng - (np - 1) * F, var, precision,
-                    sc.ptr(), s, np, pbytes);
+                    sc.ptr(), s, np, (uint64_t)F * sizeof(float), FR);
       });
     } else {
       for (uint32_t j = 0; j * F < ng; j++) {
@@ -1657,17 +1663,18 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
     }
     {
       const void* pr = sc.ptr();
+      const uint32_t FS = db ? FR : F;
       double* I = intensity + g0 * t.nbins;
       double* S = var ? sigma + g0 * t.nbins : nullptr;
       if (f64) {
-        if (var) launch_final<false, true, double>(t, pr, F, BS, ng, RF, raw, I, S, s);
-        else launch_final<false, false, double>(t, pr, F, BS, ng, RF, raw, I, S, s);
+        if (var) launch_final<false, true, double>(t, pr, FS, BS, ng, RF, raw, I, S, s);
+        else launch_final<false, false, double>(t, pr, FS, BS, ng, RF, raw, I, S, s);
       } else if (plain) {
-        if (var) launch_final<true, true, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
-        else launch_final<true, false, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
+        if (var) launch_final<true, true, float>(t, pr, FS, BS, ng, RF, raw, I, S, s);
+        else launch_final<true, false, float>(t, pr, FS, BS, ng, RF, raw, I, S, s);
       } else {
-        if (var) launch_final<false, true, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
-        else launch_final<false, false, float>(t, pr, F, BS, ng, RF, raw, I, S, s);
+        if (var) launch_final<false, true, float>(t, pr, FS, BS, ng, RF, raw, I, S, s);
+        else launch_final<false, false, float>(t, pr, FS, BS, ng, RF, raw, I, S, s);
       }
     }
     DIO_HIP(hipGetLastError());

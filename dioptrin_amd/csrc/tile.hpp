@@ -57,9 +57,11 @@ void build_tiled(TiledTable& t, uint32_t dim1, uint32_t dim2, uint32_t nbins, ui
                  const uint32_t* row_ptr, const uint32_t* col, const float* w32,
                  const double* w64, bool keep_w64, hipStream_t s);
 
-// Segment partials: per pass a block [1 + errors][nseg][frames per pass],
-// float32 (float64 for the F64 precision), for up to kMaxPassesPerLaunch
-// passes of float32 partials (or one pass of float64 ones).
+// Segment partials, float32 (float64 for the F64 precision), for up to
+// kMaxPassesPerLaunch passes of float32 partials (or one pass of float64
+// ones): per pass a block [1 + errors][nseg][frames per pass], or, for the
+// passes of one launch of the double-buffered kernel, one block
+// [1 + errors][nseg][passes x frames per pass] (the same size).
 // Either owned (grown on demand) or external memory of a stated size (the
 // serving loop's stream-ordered slot buffers).
 struct TileScratch {

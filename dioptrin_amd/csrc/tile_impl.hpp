@@ -117,6 +117,11 @@ struct TileArgs {
   uint32_t npass = 1;
   uint32_t nf_last = 0;
   uint64_t pblock = 0;
+  // Elements per segment row of partials, 0 for F (a row per pass). With
+  // prow = npass * F and pblock the bytes of F elements, the passes of a
+  // launch share one row per segment: columns h * F .. h * F + F - 1 hold
+  // pass h.
+  uint32_t prow = 0;
   // Per-frame masks folded into the staging (double-buffered kernel, masked
   // rows): a pass of 16 rows holds 8 frames (rows 0-7, masked pixels read as
   // 0) and their masked float32 normalization (rows 8-15). mbits holds one
@@ -161,7 +166,8 @@ template <int F, int G, int MODE, bool VAR>
 __device__ __forceinline__ void seg_write(const TileArgs& a, uint32_t s, uint32_t q,
                                           const SegVals<G, MODE, VAR>& o) {
   using PT = typename TAcc<MODE>::P;
-  PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * F + q * G;
+  const size_t RS = a.prow ? a.prow : (uint32_t)F;
+  PT* out = reinterpret_cast<PT*>(a.part) + (size_t)s * RS + q * G;
   if constexpr (sizeof(PT) == 4 && (G == 2 || G == 4)) {
     // All G frames present: one vector store per kind (the row offset is a
     // multiple of G floats).
@@ -174,7 +180,7 @@ __device__ __forceinline__ void seg_write(const TileArgs& a, uint32_t s, uint32_
         if constexpr (VAR) e[j] = o.e[j];
       }
       *reinterpret_cast<V*>(out) = v;
-      if constexpr (VAR) *reinterpret_cast<V*>(out + (size_t)a.nseg * F) = e;
+      if constexpr (VAR) *reinterpret_cast<V*>(out + (size_t)a.nseg * RS) = e;
       return;
     }
   }
@@ -182,7 +188,7 @@ __device__ __forceinline__ void seg_write(const TileArgs& a, uint32_t s, uint32_
   for (int j = 0; j < G; j++) {
     if (q * G + j < a.nf) {
       out[j] = o.v[j];
-      if constexpr (VAR) out[(size_t)a.nseg * F + j] = o.e[j];
+      if constexpr (VAR) out[(size_t)a.nseg * RS + j] = o.e[j];
     }
   }
 }
