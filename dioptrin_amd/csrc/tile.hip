@@ -1697,9 +1697,11 @@ void tiled_gather_masked(const TiledTable& t, TileScratch& sc, const void* frame
   const uint64_t npix = (uint64_t)t.dim1 * t.dim2;
   const char* fb = static_cast<const char*>(frames);
   const uint32_t P = kMaxPassesPerLaunch;
-  const size_t pbytes = pass_bytes(t, F, var, precision);
   sc.ensure(t, F, var);
-  const uint64_t BS = (uint64_t)t.nseg * F * (var ? 2 : 1);  // elements per pass block
+  // Each segment's partials of the launch's passes in one row of P * F
+  // columns (one block; BS is not used).
+  const uint32_t FS = P * F;
+  const uint64_t BS = 0;
   const bool plain = precision == DIO_F32;
   for (uint64_t g0 = 0; g0 < n_frames; g0 += (uint64_t)P * kMaskedFrames) {
     const uint32_t ng = (uint32_t)std::min<uint64_t>((uint64_t)P * kMaskedFrames, n_frames - g0);
@@ -1713,7 +1715,8 @@ void tiled_gather_masked(const TiledTable& t, TileScratch& sc, const void* frame
     a.flip = t.flip ? 1u : 0u;
     a.npass = np;
     a.nf_last = F;
-    a.pblock = pbytes;
+    a.pblock = F * sizeof(float);
+    a.prow = FS;
     a.mbits = mbits + pass0 * mbits_stride;
     a.mbits_stride = mbits_stride;
     a.norm32 = norm32;
@@ -1732,11 +1735,11 @@ void tiled_gather_masked(const TiledTable& t, TileScratch& sc, const void* frame
     double* I = rows_sum + pass0 * F * t.nbins;
     double* V = var ? rows_var + pass0 * F * t.nbins : nullptr;
     if (plain) {
-      if (var) launch_final<true, true, float>(t, sc.ptr(), F, BS, np * F, 32, true, I, V, s);
-      else launch_final<true, false, float>(t, sc.ptr(), F, BS, np * F, 32, true, I, V, s);
+      if (var) launch_final<true, true, float>(t, sc.ptr(), FS, BS, np * F, 32, true, I, V, s);
+      else launch_final<true, false, float>(t, sc.ptr(), FS, BS, np * F, 32, true, I, V, s);
     } else {
-      if (var) launch_final<false, true, float>(t, sc.ptr(), F, BS, np * F, 32, true, I, V, s);
-      else launch_final<false, false, float>(t, sc.ptr(), F, BS, np * F, 32, true, I, V, s);
+      if (var) launch_final<false, true, float>(t, sc.ptr(), FS, BS, np * F, 32, true, I, V, s);
+      else launch_final<false, false, float>(t, sc.ptr(), FS, BS, np * F, 32, true, I, V, s);
     }
     DIO_HIP(hipGetLastError());
   }
