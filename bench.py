@@ -466,7 +466,10 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("precision") == args.precision and tr.get("frames_per_launch") == fl:
+        # Counters were taken for one configuration (precision, errors, frames
+        # per launch and tile); any other reports no traffic.
+        if (tr.get("precision") == args.precision and tr.get("frames_per_launch") == fl
+                and tr.get("errors") == bool(args.errors)):
             for e in tr.get("entries", []):
                 if e.get("tile") == [tile, lanes]:
                     traffic = e.get("hbm_bytes_per_launch")
