@@ -1412,19 +1412,22 @@ void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, in
   DIO_HIP(hipGetLastError());
 }
 
-// Reduces ntot frames of consecutive passes of F frames each (pass j's
-// partials are the block at part + j * BS elements), in groups of RF frames
-// (a multiple of F, at most 32), one launch for all groups. raw: the sums
-// before division instead of I and sigma.
+// Reduces ntot frames whose partials are rows of F frames (frame f in column
+// f % F of the block at part + (f / F) * BS elements), in groups of RF frames
+// (at most 32), one launch for all groups. raw: the sums before division
+// instead of I and sigma.
 template <bool PLAIN, bool VAR, class PT>
 void launch_final(const TiledTable& t, const void* part, uint32_t F, uint64_t BS, uint32_t ntot,
                   uint32_t RF, bool raw, double* I, double* S, hipStream_t s) {
   const PT* p = static_cast<const PT*>(part);
-  const uint32_t fpt = std::min(F, 4u);  // F is a power of two
+  // F: frames per row (a pass's F, or a launch's passes times F). A quad of
+  // fpt frames stays inside one row when fpt divides F and RF.
+  const uint32_t fpt = std::min(F, 4u);
   const uint32_t nf = std::min(ntot, RF);
   unsigned quads = 1;
   while (quads * fpt < nf) quads *= 2;
-  if (quads > kFinalMaxQuads || (RF % F && F % RF)) fail(DIO_ERR_INVALID, "invalid reduction shape");
+  if (quads > kFinalMaxQuads || F % fpt || RF % fpt)
+    fail(DIO_ERR_INVALID, "invalid reduction shape");
   const unsigned threads = kFinalLanes * quads;
   const dim3 grid(t.nbins, (ntot + RF - 1) / RF);
   if (fpt == 4)

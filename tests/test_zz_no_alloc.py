@@ -95,3 +95,24 @@ def test_noalloc_azimuthal_std_and_refusal(monkeypatch):
     synchronize()
     assert np.array_equal(oi.numpy(), ri.numpy())
     assert N.DIO_ERR_INVALID == 3
+
+
+@pytest.mark.parametrize("errors", [False, True])
+def test_noalloc_three_and_two_passes_per_launch(monkeypatch, errors):
+    """A usable scratch of 3 (then 2) pass blocks: launches of 48 (32)
+    frames whose partial rows hold 3 (2) passes side by side, reduced in
+    groups of 32 frames; the bits equal the device entry point's 64-frame
+    launches."""
+    w = synth.WORKLOADS["pilatus1m_ps1d"]
+    plan = GpuPlan(synth.workload_poni("pilatus1m_ps1d"), w["shape"], n_radial=w["n_radial"],
+                   precision="f32_wide")
+    plan.set_tile(16, 4)
+    frames = DeviceArray.from_numpy(
+        np.random.default_rng(43).poisson(90.0, size=(100,) + w["shape"]).astype(np.uint32))
+    pass_bytes = plan.info["n_segments"] * 16 * 4 * (2 if errors else 1)
+    for passes in (3, 2):
+        print(f"case noalloc {passes} passes errors={errors}", flush=True)
+        monkeypatch.setenv("DIOPTRIN_NOALLOC_LIMIT", str(passes * pass_bytes + 256))
+        run_both(plan, frames, errors)
+        run_both(plan, frames[:37], errors)
+    monkeypatch.delenv("DIOPTRIN_NOALLOC_LIMIT")
