@@ -13,13 +13,17 @@
 //    where n is not a positive finite number with a finite reciprocal or x
 //    is not finite;
 //  - per entry, once per tile: w = c * n, and an entry whose c or w fails the
-//    test is pointed at the tile's NaN slot.
+//    test is pointed at the tile's NaN slot (table padding at its zero slot,
+//    with w = 0).
 // An entry then counts for a frame exactly when its staged x is finite.
 // Per segment half and frame: one shifted pass with the part's first valid x
 // as the shift o (a short scan finds it first): W = sum w, A = sum w (x - o),
 // B = sum w (x - o)^2; mean o + A / W, second moment B - A^2 / W (the shift
 // keeps the cancellation at the scale of the spread, as the reference's
-// origin does). Entries that do not count add exact zeros.
+// origin does). Entries that do not count add exact zeros. The sums are
+// first taken without the validity test (padding adds exact zeros as it is);
+// a part that meets a NaN x gets non-finite A or B and is summed again with
+// the test, so both give the same bits.
 // Per bin the segments are merged in bin order by a fixed scheme, so a
 // frame's result depends only on the table.
 // I = mean (sum w x / sum w, which is the reference's sum c s / sum w),
@@ -51,10 +55,13 @@ constexpr uint32_t kStdCap = 1024;                       // table groups per LDS
 static_assert(kStdCap == kStdThreads, "one table group per thread");
 static_assert(kTilePixels == kStdThreads, "a thread stages one pixel");
 
-// LDS: x of the pass's frames [kStdFrames][kLdsStride] f64 (slot kPadSlot
-// NaN), the tile's normalization [kLdsStride] f64, then a table chunk: entry
-// byte offsets into a frame row (u16, 4 per group), w = c n (f64) and, with
+// LDS: x of the pass's frames [kStdFrames][kLdsStride] f64 (slot kPadSlot 0
+// for table padding, slot kStdNanSlot NaN for entries that never count), the
+// tile's normalization [kLdsStride] f64, then a table chunk: entry byte
+// offsets into a frame row (u16, 4 per group), w = c n (f64) and, with
 // errors, c w (f64).
+constexpr uint32_t kStdNanSlot = kPadSlot + 1;
+static_assert(kStdNanSlot < kLdsStride, "NaN slot inside the row");
 constexpr size_t kStdX = 0;
 constexpr size_t kStdNN = kStdX + (size_t)kStdFrames * kLdsStride * 8;
 constexpr size_t kStdTL = kStdNN + (size_t)kLdsStride * 8;
@@ -66,7 +73,7 @@ static_assert(kStdLdsVar <= 160 * 1024, "LDS plan exceeds one CU");
 static_assert(kStdCap >= kTilePixels / kGroup, "a whole segment must fit one table chunk");
 static_assert(kStdNN % 16 == 0 && kStdTL % 16 == 0 && kStdTW % 16 == 0 && kStdTC % 16 == 0,
               "16-byte LDS rows");
-static_assert(kPadSlot * 8 < 65536, "byte offsets fit 16 bits");
+static_assert(kStdNanSlot * 8 < 65536, "byte offsets fit 16 bits");
 
 template <bool VAR> constexpr size_t std_lds() { return VAR ? kStdLdsVar : kStdLdsPlain; }
 
@@ -141,7 +148,8 @@ __device__ __forceinline__ void std_prefetch(const StdArgs& a, uint32_t t, uint4
 }
 
 // The thread's pixel of each frame as x = s / n (NaN where it does not
-// count); the tile's normalization must already be in LDS.
+// count), and the zero and NaN slots; the tile's normalization must already
+// be in LDS.
 template <class P>
 __device__ __forceinline__ void std_store(const StdRegs<P>& g, unsigned char* lds) {
   double* X = reinterpret_cast<double*>(lds + kStdX);
@@ -152,12 +160,16 @@ __device__ __forceinline__ void std_store(const StdRegs<P>& g, unsigned char* ld
     const double x = (double)g.x[f] * rn;
     X[f * kLdsStride + threadIdx.x] = pix && isfinite(x) ? x : __builtin_nan("");
   }
-  if (threadIdx.x < kStdFrames) X[threadIdx.x * kLdsStride + kPadSlot] = __builtin_nan("");
+  if (threadIdx.x < kStdFrames) {
+    X[threadIdx.x * kLdsStride + kPadSlot] = 0.0;
+    X[threadIdx.x * kLdsStride + kStdNanSlot] = __builtin_nan("");
+  }
 }
 
 // Table group i of the chunk from its index word pair and weights: w = c n
 // per entry from the tile's normalization in LDS; entries failing the
-// reference's c and w tests point at the NaN slot.
+// reference's c and w tests point at the NaN slot, table padding at the zero
+// slot, both with w = c w = 0.
 template <bool VAR>
 __device__ __forceinline__ void std_prep(unsigned char* lds, uint32_t i, u32x2 li, f32x4 cw) {
   const double* NN = reinterpret_cast<const double*>(lds + kStdNN);
@@ -170,9 +182,9 @@ __device__ __forceinline__ void std_prep(unsigned char* lds, uint32_t i, u32x2 l
     const double cd = (double)c[e];
     const double we = cd * NN[l[e]];
     const bool ok = (cd > 0.0) & (we > 0.0) & (we <= DBL_MAX);
-    off[e] = (ok ? l[e] : kPadSlot) * 8u;
-    w[e] = we;
-    cwv[e] = cd * we;
+    off[e] = (ok ? l[e] : l[e] == kPadSlot ? kPadSlot : kStdNanSlot) * 8u;
+    w[e] = ok ? we : 0.0;
+    cwv[e] = ok ? cd * we : 0.0;
   }
   reinterpret_cast<u32x2*>(lds + kStdTL)[i] = u32x2{off[0] | off[1] << 16, off[2] | off[3] << 16};
   f64x2* TW = reinterpret_cast<f64x2*>(lds + kStdTW);
@@ -211,8 +223,10 @@ __device__ __forceinline__ double std_shift(const unsigned char* lds, uint32_t g
 
 // Shifted sums over chunk groups [ga, gb) of frame q: W = sum w,
 // A = sum w (x - o), B = sum w (x - o)^2 and (VAR) v = sum c w max(x, 0) over
-// the entries that count; the others add exact zeros.
-template <bool VAR>
+// the entries that count; the others add exact zeros. Without CHECK every
+// entry is taken as it is: the same sums where every x is finite (padding:
+// x = 0, w = 0), non-finite A and B where one is NaN.
+template <bool VAR, bool CHECK>
 __device__ __forceinline__ void std_sums(const unsigned char* lds, uint32_t ga, uint32_t gb,
                                          uint32_t q, double o, double& W, double& A, double& B,
                                          double& v) {
@@ -235,12 +249,19 @@ __device__ __forceinline__ void std_sums(const unsigned char* lds, uint32_t ga, 
     }
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      const bool ok = __builtin_isfinite(x[e]);
-      const double wv = ok ? w[e] : 0.0;
-      const double d = (ok ? x[e] : o) - o, t = wv * d;
-      W = W + wv;
-      A = A + t;
-      B = fma(t, d, B);
+      if constexpr (CHECK) {
+        const bool ok = __builtin_isfinite(x[e]);
+        const double wv = ok ? w[e] : 0.0;
+        const double d = (ok ? x[e] : o) - o, t = wv * d;
+        W = W + wv;
+        A = A + t;
+        B = fma(t, d, B);
+      } else {
+        const double d = x[e] - o, t = w[e] * d;
+        W = W + w[e];
+        A = A + t;
+        B = fma(t, d, B);
+      }
       if constexpr (VAR) v = fma(cw[e], fmax(x[e], 0.0), v);
     }
   }
@@ -277,7 +298,12 @@ __device__ __forceinline__ void std_segment(const StdArgs& a, const unsigned cha
   double W = 0.0, A = 0.0, B = 0.0, v = 0.0, o = 0.0;
   if (write) {
     o = std_shift(lds, ga, gb, q);
-    std_sums<VAR>(lds, h ? gm : ga, h ? gb : gm, q, o, W, A, B, v);
+    const uint32_t g0 = h ? gm : ga, g1 = h ? gb : gm;
+    std_sums<VAR, false>(lds, g0, g1, q, o, W, A, B, v);
+    if (!__builtin_isfinite(A) || !__builtin_isfinite(B)) {
+      W = A = B = v = 0.0;
+      std_sums<VAR, true>(lds, g0, g1, q, o, W, A, B, v);
+    }
   }
   // Sums of the two halves (addition is commutative, so both lanes agree).
   W = W + __shfl_xor(W, (int)kStdFrames);
