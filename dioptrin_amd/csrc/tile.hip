@@ -4,7 +4,8 @@
 // 4-entry groups; segment -> first group u32[nseg + 1]; tile -> first
 // segment u32[ntiles + 1]; bin -> segment list u32[nseg] + u32[nbins + 1];
 // denominators f64[nbins]. Partials of one pass: [1 + errors][nseg][F]
-// float32 (float64 for the F64 precision).
+// float32 (float64 for the F64 precision); a launch of the double-buffered
+// kernel writes its passes side by side in one row per segment.
 //
 // k_tile: persistent workgroups of 512 threads, each walking detector tiles
 // of kTileRows x kTileCols pixels (tile b, b + grid, ...).
@@ -749,7 +750,8 @@ __attribute__((amdgpu_waves_per_eu(Plan<F, MODE>::kWgs * tile_threads(F) / 256))
 
 // ---------------------------------------------------------------- double-buffered tile kernel
 
-// Double-buffered variant for 16-frame passes of 32-bit frames: one workgroup
+// Double-buffered variant for 16-frame passes of 32-bit and 16-bit frames
+// (float32 or float64 sums of float32 weights): one workgroup
 // of 1024 threads per CU holds two LDS frame tiles and two table chunks.
 // The work of a workgroup is a sequence of items, an item being one tile and
 // one pass: with several passes per launch (npass, up to four), the passes of
@@ -786,21 +788,30 @@ template <int F> struct DbFrames {
   uint32_t mk0, mk1;  // masked rows: the pair's mask bit bytes (pixel 0, pixel 1)
 };
 
+// Bytes per pixel of the frames a pair load of A bytes reads (16-bit frames:
+// A = 2, one 4-byte load per pixel pair, or A = 1, two 2-byte loads).
+template <int A> constexpr uint32_t db_esz() { return A <= 2 ? 2u : 4u; }
+
 // Arguments of pass h < npass of a launch.
-template <int F>
+template <int F, int A>
 __device__ __forceinline__ TileArgs db_pass(const TileArgs& a, uint32_t h) {
   TileArgs r = a;
   if (a.npass > 1) {
-    r.frames = static_cast<const unsigned char*>(a.frames) + (uint64_t)h * F * a.npix * 4u;
+    r.frames = static_cast<const unsigned char*>(a.frames) +
+               (uint64_t)h * F * a.npix * db_esz<A>();
     r.nf = h + 1 == a.npass ? a.nf_last : (uint32_t)F;
     r.part = static_cast<unsigned char*>(a.part) + h * a.pblock;
   }
   return r;
 }
 
+// Pair loads: 32-bit frames one 8-byte load (A = 8) or two 4-byte loads
+// (A = 4); 16-bit frames one 4-byte load holding both pixels (A = 2, kept
+// packed until the store) or two 2-byte loads (A = 1).
 template <int F, int A>
 __device__ __forceinline__ void db_load(const TileArgs& a, uint32_t t, DbFrames<F>& fr) {
-  static_assert(A == 4 || A == 8, "pair loads are 4 or 8 bytes");
+  static_assert(A == 1 || A == 2 || A == 4 || A == 8, "pair loads of 1, 2, 4 or 8 bytes");
+  constexpr uint32_t E = db_esz<A>();
   const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
   const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
   const uint32_t c = (t % a.ntx) * kTileCols + cp * 2;
@@ -808,20 +819,26 @@ __device__ __forceinline__ void db_load(const TileArgs& a, uint32_t t, DbFrames<
   const uint32_t nvalid = (r < a.dim1 && c < a.dim2) ? min(2u, a.dim2 - c) : 0u;
   const uint32_t off = (a.flip && nvalid ? a.dim1 - 1 - r : r) * a.dim2 + c;
   // Unconditional loads from valid addresses, as in frames_load32.
-  const uint32_t o0 = nvalid > 0 ? off * 4u : 0u, o1 = nvalid > 1 ? off * 4u + 4u : 0u;
+  const uint32_t o0 = nvalid > 0 ? off * E : 0u, o1 = nvalid > 1 ? off * E + E : 0u;
   const unsigned char* src = static_cast<const unsigned char*>(a.frames);
   const uint32_t flast = a.nf - 1;
 #pragma unroll
   for (int i = 0; i < F / 2; i++) {
     const uint32_t f = half * (F / 2) + i;
-    const unsigned char* pf = src + (uint64_t)min(f, flast) * a.npix * 4u;
+    const unsigned char* pf = src + (uint64_t)min(f, flast) * a.npix * E;
     if constexpr (A == 8) {
       const u32x2 v = load_at<u32x2>(pf, o0);
       fr.v[i][0] = v.x;
       fr.v[i][1] = v.y;
-    } else {
+    } else if constexpr (A == 4) {
       fr.v[i][0] = load_at<uint32_t>(pf, o0);
       fr.v[i][1] = load_at<uint32_t>(pf, o1);
+    } else if constexpr (A == 2) {
+      fr.v[i][0] = load_at<uint32_t>(pf, o0);
+      fr.v[i][1] = 0u;
+    } else {
+      fr.v[i][0] = load_at<uint16_t>(pf, o0);
+      fr.v[i][1] = load_at<uint16_t>(pf, o1);
     }
   }
 }
@@ -842,7 +859,8 @@ __device__ __forceinline__ void pm_store_pair(float* fl, uint32_t px, uint32_t h
   }
 }
 
-template <int F, class W, class LT>
+// PACKED: both pixels of a 16-bit pair in v[i][0] (low half first).
+template <int F, class W, class LT, bool PACKED = false>
 __device__ __forceinline__ void db_store_as(const DbFrames<F>& fr, LT* fl) {
   static_assert(F == 16 && std::is_same<LT, float>::value, "pixel-major 16-frame tiles");
   const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
@@ -850,8 +868,13 @@ __device__ __forceinline__ void db_store_as(const DbFrames<F>& fr, LT* fl) {
   float v[8][2];
 #pragma unroll
   for (int i = 0; i < F / 2; i++) {
-    v[i][0] = word_to<W, LT>(fr.v[i][0]);
-    v[i][1] = word_to<W, LT>(fr.v[i][1]);
+    if constexpr (PACKED) {
+      v[i][0] = word_to<W, LT>(fr.v[i][0] & 0xffffu);
+      v[i][1] = word_to<W, LT>(fr.v[i][0] >> 16);
+    } else {
+      v[i][0] = word_to<W, LT>(fr.v[i][0]);
+      v[i][1] = word_to<W, LT>(fr.v[i][1]);
+    }
   }
   pm_store_pair(fl, r0 * kTileCols + cp * 2, half, v);
 }
@@ -861,11 +884,15 @@ __device__ __forceinline__ void pm_pad(float* fl) {
   if (threadIdx.x < kPmFrames) fl[pm_word(kPadSlot, threadIdx.x)] = 0.0f;
 }
 
-template <int F, class LT>
+template <int F, int A, class LT>
 __device__ __forceinline__ void db_store(const TileArgs& a, const DbFrames<F>& fr, LT* fl) {
-  if (a.dtype == DIO_U32) db_store_as<F, uint32_t, LT>(fr, fl);
-  else if (a.dtype == DIO_I32) db_store_as<F, int32_t, LT>(fr, fl);
-  else db_store_as<F, float, LT>(fr, fl);
+  if constexpr (A <= 2) {
+    db_store_as<F, uint16_t, LT, A == 2>(fr, fl);
+  } else {
+    if (a.dtype == DIO_U32) db_store_as<F, uint32_t, LT>(fr, fl);
+    else if (a.dtype == DIO_I32) db_store_as<F, int32_t, LT>(fr, fl);
+    else db_store_as<F, float, LT>(fr, fl);
+  }
 }
 
 // Masked rows (see TileArgs::mbits): pass h of a launch holds frames 8 h ..
@@ -980,7 +1007,8 @@ __device__ __forceinline__ uint32_t db_prefetch(const TileArgs& ap, uint32_t tt,
 template <int F, int G, int MODE, bool VAR, int FA, bool MASKED = false>
 __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_db(
     TileArgs a) {
-  static_assert(FA == 4 || FA == 8, "32-bit frames only");
+  static_assert(FA == 1 || FA == 2 || FA == 4 || FA == 8, "16- and 32-bit frames only");
+  static_assert(!MASKED || FA >= 4, "masked rows need 32-bit frames");
   static_assert(F % 2 == 0, "frames split over two thread halves");
   using LT = typename TAcc<MODE>::L;
   using PL = PlanDb<F, MODE>;
@@ -1034,7 +1062,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   // Arguments of pass h of the launch.
   auto pass_args = [&](uint32_t h) {
     if constexpr (MASKED) return db_pass_masked<F>(a, h);
-    else return db_pass<F>(a, h);
+    else return db_pass<F, FA>(a, h);
   };
   MetaLanes ml;
   ml.load(a, t0, grid, 0);
@@ -1066,7 +1094,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   if (mA.x != mA.y) {
     LT* fb0 = reinterpret_cast<LT*>(frames_buf(0));
     if constexpr (MASKED) db_store_masked<F, LT>(a, fr, fb0);
-    else db_store<F, LT>(a, fr, fb0);
+    else db_store<F, FA, LT>(a, fr, fb0);
     table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(iA)),
                                      table_buf(iA) + (size_t)C * 8);
   }
@@ -1086,7 +1114,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
     if (mB.x != mB.y) {
       LT* fb1 = reinterpret_cast<LT*>(frames_buf(i + 1));
       if constexpr (MASKED) db_store_masked<F, LT>(a, fr, fb1);
-      else db_store<F, LT>(a, fr, fb1);
+      else db_store<F, FA, LT>(a, fr, fb1);
       if (iB.first)
         table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(iB)),
                                          table_buf(iB) + (size_t)C * 8);
@@ -1340,14 +1368,21 @@ void launch_v(const TiledTable& t, const TileArgs& a, bool var, hipStream_t s) {
   const bool w32 = a.dtype == DIO_U32 || a.dtype == DIO_I32 || a.dtype == DIO_F32T;
   const int fa = !w32 ? 0 : a.aligned8 ? 8 : 4;
   if constexpr (F == 16 && MODE != 2) {
-    if (t.db && fa != 0) {
-      if (var) {
-        if (fa == 8) launch_tile_db<F, G, MODE, true, 8>(t, a, s);
-        else launch_tile_db<F, G, MODE, true, 4>(t, a, s);
-      } else {
-        if (fa == 8) launch_tile_db<F, G, MODE, false, 8>(t, a, s);
-        else launch_tile_db<F, G, MODE, false, 4>(t, a, s);
-      }
+    // Pair loads of the double-buffered kernel: 8 or 4 bytes for 32-bit
+    // frames, 2 or 1 for 16-bit ones.
+    const int da = a.dtype == DIO_U16 ? (a.aligned8 ? 2 : 1) : fa;
+    if (t.db && da != 0) {
+      auto go = [&](auto v) {
+        constexpr bool V = decltype(v)::value;
+        switch (da) {
+          case 8: launch_tile_db<F, G, MODE, V, 8>(t, a, s); break;
+          case 4: launch_tile_db<F, G, MODE, V, 4>(t, a, s); break;
+          case 2: launch_tile_db<F, G, MODE, V, 2>(t, a, s); break;
+          default: launch_tile_db<F, G, MODE, V, 1>(t, a, s); break;
+        }
+      };
+      if (var) go(std::true_type{});
+      else go(std::false_type{});
       return;
     }
   }
@@ -1383,7 +1418,11 @@ void launch_m(const TiledTable& t, const TileArgs& a, bool var, int precision, h
 void launch_pass(const TiledTable& t, uint32_t F, uint32_t G, const void* fr, int dtype,
                  uint32_t nf, bool var, int precision, void* part, hipStream_t s,
                  uint32_t npass = 1, uint64_t pblock = 0, uint32_t prow = 0) {
-  const uint32_t al8 = (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % 8 == 0) ? 1u : 0u;
+  // Pixel pairs aligned to their size (8 bytes for 32-bit frames, 4 for
+  // 16-bit ones) in every frame.
+  const uintptr_t pair = dtype == DIO_U16 ? 4 : 8;
+  const uint32_t al8 =
+      (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % pair == 0) ? 1u : 0u;
   TileArgs a{fr,     dtype,   t.lidx.p, t.w32.p, t.w64.p, t.seg_group.p,
              t.tile_meta.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
              (uint64_t)t.dim1 * t.dim2, nf, al8, part};
@@ -1591,8 +1630,8 @@ void tiled_pass_only(const TiledTable& t, TileScratch& sc, const void* frames, i
 }
 
 bool double_buffered(const TiledTable& t, uint32_t F, int dtype, int precision) {
-  const bool w32 = dtype == DIO_U32 || dtype == DIO_I32 || dtype == DIO_F32T;
-  return F == 16 && precision != DIO_F64 && t.db && w32;
+  const bool w = dtype == DIO_U32 || dtype == DIO_I32 || dtype == DIO_F32T || dtype == DIO_U16;
+  return F == 16 && precision != DIO_F64 && t.db && w;
 }
 
 size_t pass_bytes(const TiledTable& t, uint32_t F, bool var, int precision) {
@@ -1685,7 +1724,8 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
 }
 
 bool masked_rows_supported(const TiledTable& t, int dtype, int precision, uint32_t G) {
-  return double_buffered(t, 16, dtype, precision) && valid_tile(precision, 16, G);
+  return dtype != DIO_U16 && double_buffered(t, 16, dtype, precision) &&
+         valid_tile(precision, 16, G);
 }
 
 uint32_t masked_rows_frames_per_pass() { return kMaskedFrames; }

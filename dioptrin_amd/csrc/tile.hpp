@@ -31,7 +31,7 @@ struct TiledTable {
   uint32_t dim1 = 0, dim2 = 0, nbins = 0, ntx = 0, nty = 0, ntiles = 0;
   uint32_t cus = 256;          // compute units of the device (persistent grid size)
   uint32_t max_wgs = 0;        // cap on the persistent grid, 0 for none
-  bool db = true;              // double-buffered kernel for 16-frame passes of 32-bit frames
+  bool db = true;              // double-buffered kernel for 16-frame passes of 16/32-bit frames
   bool flip = false;           // frames hold the detector's rows bottom-up
   uint64_t nnz = 0;
   uint32_t nseg = 0, ngroups = 0;

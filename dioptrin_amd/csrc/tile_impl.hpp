@@ -107,7 +107,7 @@ struct TileArgs {
   uint32_t nseg, ntx, ntiles, dim1, dim2;
   uint64_t npix;
   uint32_t nf;
-  uint32_t aligned8;  // frame rows and blocks are 8-byte aligned (32-bit frames)
+  uint32_t aligned8;  // pixel pairs are aligned to their size (8 bytes, 4 for 16-bit frames)
   void* part;
   uint32_t flip = 0;  // frames hold the detector's rows bottom-up (row r is frame row dim1-1-r)
   // Double-buffered kernel only: npass passes over the same table in one
