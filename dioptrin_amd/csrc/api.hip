@@ -448,7 +448,7 @@ static void run_masked(dio_plan* p, const void* frames, int dtype, uint64_t n_fr
   // Frames per lane of the masked-row kernel: 2 with the variance sums (4
   // needs more registers than the kernel holds), else the plan's.
   const uint32_t lanes = var ? 2 : p->tile == 16 ? p->lanes : 4;
-  if (masked_rows_supported(p->tt, dtype, p->desc.precision, lanes)) {
+  if (masked_rows_supported(p->tt, frames, dtype, p->desc.precision, lanes)) {
     // The masks folded into the tile kernel's staging: 8 frames and their
     // masked normalization per 16-row pass, no masked copies in HBM.
     const uint32_t FP = dio::masked_rows_frames_per_pass();

@@ -899,11 +899,11 @@ __device__ __forceinline__ void db_store(const TileArgs& a, const DbFrames<F>& f
 // 8 h + 7 (the last, nfr of them) and their masks' bit byte plane h.
 constexpr uint32_t kMaskedFrames = 8;
 
-template <int F>
+template <int F, int A>
 __device__ __forceinline__ TileArgs db_pass_masked(const TileArgs& a, uint32_t h) {
   TileArgs r = a;
   r.frames = static_cast<const unsigned char*>(a.frames) +
-             (uint64_t)h * kMaskedFrames * a.npix * 4u;
+             (uint64_t)h * kMaskedFrames * a.npix * db_esz<A>();
   r.mbits = a.mbits + (uint64_t)h * a.mbits_stride;
   r.nfr = h + 1 == a.npass ? a.nfr : kMaskedFrames;
   r.nf = (uint32_t)F;
@@ -915,10 +915,13 @@ __device__ __forceinline__ TileArgs db_pass_masked(const TileArgs& a, uint32_t h
 // last one), threads 512-1023 the pair's float32 normalization (detector
 // order; the same words for all 8 slots, so both halves run the same loads
 // and fill the same registers); both the pair's mask bits. Unconditional
-// loads from valid addresses, as in db_load.
+// loads from valid addresses, as in db_load. 16-bit frames (A = 2, rows of
+// even length, pairs 4-byte aligned): each load is one 4-byte word, the
+// frame pair packed, or normalization pixel i & 1 of the pair.
 template <int F, int A>
 __device__ __forceinline__ void db_load_masked(const TileArgs& a, uint32_t t, DbFrames<F>& fr) {
-  static_assert(A == 4 || A == 8, "pair loads are 4 or 8 bytes");
+  static_assert(A == 2 || A == 4 || A == 8, "pair loads of 2, 4 or 8 bytes");
+  constexpr uint32_t E = db_esz<A>();
   static_assert(F == 2 * kMaskedFrames, "8 frames and their normalization");
   const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
   const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
@@ -929,7 +932,7 @@ __device__ __forceinline__ void db_load_masked(const TileArgs& a, uint32_t t, Db
   const uint32_t o0 = nvalid > 0 ? off : 0u, o1 = nvalid > 1 ? off + 1 : 0u;
   const uint32_t d0 = nvalid > 0 ? r * a.dim2 + c : 0u, d1 = nvalid > 1 ? d0 + 1 : 0u;
   const uint32_t p0 = half ? d0 : o0, p1 = half ? d1 : o1;
-  const uint64_t step = half ? 0u : (uint64_t)a.npix * 4u;
+  const uint64_t step = half ? 0u : (uint64_t)a.npix * E;
   const unsigned char* src =
       half ? reinterpret_cast<const unsigned char*>(a.norm32)
            : static_cast<const unsigned char*>(a.frames);
@@ -940,9 +943,12 @@ __device__ __forceinline__ void db_load_masked(const TileArgs& a, uint32_t t, Db
       const u32x2 v = load_at<u32x2>(pf, p0 * 4u);
       fr.v[i][0] = v.x;
       fr.v[i][1] = v.y;
-    } else {
+    } else if constexpr (A == 4) {
       fr.v[i][0] = load_at<uint32_t>(pf, p0 * 4u);
       fr.v[i][1] = load_at<uint32_t>(pf, p1 * 4u);
+    } else {
+      fr.v[i][0] = load_at<uint32_t>(pf, half ? (p0 + (i & 1u)) * 4u : p0 * 2u);
+      fr.v[i][1] = 0u;
     }
   }
   // Kept apart until the store: combining them here would wait for these
@@ -952,8 +958,9 @@ __device__ __forceinline__ void db_load_masked(const TileArgs& a, uint32_t t, Db
 }
 
 // Rows 0-7: the frames with masked pixels set to 0; rows 8-15: the
-// normalization with masked pixels set to 0.
-template <int F, class W, class LT>
+// normalization with masked pixels set to 0. PACKED: 16-bit frame pairs in
+// v[i][0], the normalization pair in v[0][0] and v[1][0].
+template <int F, class W, class LT, bool PACKED = false>
 __device__ __forceinline__ void db_store_masked_as(const DbFrames<F>& fr, LT* fl) {
   const uint32_t half = threadIdx.x / 512, lt = threadIdx.x % 512;
   const uint32_t r0 = lt / kRunLanes, cp = lt % kRunLanes;
@@ -964,12 +971,14 @@ __device__ __forceinline__ void db_store_masked_as(const DbFrames<F>& fr, LT* fl
   if (half == 0) {
 #pragma unroll
     for (uint32_t i = 0; i < kMaskedFrames; i++) {
-      v[i][0] = (m >> i) & 1u ? 0.0f : word_to<W, LT>(fr.v[i][0]);
-      v[i][1] = (m >> (8 + i)) & 1u ? 0.0f : word_to<W, LT>(fr.v[i][1]);
+      const uint32_t w0 = PACKED ? fr.v[i][0] & 0xffffu : fr.v[i][0];
+      const uint32_t w1 = PACKED ? fr.v[i][0] >> 16 : fr.v[i][1];
+      v[i][0] = (m >> i) & 1u ? 0.0f : word_to<W, LT>(w0);
+      v[i][1] = (m >> (8 + i)) & 1u ? 0.0f : word_to<W, LT>(w1);
     }
   } else {
     const float n0 = __uint_as_float(fr.v[0][0]);
-    const float n1 = __uint_as_float(fr.v[0][1]);
+    const float n1 = __uint_as_float(PACKED ? fr.v[1][0] : fr.v[0][1]);
 #pragma unroll
     for (uint32_t i = 0; i < kMaskedFrames; i++) {
       v[i][0] = (m >> i) & 1u ? 0.0f : n0;
@@ -979,10 +988,11 @@ __device__ __forceinline__ void db_store_masked_as(const DbFrames<F>& fr, LT* fl
   pm_store_pair(fl, px, half, v);
 }
 
-template <int F, class LT>
+template <int F, int A, class LT>
 __device__ __forceinline__ void db_store_masked(const TileArgs& a, const DbFrames<F>& fr,
                                                 LT* fl) {
-  if (a.dtype == DIO_U32) db_store_masked_as<F, uint32_t, LT>(fr, fl);
+  if constexpr (A == 2) db_store_masked_as<F, uint16_t, LT, true>(fr, fl);
+  else if (a.dtype == DIO_U32) db_store_masked_as<F, uint32_t, LT>(fr, fl);
   else if (a.dtype == DIO_I32) db_store_masked_as<F, int32_t, LT>(fr, fl);
   else db_store_masked_as<F, float, LT>(fr, fl);
 }
@@ -1008,7 +1018,7 @@ template <int F, int G, int MODE, bool VAR, int FA, bool MASKED = false>
 __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_db(
     TileArgs a) {
   static_assert(FA == 1 || FA == 2 || FA == 4 || FA == 8, "16- and 32-bit frames only");
-  static_assert(!MASKED || FA >= 4, "masked rows need 32-bit frames");
+  static_assert(!MASKED || FA >= 2, "masked rows need pair-aligned 16-bit or 32-bit frames");
   static_assert(F % 2 == 0, "frames split over two thread halves");
   using LT = typename TAcc<MODE>::L;
   using PL = PlanDb<F, MODE>;
@@ -1061,7 +1071,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
   };
   // Arguments of pass h of the launch.
   auto pass_args = [&](uint32_t h) {
-    if constexpr (MASKED) return db_pass_masked<F>(a, h);
+    if constexpr (MASKED) return db_pass_masked<F, FA>(a, h);
     else return db_pass<F, FA>(a, h);
   };
   MetaLanes ml;
@@ -1093,7 +1103,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
                                                            true, slot, fr, tr, sr);
   if (mA.x != mA.y) {
     LT* fb0 = reinterpret_cast<LT*>(frames_buf(0));
-    if constexpr (MASKED) db_store_masked<F, LT>(a, fr, fb0);
+    if constexpr (MASKED) db_store_masked<F, FA, LT>(a, fr, fb0);
     else db_store<F, FA, LT>(a, fr, fb0);
     table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(iA)),
                                      table_buf(iA) + (size_t)C * 8);
@@ -1113,7 +1123,7 @@ __global__ __launch_bounds__(kDbThreads) __attribute__((amdgpu_waves_per_eu(4)))
     //    table buffer, last read by the previous tile).
     if (mB.x != mB.y) {
       LT* fb1 = reinterpret_cast<LT*>(frames_buf(i + 1));
-      if constexpr (MASKED) db_store_masked<F, LT>(a, fr, fb1);
+      if constexpr (MASKED) db_store_masked<F, FA, LT>(a, fr, fb1);
       else db_store<F, FA, LT>(a, fr, fb1);
       if (iB.first)
         table_store<1, MODE, kDbThreads>(tr, pn, reinterpret_cast<u32x2*>(table_buf(iB)),
@@ -1352,7 +1362,8 @@ template <int G, bool VAR>
 void launch_masked(const TiledTable& t, const TileArgs& a, int precision, hipStream_t s) {
   auto go = [&](auto mode) {
     constexpr int M = decltype(mode)::value;
-    if (a.aligned8) launch_tile_db<16, G, M, VAR, 8, true>(t, a, s);
+    if (a.dtype == DIO_U16) launch_tile_db<16, G, M, VAR, 2, true>(t, a, s);
+    else if (a.aligned8) launch_tile_db<16, G, M, VAR, 8, true>(t, a, s);
     else launch_tile_db<16, G, M, VAR, 4, true>(t, a, s);
   };
   switch (precision) {
@@ -1723,9 +1734,12 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
   }
 }
 
-bool masked_rows_supported(const TiledTable& t, int dtype, int precision, uint32_t G) {
-  return dtype != DIO_U16 && double_buffered(t, 16, dtype, precision) &&
-         valid_tile(precision, 16, G);
+bool masked_rows_supported(const TiledTable& t, const void* frames, int dtype, int precision,
+                           uint32_t G) {
+  // 16-bit frames: pixel pairs read as 4-byte words (even rows, 4-byte
+  // aligned frames; passes of 8 frames keep the alignment).
+  if (dtype == DIO_U16 && (t.dim2 % 2 || reinterpret_cast<uintptr_t>(frames) % 4)) return false;
+  return double_buffered(t, 16, dtype, precision) && valid_tile(precision, 16, G);
 }
 
 uint32_t masked_rows_frames_per_pass() { return kMaskedFrames; }
@@ -1734,8 +1748,9 @@ void tiled_gather_masked(const TiledTable& t, TileScratch& sc, const void* frame
                          const uint8_t* mbits, uint64_t mbits_stride, const float* norm32,
                          uint64_t n_frames, bool var, int precision, uint32_t G,
                          double* rows_sum, double* rows_var, hipStream_t s) {
-  if (!masked_rows_supported(t, dtype, precision, G))
-    fail(DIO_ERR_INVALID, "masked rows need 32-bit frames and 16-row passes");
+  if (!masked_rows_supported(t, frames, dtype, precision, G))
+    fail(DIO_ERR_INVALID, "masked rows need 32-bit or pair-aligned 16-bit frames and 16-row passes");
+  const uint32_t esz = (uint32_t)dtype_bytes(dtype);
   constexpr uint32_t F = 16;
   const uint64_t npix = (uint64_t)t.dim1 * t.dim2;
   const char* fb = static_cast<const char*>(frames);
@@ -1750,8 +1765,9 @@ void tiled_gather_masked(const TiledTable& t, TileScratch& sc, const void* frame
     const uint32_t ng = (uint32_t)std::min<uint64_t>((uint64_t)P * kMaskedFrames, n_frames - g0);
     const uint32_t np = (ng + kMaskedFrames - 1) / kMaskedFrames;
     const uint64_t pass0 = g0 / kMaskedFrames;
-    const void* fr = fb + g0 * npix * 4;
-    const uint32_t al8 = (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % 8 == 0) ? 1u : 0u;
+    const void* fr = fb + g0 * npix * esz;
+    const uint32_t al8 =
+        (t.dim2 % 2 == 0 && reinterpret_cast<uintptr_t>(fr) % (2 * esz) == 0) ? 1u : 0u;
     TileArgs a{fr,     dtype,   t.lidx.p, t.w32.p, t.w64.p, t.seg_group.p,
                t.tile_meta.p, t.nseg, t.ntx, t.ntiles, t.dim1, t.dim2,
                npix,   F,       al8,      sc.ptr()};

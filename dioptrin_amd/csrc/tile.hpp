@@ -131,9 +131,11 @@ void tiled_gather(const TiledTable& t, TileScratch& sc, const void* frames, int 
 // frame i of the pass masked), input order, passes mbits_stride bytes apart.
 // rows_sum / rows_var receive, per pass, the 16 rows' raw sums (sum w x and
 // sum w^2 max(x, 0), the latter only with var; the normalization rows' sums
-// are the frames' denominators) as [pass][16][nbins]. Needs 32-bit frames and
-// float32 weights (masked_rows_supported).
-bool masked_rows_supported(const TiledTable& t, int dtype, int precision, uint32_t lanes);
+// are the frames' denominators) as [pass][16][nbins]. Needs 32-bit frames, or
+// 16-bit ones with even rows starting 4-byte aligned, and float32 weights
+// (masked_rows_supported).
+bool masked_rows_supported(const TiledTable& t, const void* frames, int dtype, int precision,
+                           uint32_t lanes);
 uint32_t masked_rows_frames_per_pass();
 void tiled_gather_masked(const TiledTable& t, TileScratch& sc, const void* frames, int dtype,
                          const uint8_t* mbits, uint64_t mbits_stride, const float* norm32,

@@ -72,3 +72,39 @@ def test_u16_frames_equal_u32(name, precision):
         ri, rs = run(plan, d32, n, True)
         gi, gs = run(plan, misaligned_u16(h), n, True)
         assert np.array_equal(gi, ri) and np.array_equal(gs, rs)
+
+
+@pytest.mark.parametrize("precision", ["f32_wide", "f32_compensated"])
+def test_u16_masked_rows_equal_u32(precision):
+    """Per-frame masks on 16-bit frames with even rows take the fused
+    masked-row kernel (4-byte pair loads): the bits of the same values as
+    uint32 frames, whose route is checked against the oracle elsewhere; a
+    2-byte aligned start takes the other route (masked copies, then two
+    gathers), equal within the gather tolerance."""
+    name = "eiger4m_ss2d"
+    w = synth.WORKLOADS[name]
+    print(f"case u16 masked {precision}", flush=True)
+    plan = GpuPlan(synth.workload_poni(name), w["shape"], n_radial=1000, precision=precision)
+    h, d16, d32 = frames_of(name)
+    n = 37
+    rng = np.random.default_rng(8)
+    masks = DeviceArray.from_numpy((rng.random((n,) + w["shape"]) < 0.05).astype(np.uint8))
+    for errors in (False, True):
+        out = []
+        for fr in (d32[:n], d16[:n]):
+            oi = DeviceArray((n, plan.nbins), np.float64)
+            os_ = DeviceArray((n, plan.nbins), np.float64) if errors else None
+            plan.integrate_masked_device(fr, masks, oi, os_, errors=errors)
+            synchronize()
+            out.append((oi.numpy(), os_.numpy() if errors else None))
+        assert np.array_equal(out[0][0], out[1][0])
+        if errors:
+            assert np.array_equal(out[0][1], out[1][1])
+    mis = misaligned_u16(h[:n])
+    oi = DeviceArray((n, plan.nbins), np.float64)
+    plan.integrate_masked_device(mis, masks, oi)
+    synchronize()
+    ref = DeviceArray((n, plan.nbins), np.float64)
+    plan.integrate_masked_device(d32[:n], masks, ref)
+    synchronize()
+    assert np.allclose(oi.numpy(), ref.numpy(), rtol=2e-6, atol=0)
