@@ -107,6 +107,26 @@ def azimuthal_std_rate(plan, frames_dev, n):
     return round(best, 1)
 
 
+def uint16_rate(plan, shape, n):
+    """frames/s of plan.integrate_device on n uint16 frames (random values,
+    made on the host and copied to the device once), eager calls of the
+    whole batch, best of three."""
+    from dioptrin_amd.device import DeviceArray, synchronize
+    h = np.random.default_rng(16).integers(0, 30000, size=(n,) + tuple(shape), dtype=np.uint16)
+    fr = DeviceArray.from_numpy(h)
+    del h
+    out = DeviceArray((n, plan.nbins), np.float64)
+    plan.integrate_device(fr, out)
+    synchronize()
+    best = 0.0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        plan.integrate_device(fr, out)
+        synchronize()
+        best = max(best, n / (time.perf_counter() - t0))
+    return round(best, 1)
+
+
 def host_cpu():
     """nproc, the model name from /proc/cpuinfo and the cores this job may
     use (OMP_NUM_THREADS when set, as the GPU host sets it to the job's share
@@ -445,6 +465,12 @@ def main():
     log(f"[bench] host frames: {host_fps}")
     dyn_fps = dynamic_mask_rate(plan, frames, min(nfr, 64), args.errors)
     log(f"[bench] per-frame masks: {dyn_fps} frames/s")
+    try:
+        u16_fps = uint16_rate(plan, w["shape"], min(nfr, 64))
+    except Exception as e:  # the 16-bit leg is a side measurement
+        log(f"[bench] uint16 frames: not measured ({e})")
+        u16_fps = None
+    log(f"[bench] uint16 frames: {u16_fps} frames/s")
     std_fps = azimuthal_std_rate(plan, frames, min(nfr, 64))
     std_bpf = std_bytes_per_frame(npix, info["nnz_masked"], bins, 8)
     log(f"[bench] azimuthal std: {std_fps} frames/s")
@@ -514,6 +540,7 @@ def main():
         "per_frame_fps": round(per_frame_fps, 1),
         "host_frames_fps_pcie": host_fps,
         "dynamic_masks_fps": dyn_fps,
+        "uint16_frames_fps": u16_fps,
         "azimuthal_std": {"fps": std_fps, "outputs": "intensity, sigma, std",
                           "frames_per_pass": 8,
                           "bytes_per_frame_algorithmic": int(std_bpf),
